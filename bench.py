@@ -1,0 +1,221 @@
+"""Benchmark: FP64 DOF-RHS evals/s of the FR residual on one MI355X.
+
+Workload (one "step" = one SSP-RK3 time step = 3 fused residual+stage
+launches): 2D isentropic vortex, periodic [0,10]^2 (the reference's box with
+the vortex at its centre), 128x128 affine quads, p=4 (25 nodes/element, 4
+conserved variables, 1,638,400 DOFs), HLLC, ideal gas gamma=1.4, beta=5,
+stream (1,1,1,1), CFL 0.1.  Steps are replayed from a captured hipGraph.
+
+    python bench.py [--gpus 1] [--steps 1000] [--warmup 50]
+
+Prints ONE JSON line.  Extra fields: tangent vs complex-step J*v kernel time
+(same GPU), and the residual at the large curved configuration (1024^2 warped
+quads, p=7, van der Waals, HLLC) where the state (2.15 GB) is far larger than
+the caches.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP64 DOF-RHS evals/s + Newton step time (tangent vs complex-step), 1 GPU"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-large", action="store_true", help="skip the large curved residual")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+def vortex_cfg(nx=128, ny=128, p=4):
+    return dict(dims=2, nx=nx, ny=ny, p=p, lx=10.0, ly=10.0, riemann=1, bc_type=[5, 5, 5, 5])
+
+
+VORTEX_PRM = [1.0, 1.0, 5.0, 1.0, 10.0]
+
+
+def flops_per_element(n):
+    # residual, HLLC / ideal gas: 32 n^3 + 89 n^2 + 424 n
+    return 32 * n ** 3 + 89 * n ** 2 + 424 * n
+
+
+def time_kernel(torch, fn, reps):
+    s = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fn()
+    torch.cuda.synchronize()
+    a.record(s)
+    for _ in range(reps):
+        fn()
+    b.record(s)
+    b.synchronize()
+    return a.elapsed_time(b) / 1e3 / reps
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; use --gpus 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    import fr_sdg_amd as fr
+
+    torch.cuda.set_device(0)
+    cfg = vortex_cfg()
+    ev = fr.PhysicsEvaluator(cfg)
+    n = cfg["p"] + 1
+    E = cfg["nx"] * cfg["ny"]
+    ndof = ev.ndofs
+    u0 = ev.impose_ic(fr.IC_VORTEX, VORTEX_PRM)
+    # dt = CFL * h / max(|u| + c) at t = 0
+    st = u0.view(-1, 4)
+    rho = st[:, 0]
+    uu = st[:, 1] / rho
+    vv = st[:, 2] / rho
+    pres = 0.4 * (st[:, 3] - 0.5 * rho * (uu * uu + vv * vv))
+    c = torch.sqrt(1.4 * pres / rho)
+    smax = float(torch.max(torch.sqrt(uu * uu + vv * vv) + c))
+    h = cfg["lx"] / cfg["nx"]
+    dt = 0.1 * h / smax
+
+    u = u0.clone()
+    ev.ssprk_run(u, dt, args.warmup)
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_wall0 = time.perf_counter()
+    e0.record(s)
+    ev.ssprk_run(u, dt, args.steps)
+    e1.record(s)
+    e1.synchronize()
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall0
+    elapsed = e0.elapsed_time(e1) / 1e3
+    if not bool(torch.isfinite(u).all()):
+        raise RuntimeError("non-finite state after the timed steps")
+    n_res = 3 * args.steps
+    value = ndof * n_res / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    # dominant kernel: fused residual + SSP-RK stage, 3 per step.
+    # algorithmic HBM bytes per step: stage 1 reads u, writes w1 (16 B/DOF);
+    # stages 2,3 read u_s and u^n, write (24 B/DOF each) -> 64 B/DOF per step.
+    bytes_launch = 64.0 * ndof / 3.0
+    t_launch = elapsed / n_res
+    achieved = bytes_launch / t_launch / 1e9
+    flops_launch = flops_per_element(n) * E
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_vortex_p4_128.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    # tangent vs complex-step J*v kernels (same GPU, same state)
+    v = torch.randn(ndof, dtype=torch.float64, device="cuda")
+    out = ev.empty()
+    t_tan = time_kernel(torch, lambda: ev.jvp(u, v, 1.0, out=out), 50)
+    t_cs = time_kernel(torch, lambda: ev.jvp_cstep(u, v, 1.0, 1e-7, out=out), 50)
+    R = ev.empty()
+    t_res = time_kernel(torch, lambda: ev.residual_kernel(u, out=R), 100)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF-RHS evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "2D isentropic vortex 128x128 p=4 HLLC ideal, SSP-RK3, CFL 0.1",
+                   "ndofs": ndof, "elements": E, "p": cfg["p"], "dt": dt,
+                   "residual_evals_per_step": 3, "graph_captured": True},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_res2d<5,IOStage> (fused residual + SSP-RK stage)",
+                     "bytes_per_launch_alg": bytes_launch, "avg_launch_s": t_launch,
+                     "fp64_tflops": flops_launch / t_launch / 1e12,
+                     "fp64_frac": flops_launch / t_launch / 1e12 / FP64_PEAK_TFS},
+        "residual_only": {"avg_s": t_res, "dof_rhs_per_s": ndof / t_res,
+                          "gbs_alg": 16.0 * ndof / t_res / 1e9},
+        "jvp": {"tangent_s": t_tan, "complex_step_s": t_cs, "speedup": t_cs / t_tan},
+        "wall_s_timed": t_wall,
+    }
+
+    if not args.no_large:
+        try:
+            line["large_curved"] = bench_large(torch, fr)
+        except Exception as exc:  # report, keep the headline line valid
+            line["large_curved"] = {"error": repr(exc)}
+
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(line))
+
+
+def bench_large(torch, fr):
+    """Residual at 1024^2 warped quads, p=7, VdW, HLLC (2.15 GB per state)."""
+    cfg = dict(dims=2, nx=1024, ny=1024, p=7, lx=2.0, ly=2.0, riemann=1, geometry=fr.GEOM_WARPED,
+               geom_param=[0.05, -1.0, -1.0, 0.0], bc_type=[1, 1, 1, 1], gas_kind=1, vdw_a=0.1,
+               vdw_b=0.02)
+    ev = fr.PhysicsEvaluator(cfg)
+    u = ev.impose_ic(fr.IC_EXPLOSION, [1.0, 1.0, 0.125, 0.1, 0.4, 0.0, 0.0])
+    u.add_(1e-3 * torch.sin(torch.arange(u.numel(), device="cuda", dtype=torch.float64)) * u)
+    R = ev.empty()
+    t = time_kernel(torch, lambda: ev.residual_kernel(u, out=R), 20)
+    n = 8
+    ndof = ev.ndofs
+    E = cfg["nx"] * cfg["ny"]
+    bytes_alg = (16.0 + 8.0 + 12.0 / n) * ndof  # u, R, volume metrics, face (nx,ny,scale)
+    ok = bool(torch.isfinite(R).all())
+    del ev, u, R
+    torch.cuda.empty_cache()
+    return {"workload": "warped 1024x1024 p=7 VdW HLLC residual", "ndofs": ndof, "avg_s": t,
+            "dof_rhs_per_s": ndof / t, "gbs_alg": bytes_alg / t / 1e9,
+            "hbm_frac": bytes_alg / t / 1e9 / HBM_PEAK_GBS,
+            "fp64_tflops": (32 * n ** 3 + 89 * n ** 2 + 424 * n) * E / t / 1e12, "finite": ok}
+
+
+def cpu_baseline(seconds):
+    """The CPU oracle (a restatement of the reference loops, 1 thread) on the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    from pyoracle import Oracle
+
+    cfg = vortex_cfg()
+    o = Oracle(cfg)
+    u = o.impose_ic(1, VORTEX_PRM)
+    o.residual(u)
+    reps = 0
+    t0 = time.perf_counter()
+    while True:
+        o.residual(u)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {"value": o.ndofs * reps / el, "unit": "DOF-RHS evals/s", "cores": 1, "kind": "port",
+            "sample": f"{reps} residual evaluations of the full 128x128 p=4 vortex state "
+                      f"({o.ndofs} DOFs) by the oracle, {el:.1f} s on 1 host thread"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,286 @@
+"""fr_sdg_amd — MI355X-native FR residual (FP64) behind the reference's callback API.
+
+The compute path is libfrsdg.so (hand-written HIP kernels for gfx950, C-ABI
+in include/frsdg.h).  This module is a thin ctypes binding over that ABI with
+torch tensors as device-memory plumbing:
+
+* ``PhysicsEvaluator`` mirrors the reference's evaluator / PETSc callback set
+  (``residual_kernel``, ``rhs_func``, ``fun_implicit``, ``pre_jv_eval`` +
+  ``matmult``, ``limit_solution`` / ``apply_stabilization``), with the same
+  sign conventions (R = +div F, rhs = -R, F = U_t + R; petsc_solver_impl.hpp:
+  626-670) and the reference's natural DMDA layout;
+* ``FRResidual`` is a ``torch.autograd.Function``: forward = residual kernel,
+  backward = adjoint (J^T w) kernel.
+
+There is no CPU fallback: without a GPU or without libfrsdg.so every call
+raises.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (must load torch's HIP runtime before libfrsdg.so)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfrsdg.so")
+
+# boundary types (boundary_conditions.hpp:8-16) + the curvilinear slip wall
+BC_WALL, BC_OUTFLOW, BC_DIRICHLET, BC_PERIODIC, BC_SLIPWALL = 0, 1, 2, 5, 6
+# directions (boundary_conditions.hpp:17-23)
+LEFT, RIGHT, TOP, BOTTOM = 0, 1, 2, 3
+GEOM_CARTESIAN, GEOM_DELAVAL, GEOM_WEDGE, GEOM_BUMP, GEOM_WARPED = 0, 1, 2, 3, 4
+RUSANOV, HLLC = 0, 1
+IDEAL, VAN_DER_WAALS = 0, 1
+IC_UNIFORM, IC_VORTEX, IC_EXPLOSION, IC_SOD = 0, 1, 2, 3
+
+
+class FrsdgError(RuntimeError):
+    pass
+
+
+class FrsdgDesc(ctypes.Structure):
+    _fields_ = [
+        ("dims", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("p", ctypes.c_int),
+        ("lx", ctypes.c_double), ("ly", ctypes.c_double),
+        ("gas_kind", ctypes.c_int), ("gamma", ctypes.c_double), ("R_spec", ctypes.c_double),
+        ("vdw_a", ctypes.c_double), ("vdw_b", ctypes.c_double),
+        ("riemann", ctypes.c_int), ("geometry", ctypes.c_int),
+        ("geom_param", ctypes.c_double * 4),
+        ("bc_type", ctypes.c_int * 4), ("bc_state", (ctypes.c_double * 4) * 4),
+        ("c_vcjh", ctypes.c_double),
+        ("apply_filter", ctypes.c_int), ("filt_cutoff", ctypes.c_int),
+        ("filt_strength", ctypes.c_int), ("filt_alpha", ctypes.c_double),
+        ("jac_type", ctypes.c_int), ("tangent_bc_mode", ctypes.c_int),
+    ]
+
+
+EXPORTS = [
+    "frsdg_create", "frsdg_destroy", "frsdg_ndofs", "frsdg_get_tables", "frsdg_residual",
+    "frsdg_rhs", "frsdg_ifunction", "frsdg_jvp", "frsdg_jvp_cstep", "frsdg_vjp",
+    "frsdg_pre_jv_eval", "frsdg_matmult", "frsdg_filter", "frsdg_apply_stabilization",
+    "frsdg_ssprk_step", "frsdg_ssprk_run", "frsdg_impose_ic", "frsdg_l2_error", "frsdg_strerror",
+]
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libfrsdg.so (built in-tree by ``fr_sdg_amd.build``); raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FrsdgError(f"libfrsdg.so not built ({path}); run __graft_entry__.build()")
+    L = ctypes.CDLL(path)
+    vp, dp, d, i = ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+    L.frsdg_create.argtypes = [ctypes.POINTER(FrsdgDesc), ctypes.POINTER(ctypes.c_void_p)]
+    L.frsdg_destroy.argtypes = [vp]
+    L.frsdg_ndofs.argtypes = [vp]
+    L.frsdg_ndofs.restype = ctypes.c_longlong
+    L.frsdg_get_tables.argtypes = [vp] + [ctypes.POINTER(ctypes.c_double)] * 6
+    L.frsdg_residual.argtypes = [vp, d, dp, dp, vp]
+    L.frsdg_rhs.argtypes = [vp, d, dp, dp, vp]
+    L.frsdg_ifunction.argtypes = [vp, d, dp, dp, dp, vp]
+    L.frsdg_jvp.argtypes = [vp, d, dp, dp, d, dp, vp]
+    L.frsdg_jvp_cstep.argtypes = [vp, d, dp, dp, d, d, dp, vp]
+    L.frsdg_vjp.argtypes = [vp, d, dp, dp, d, dp, vp]
+    L.frsdg_pre_jv_eval.argtypes = [vp, d, dp, d]
+    L.frsdg_matmult.argtypes = [vp, dp, dp, vp]
+    L.frsdg_filter.argtypes = [vp, dp, dp, vp]
+    L.frsdg_apply_stabilization.argtypes = [vp, dp, vp]
+    L.frsdg_ssprk_step.argtypes = [vp, d, d, dp, i, vp]
+    L.frsdg_ssprk_run.argtypes = [vp, d, d, dp, i, i, vp]
+    L.frsdg_impose_ic.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double), dp, vp]
+    L.frsdg_l2_error.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double), d, dp,
+                                 ctypes.POINTER(ctypes.c_double), vp]
+    L.frsdg_strerror.argtypes = [i]
+    L.frsdg_strerror.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        msg = _lib.frsdg_strerror(rc).decode() if _lib is not None else str(rc)
+        raise FrsdgError(f"libfrsdg error {rc}: {msg}")
+
+
+def make_desc(cfg):
+    """Descriptor from a plain config dict (keys = frsdg_desc field names)."""
+    d = FrsdgDesc()
+    d.dims = cfg.get("dims", 2)
+    d.nx = cfg["nx"]
+    d.ny = cfg.get("ny", 1)
+    d.p = cfg["p"]
+    d.lx = cfg.get("lx", 1.0)
+    d.ly = cfg.get("ly", 1.0)
+    d.gas_kind = cfg.get("gas_kind", IDEAL)
+    d.gamma = cfg.get("gamma", 1.4)
+    d.R_spec = cfg.get("R_spec", 287.05)
+    d.vdw_a = cfg.get("vdw_a", 0.0)
+    d.vdw_b = cfg.get("vdw_b", 0.0)
+    d.riemann = cfg.get("riemann", HLLC)
+    d.geometry = cfg.get("geometry", GEOM_CARTESIAN)
+    for k, v in enumerate(cfg.get("geom_param", [0.0] * 4)):
+        d.geom_param[k] = v
+    for k, v in enumerate(cfg.get("bc_type", [BC_OUTFLOW] * 4)):
+        d.bc_type[k] = v
+    for k, st in enumerate(cfg.get("bc_state", [[0.0] * 4] * 4)):
+        for q, v in enumerate(st):
+            d.bc_state[k][q] = v
+    d.c_vcjh = cfg.get("c_vcjh", 0.0)
+    d.apply_filter = cfg.get("apply_filter", 0)
+    d.filt_cutoff = cfg.get("filt_cutoff", 2)
+    d.filt_strength = cfg.get("filt_strength", 32)
+    d.filt_alpha = cfg.get("filt_alpha", 36.0)
+    d.jac_type = cfg.get("jac_type", 0)
+    d.tangent_bc_mode = cfg.get("tangent_bc_mode", 0)
+    return d
+
+
+def _ptr(t):
+    if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+        raise FrsdgError("expected a contiguous float64 CUDA tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class PhysicsEvaluator:
+    """Device-resident FR evaluator for one problem (physics_evaluator.hpp:158-318)."""
+
+    def __init__(self, cfg, device="cuda"):
+        if not torch.cuda.is_available():
+            raise FrsdgError("fr_sdg_amd needs a ROCm GPU (no CPU fallback)")
+        self.lib = load_library()
+        self.cfg = dict(cfg)
+        self.device = torch.device(device)
+        torch.cuda.init()
+        self._desc = make_desc(cfg)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _check(self.lib.frsdg_create(ctypes.byref(self._desc), ctypes.byref(h)))
+        self.h = h
+        self.n = cfg["p"] + 1
+        self.ndofs = int(self.lib.frsdg_ndofs(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.frsdg_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    # -- helpers --
+    def empty(self):
+        return torch.empty(self.ndofs, dtype=torch.float64, device=self.device)
+
+    def tables(self):
+        import numpy as np
+        n = self.n
+        arrs = [np.zeros(n), np.zeros(n), np.zeros(n * n), np.zeros(n), np.zeros(n), np.zeros(n * n)]
+        _check(self.lib.frsdg_get_tables(self.h, *[a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+                                                   for a in arrs]))
+        csi, w, D, dL, dR, F = arrs
+        return dict(csi=csi, w=w, D=D.reshape(n, n), dL=dL, dR=dR, filt=F.reshape(n, n))
+
+    # -- residual callbacks --
+    def residual_kernel(self, u, t=0.0, out=None):
+        """R = +div F (physics_impl_2D.hpp:644-902)."""
+        out = self.empty() if out is None else out
+        _check(self.lib.frsdg_residual(self.h, t, _ptr(u), _ptr(out), _stream()))
+        return out
+
+    def rhs_func(self, u, t=0.0, out=None):
+        """rhs = -R(u) (petsc_solver_impl.hpp:657-670)."""
+        out = self.empty() if out is None else out
+        _check(self.lib.frsdg_rhs(self.h, t, _ptr(u), _ptr(out), _stream()))
+        return out
+
+    def fun_implicit(self, u, udot, t=0.0, out=None):
+        """F = U_t + R(u) (petsc_solver_impl.hpp:626-655)."""
+        out = self.empty() if out is None else out
+        _check(self.lib.frsdg_ifunction(self.h, t, _ptr(u), _ptr(udot), _ptr(out), _stream()))
+        return out
+
+    # -- Jacobian products --
+    def jvp(self, u, v, sigma=0.0, t=0.0, out=None):
+        out = self.empty() if out is None else out
+        _check(self.lib.frsdg_jvp(self.h, t, _ptr(u), _ptr(v), sigma, _ptr(out), _stream()))
+        return out
+
+    def jvp_cstep(self, u, v, sigma=0.0, h=1e-7, t=0.0, out=None):
+        out = self.empty() if out is None else out
+        _check(self.lib.frsdg_jvp_cstep(self.h, t, _ptr(u), _ptr(v), sigma, h, _ptr(out), _stream()))
+        return out
+
+    def vjp(self, u, w, sigma=0.0, t=0.0, out=None):
+        out = self.empty() if out is None else out
+        _check(self.lib.frsdg_vjp(self.h, t, _ptr(u), _ptr(w), sigma, _ptr(out), _stream()))
+        return out
+
+    def pre_jv_eval(self, u, sigma, t=0.0):
+        """Snapshot of the linearisation point (petsc_solver_impl.hpp:223-248)."""
+        self._jv_u = u  # keep the tensor alive: the library holds its pointer
+        _check(self.lib.frsdg_pre_jv_eval(self.h, t, _ptr(u), sigma))
+
+    def matmult(self, v, out=None):
+        """MatShell MATOP_MULT (petsc_solver_impl.hpp:250-267)."""
+        out = self.empty() if out is None else out
+        _check(self.lib.frsdg_matmult(self.h, _ptr(v), _ptr(out), _stream()))
+        return out
+
+    # -- stabilisation / stepping --
+    def limit_solution(self, u, out=None):
+        out = self.empty() if out is None else out
+        _check(self.lib.frsdg_filter(self.h, _ptr(u), _ptr(out), _stream()))
+        return out
+
+    def apply_stabilization(self, u):
+        _check(self.lib.frsdg_apply_stabilization(self.h, _ptr(u), _stream()))
+        return u
+
+    def ssprk_step(self, u, dt, t=0.0, stages=3):
+        _check(self.lib.frsdg_ssprk_step(self.h, t, dt, _ptr(u), stages, _stream()))
+        return u
+
+    def ssprk_run(self, u, dt, nsteps, t=0.0, stages=3):
+        _check(self.lib.frsdg_ssprk_run(self.h, t, dt, _ptr(u), stages, nsteps, _stream()))
+        return u
+
+    def impose_ic(self, kind, prm, out=None):
+        out = self.empty() if out is None else out
+        prm = (ctypes.c_double * 8)(*(list(prm) + [0.0] * (8 - len(prm))))
+        _check(self.lib.frsdg_impose_ic(self.h, kind, prm, _ptr(out), _stream()))
+        return out
+
+    def compute_error(self, kind, prm, u, t=0.0):
+        prm = (ctypes.c_double * 8)(*(list(prm) + [0.0] * (8 - len(prm))))
+        out = (ctypes.c_double * 4)()
+        _check(self.lib.frsdg_l2_error(self.h, kind, prm, t, _ptr(u), out, _stream()))
+        return list(out)
+
+
+class FRResidual(torch.autograd.Function):
+    """R(u) with the adjoint kernel as backward: grad_u = J(u)^T grad_R."""
+
+    @staticmethod
+    def forward(ctx, u, ev, t=0.0):
+        ctx.ev, ctx.t = ev, t
+        ctx.save_for_backward(u)
+        return ev.residual_kernel(u.detach().contiguous(), t)
+
+    @staticmethod
+    def backward(ctx, g):
+        (u,) = ctx.saved_tensors
+        return ctx.ev.vjp(u.detach().contiguous(), g.contiguous(), 0.0, ctx.t), None, None
+
+    @staticmethod
+    def jvp(ctx, du, _ev=None, _t=None):
+        (u,) = ctx.saved_tensors
+        return ctx.ev.jvp(u.detach().contiguous(), du.contiguous(), 0.0, ctx.t)
+
+
+def fr_residual(u, ev, t=0.0):
+    return FRResidual.apply(u, ev, t)

@@ -1,0 +1,766 @@
+// frsdg_capi.hip — context, setup kernels and the extern "C" entry points of
+// libfrsdg.so (declared in include/frsdg.h).
+#include "../../include/frsdg.h"
+#include "frsdg_kernels.hpp"
+#include "frsdg_tables.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <vector>
+
+using namespace frsdg;
+
+struct frsdg_ctx
+{
+    frsdg_desc d;
+    int        n   = 0;
+    long long  E   = 0;
+    long long  nd  = 0; // number of DOFs
+    Tables     tb;
+    double*    d_tab  = nullptr; // D, dL, dR, filt
+    double*    d_vmet = nullptr;
+    double*    d_fxm  = nullptr;
+    double*    d_fym  = nullptr;
+    double*    d_csi  = nullptr;
+    Res2DP     p2{};
+    Res1DP     p1{};
+    double*    w1 = nullptr; // RK work states
+    double*    w2 = nullptr;
+    double*    red = nullptr; // reduction partials
+    // pre_jv_eval snapshot
+    const double* jv_u     = nullptr;
+    double        jv_sigma = 0.0;
+    double        jv_t     = 0.0;
+    // graph cache for ssprk_run
+    hipGraphExec_t gexec  = nullptr;
+    hipGraph_t     graph  = nullptr;
+    double*        g_u    = nullptr;
+    double         g_dt   = 0.0;
+    int            g_st   = 0;
+    int            g_n    = 0;
+};
+
+namespace
+{
+    inline int hip_rc(hipError_t e) { return e == hipSuccess ? FRSDG_OK : FRSDG_ERR_HIP; }
+#define FR_TRY(x)                                                                                  \
+    do                                                                                             \
+    {                                                                                              \
+        hipError_t _e = (x);                                                                       \
+        if (_e != hipSuccess) return FRSDG_ERR_HIP;                                                \
+    } while (0)
+
+    inline bool aligned(const void* p) { return (reinterpret_cast<std::uintptr_t>(p) & 15u) == 0; }
+
+    // ---------------- setup kernels ----------------
+    // per-node composite metrics (physics_evaluator.hpp:235-313)
+    __global__ void k_geom_vol(GeomP g, int nx, int ny, int n, const double* __restrict__ csi,
+                               long long E, double* __restrict__ vmet)
+    {
+        const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+        const long long NN = static_cast<long long>(n) * n;
+        if (t >= E * NN) return;
+        const long long e = t / NN;
+        const int node = static_cast<int>(t - e * NN), k = node % n, l = node / n;
+        const int i = static_cast<int>(e % nx), j = static_cast<int>(e / nx);
+        const double hx = 1.0 / static_cast<double>(nx), hy = 1.0 / static_cast<double>(ny);
+        const double cx = hx * (static_cast<double>(i) + 0.5 * (1.0 + csi[k]));
+        const double cy = hy * (static_cast<double>(j) + 0.5 * (1.0 + csi[l]));
+        const VolM m    = composite_metrics(g, nx, ny, cx, cy);
+        vmet[t * 4 + 0] = m.xi_x;
+        vmet[t * 4 + 1] = m.xi_y;
+        vmet[t * 4 + 2] = m.eta_x;
+        vmet[t * 4 + 3] = m.eta_y;
+    }
+    // face normal (unit contravariant direction) and ||grad xi|| / ||grad eta||
+    // at face points (physics_impl_2D.hpp:697-719, 805-819)
+    __global__ void k_geom_face(GeomP g, int nx, int ny, int n, const double* __restrict__ csi,
+                                int xdir, double* __restrict__ fm)
+    {
+        const long long t  = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+        const long long nf = xdir ? static_cast<long long>(ny) * (nx + 1) : static_cast<long long>(ny + 1) * nx;
+        if (t >= nf * n) return;
+        const long long f = t / n;
+        const int idx     = static_cast<int>(t - f * n);
+        const double hx = 1.0 / static_cast<double>(nx), hy = 1.0 / static_cast<double>(ny);
+        double cx, cy;
+        if (xdir)
+        {
+            const int I = static_cast<int>(f % (nx + 1)), j = static_cast<int>(f / (nx + 1));
+            cx = hx * (static_cast<double>(I) + 0.5 * (1.0 + csi[0]));
+            cy = hy * (static_cast<double>(j) + 0.5 * (1.0 + csi[idx]));
+        }
+        else
+        {
+            const int i = static_cast<int>(f % nx), J = static_cast<int>(f / nx);
+            cx = hx * (static_cast<double>(i) + 0.5 * (1.0 + csi[idx]));
+            cy = hy * (static_cast<double>(J) + 0.5 * (1.0 + csi[0]));
+        }
+        const VolM m = composite_metrics(g, nx, ny, cx, cy);
+        const Jac2 J = geom_jac(g, cx, cy);
+        double a, b, sc;
+        if (xdir)
+        {
+            a  = J.y_e;
+            b  = 0.0 - J.x_e;
+            sc = ::sqrt(m.xi_x * m.xi_x + m.xi_y * m.xi_y);
+        }
+        else
+        {
+            a  = 0.0 - J.y_c;
+            b  = J.x_c;
+            sc = ::sqrt(m.eta_x * m.eta_x + m.eta_y * m.eta_y);
+        }
+        const double inv = 1.0 / ::sqrt(a * a + b * b);
+        fm[t * 4 + 0] = a * inv;
+        fm[t * 4 + 1] = b * inv;
+        fm[t * 4 + 2] = sc;
+        fm[t * 4 + 3] = 0.0;
+    }
+
+    // ---------------- initial conditions / exact solutions ----------------
+    // factory.hpp:311-352 (isentropic vortex, gamma hard-wired 1.4),
+    // factory.hpp:423-454 (circular explosion), uniform state.
+    struct ICP
+    {
+        int    kind;
+        double prm[8];
+        double t;
+    };
+    __device__ double gas_rhoE(const GasP& g, double rho, double vel, double pres)
+    {
+        if (g.kind == 0) return pres / (g.gamma - 1.0) + 0.5 * rho * vel * vel; // gas_models.hpp:81-86
+        const double cov = 1.0 - g.b * rho, arr = g.a * rho * rho;            // :144-157
+        return ((pres + arr) * cov) / (g.gamma - 1.0) - arr + 0.5 * rho * vel * vel;
+    }
+    __device__ void ic_state(const GasP& g, const ICP& c, double x, double y, double st[4])
+    {
+        if (c.kind == 0)
+        {
+            const double rho = c.prm[0], u = c.prm[1], v = c.prm[2], p = c.prm[3];
+            st[0] = rho;
+            st[1] = rho * u;
+            st[2] = rho * v;
+            st[3] = gas_rhoE(g, rho, ::sqrt(u * u + v * v), p);
+            return;
+        }
+        if (c.kind == 1)
+        {
+            const double u_0 = c.prm[0], v_0 = c.prm[1], beta = c.prm[2], r_v = c.prm[3], L = c.prm[4];
+            const double gamma = 1.4, pi = 3.141592653589793238462643383279502884;
+            double xc = L / 2.0 + u_0 * c.t, yc = L / 2.0 + v_0 * c.t;
+            xc = xc - L * ::floor(xc / L);
+            yc = yc - L * ::floor(yc / L);
+            const double dx = x - xc, dy = y - yc;
+            const double f  = 1.0 - ((dx * dx + dy * dy) / (r_v * r_v));
+            const double ef = ::exp(f);
+            const double rho = ::pow(1.0 - (beta * beta * (gamma - 1.0) * (ef * ef)) / (16.0 * gamma * pi * pi),
+                                     1.0 / (gamma - 1.0));
+            const double u = u_0 - (beta * ef * dy) / (2.0 * pi * r_v);
+            const double v = v_0 + (beta * ef * dx) / (2.0 * pi * r_v);
+            const double p = ::pow(rho, gamma);
+            st[0] = rho;
+            st[1] = rho * u;
+            st[2] = rho * v;
+            st[3] = (p / (gamma - 1.0)) + 0.5 * rho * (u * u + v * v);
+            return;
+        }
+        const double dx = x - c.prm[5], dy = y - c.prm[6];
+        const double r  = ::sqrt(dx * dx + dy * dy);
+        const double rho = (r < c.prm[4]) ? c.prm[0] : c.prm[2];
+        const double p   = (r < c.prm[4]) ? c.prm[1] : c.prm[3];
+        st[0] = rho;
+        st[1] = rho * 0.0;
+        st[2] = rho * 0.0;
+        st[3] = gas_rhoE(g, rho, 0.0, p);
+    }
+    // node position as impose_ic places it (physics_impl_2D.hpp:163-177: Cartesian
+    // h*(i + (1+csi)/2)); mapped geometries use the physical coordinates.
+    __device__ void node_xy(const GeomP& g, int cart_pos, int nx, int ny, double lx, double ly,
+                            const double* csi, int i, int j, int k, int l, double& x, double& y)
+    {
+        if (cart_pos)
+        {
+            const double hx = lx / nx, hy = ly / ny;
+            x = hx * (i + 0.5 * (1.0 + csi[k]));
+            y = hy * (j + 0.5 * (1.0 + csi[l]));
+            return;
+        }
+        const double cx = (1.0 / static_cast<double>(nx)) * (static_cast<double>(i) + 0.5 * (1.0 + csi[k]));
+        const double cy = (1.0 / static_cast<double>(ny)) * (static_cast<double>(j) + 0.5 * (1.0 + csi[l]));
+        geom_phys(g, cx, cy, x, y);
+    }
+    __global__ void k_ic2d(GeomP g, GasP gas, ICP c, int cart_pos, int nx, int ny, int n, double lx,
+                           double ly, const double* __restrict__ csi, long long E, double* __restrict__ u)
+    {
+        const long long t  = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+        const long long NN = static_cast<long long>(n) * n;
+        if (t >= E * NN) return;
+        const long long e = t / NN;
+        const int node = static_cast<int>(t - e * NN), k = node % n, l = node / n;
+        const int i = static_cast<int>(e % nx), j = static_cast<int>(e / nx);
+        double x, y, st[4];
+        node_xy(g, cart_pos, nx, ny, lx, ly, csi, i, j, k, l, x, y);
+        ic_state(gas, c, x, y, st);
+        for (int q = 0; q < 4; ++q) u[t * 4 + q] = st[q];
+    }
+    // physics_impl_1D.hpp:207-244 with factory.hpp:196-214 (Sod) or a uniform state
+    __global__ void k_ic1d(GasP gas, ICP c, int ne, int n, double lx, const double* __restrict__ csi,
+                           double* __restrict__ u)
+    {
+        const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+        if (t >= static_cast<long long>(ne) * n) return;
+        const int e = static_cast<int>(t / n), q = static_cast<int>(t % n);
+        const double h = lx / ne, pos = h * (e + 0.5 * (1.0 + csi[q]));
+        double rho, vel, p;
+        if (c.kind == 3)
+        {
+            const bool left = pos < lx / 2.0;
+            rho = left ? c.prm[0] : c.prm[3];
+            vel = left ? c.prm[1] : c.prm[4];
+            p   = left ? c.prm[2] : c.prm[5];
+        }
+        else
+        {
+            rho = c.prm[0];
+            vel = c.prm[1];
+            p   = c.prm[3];
+        }
+        double* sp = u + static_cast<long long>(e) * 3 * n;
+        sp[0 * n + q] = rho;
+        sp[1 * n + q] = rho * vel;
+        sp[2 * n + q] = gas_rhoE(gas, rho, vel, p);
+    }
+    // compute_error partial sums (physics_impl_2D.hpp:203-265): one 4-vector per
+    // block, reduced in fixed order by k_sum_blocks (deterministic).
+    __global__ __launch_bounds__(256) void k_l2_part(GeomP g, GasP gas, ICP c, int nx, int ny, int n,
+                                                     const double* __restrict__ csi,
+                                                     const double* __restrict__ w, long long E,
+                                                     const double* __restrict__ u, double* part)
+    {
+        __shared__ double s[256][4];
+        const long long NN = static_cast<long long>(n) * n;
+        const long long t  = blockIdx.x * 256LL + threadIdx.x;
+        double acc[4] = {0, 0, 0, 0};
+        if (t < E * NN)
+        {
+            const long long e = t / NN;
+            const int node = static_cast<int>(t - e * NN), k = node % n, l = node / n;
+            const int i = static_cast<int>(e % nx), j = static_cast<int>(e / nx);
+            const double cx = (1.0 / static_cast<double>(nx)) * (static_cast<double>(i) + 0.5 * (1.0 + csi[k]));
+            const double cy = (1.0 / static_cast<double>(ny)) * (static_cast<double>(j) + 0.5 * (1.0 + csi[l]));
+            double x, y, st[4];
+            geom_phys(g, cx, cy, x, y);
+            const VolM m = composite_metrics(g, nx, ny, cx, cy);
+            ic_state(gas, c, x, y, st);
+            const double cv = w[k] * w[l] * m.detJ;
+            for (int q = 0; q < 4; ++q)
+            {
+                const double dd = u[t * 4 + q] - st[q];
+                acc[q]          = cv * dd * dd;
+            }
+        }
+        for (int q = 0; q < 4; ++q) s[threadIdx.x][q] = acc[q];
+        __syncthreads();
+        for (int stp = 128; stp > 0; stp >>= 1)
+        {
+            if (threadIdx.x < stp)
+                for (int q = 0; q < 4; ++q) s[threadIdx.x][q] += s[threadIdx.x + stp][q];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            for (int q = 0; q < 4; ++q) part[blockIdx.x * 4LL + q] = s[0][q];
+    }
+    __global__ __launch_bounds__(256) void k_sum_blocks(long long nb, const double* part, double* out)
+    {
+        __shared__ double s[256][4];
+        double acc[4] = {0, 0, 0, 0};
+        for (long long b = threadIdx.x; b < nb; b += 256)
+            for (int q = 0; q < 4; ++q) acc[q] += part[b * 4 + q];
+        for (int q = 0; q < 4; ++q) s[threadIdx.x][q] = acc[q];
+        __syncthreads();
+        for (int stp = 128; stp > 0; stp >>= 1)
+        {
+            if (threadIdx.x < stp)
+                for (int q = 0; q < 4; ++q) s[threadIdx.x][q] += s[threadIdx.x + stp][q];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            for (int q = 0; q < 4; ++q) out[q] = s[0][q];
+    }
+
+    // ---------------- dispatch over n = p+1 ----------------
+    template <template <int> class F, class... A>
+    int by_n(int n, A&&... a)
+    {
+        switch (n)
+        {
+            case 2: return F<2>::run(a...);
+            case 3: return F<3>::run(a...);
+            case 4: return F<4>::run(a...);
+            case 5: return F<5>::run(a...);
+            case 6: return F<6>::run(a...);
+            case 7: return F<7>::run(a...);
+            case 8: return F<8>::run(a...);
+            case 9: return F<9>::run(a...);
+        }
+        return FRSDG_ERR_UNSUPPORTED;
+    }
+
+    template <class IO>
+    struct Launch
+    {
+        template <int N>
+        struct R2
+        {
+            static int run(const frsdg_ctx* c, const IO& io, hipStream_t s)
+            {
+                using S          = typename IO::S;
+                constexpr int BS = Tile2D<N, S>::BS, EPB = Tile2D<N, S>::EPB;
+                const long long nb = (c->E + EPB - 1) / EPB;
+                hipLaunchKernelGGL((k_res2d<N, IO>), dim3(static_cast<unsigned>(nb)), dim3(BS), 0, s, c->p2, io);
+                return hip_rc(hipGetLastError());
+            }
+        };
+        template <int N>
+        struct R1
+        {
+            static int run(const frsdg_ctx* c, const IO& io, hipStream_t s)
+            {
+                const int nb = (c->p1.ne + 63) / 64;
+                hipLaunchKernelGGL((k_res1d<N, IO>), dim3(nb), dim3(64), 0, s, c->p1, io);
+                return hip_rc(hipGetLastError());
+            }
+        };
+    };
+    template <class IO>
+    int launch_res(const frsdg_ctx* c, const IO& io, hipStream_t s)
+    {
+        if (c->d.dims == 2) return by_n<Launch<IO>::template R2>(c->n, c, io, s);
+        return by_n<Launch<IO>::template R1>(c->n, c, io, s);
+    }
+
+    template <int N>
+    struct Filt
+    {
+        static int run(const frsdg_ctx* c, const double* u, double* o, hipStream_t s)
+        {
+            if (c->d.dims == 2)
+            {
+                constexpr int EPB = (256 / (N * N)) > 0 ? 256 / (N * N) : 1;
+                const long long nb = (c->E + EPB - 1) / EPB;
+                hipLaunchKernelGGL((k_filter2d<N>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, c->E,
+                                   c->d_tab, u, o);
+            }
+            else
+            {
+                const int nb = (c->p1.ne + 63) / 64;
+                hipLaunchKernelGGL((k_filter1d<N>), dim3(nb), dim3(64), 0, s, c->p1.ne, c->d_tab, u, o);
+            }
+            return hip_rc(hipGetLastError());
+        }
+    };
+
+    int ensure_work(frsdg_ctx* c)
+    {
+        if (!c->w1) FR_TRY(hipMalloc(&c->w1, sizeof(double) * c->nd));
+        if (!c->w2) FR_TRY(hipMalloc(&c->w2, sizeof(double) * c->nd));
+        return FRSDG_OK;
+    }
+
+    int do_filter(frsdg_ctx* c, const double* u, double* out, hipStream_t s)
+    {
+        if (!c->d.apply_filter)
+        {
+            if (out != u) FR_TRY(hipMemcpyAsync(out, u, sizeof(double) * c->nd, hipMemcpyDeviceToDevice, s));
+            return FRSDG_OK;
+        }
+        return by_n<Filt>(c->n, c, u, out, s);
+    }
+
+    // one explicit step, stages fused with the residual kernel
+    int do_step(frsdg_ctx* c, double dt, double* u, int stages, hipStream_t s)
+    {
+        auto stage = [&](const double* us, double a0, double b, double cdt, double* out)
+        {
+            IOStage io{us, u, out, a0, b, cdt};
+            return launch_res(c, io, s);
+        };
+        int rc;
+        if (stages == 3)
+        {
+            if ((rc = stage(u, 0.0, 1.0, dt, c->w1))) return rc;
+            if ((rc = stage(c->w1, 0.75, 0.25, dt, c->w2))) return rc;
+            if ((rc = stage(c->w2, 1.0 / 3.0, 2.0 / 3.0, dt, u))) return rc;
+        }
+        else if (stages == 4)
+        {
+            const double h = 0.5 * dt;
+            if ((rc = stage(u, 0.0, 1.0, h, c->w1))) return rc;
+            if ((rc = stage(c->w1, 0.0, 1.0, h, c->w2))) return rc;
+            if ((rc = stage(c->w2, 2.0 / 3.0, 1.0 / 3.0, h, c->w1))) return rc;
+            // last stage reads w1 (neighbour traces) and writes u
+            IOStage io{c->w1, nullptr, u, 0.0, 1.0, h};
+            if ((rc = launch_res(c, io, s))) return rc;
+        }
+        else
+            return FRSDG_ERR_INVALID;
+        if (c->d.apply_filter) return by_n<Filt>(c->n, c, u, u, s);
+        return FRSDG_OK;
+    }
+
+    GeomP geom_of(const frsdg_desc& d)
+    {
+        GeomP g{};
+        g.kind = d.geometry;
+        g.Lx   = d.lx;
+        g.Ly   = d.ly;
+        for (int i = 0; i < 4; ++i) g.prm[i] = d.geom_param[i];
+        return g;
+    }
+} // namespace
+
+extern "C" {
+
+const char* frsdg_strerror(int code)
+{
+    switch (code)
+    {
+        case FRSDG_OK: return "success";
+        case FRSDG_ERR_INVALID: return "invalid argument or descriptor";
+        case FRSDG_ERR_HIP: return "HIP runtime error";
+        case FRSDG_ERR_UNSUPPORTED: return "unsupported on this path";
+        case FRSDG_ERR_ALIGN: return "device pointer not 16-byte aligned";
+    }
+    return "unknown error";
+}
+
+int frsdg_create(const frsdg_desc* d, frsdg_ctx** out)
+{
+    if (!d || !out) return FRSDG_ERR_INVALID;
+    *out = nullptr;
+    if (d->dims != 1 && d->dims != 2) return FRSDG_ERR_INVALID;
+    if (d->p < 1 || d->p > 8 || d->nx < 1 || (d->dims == 2 && d->ny < 1)) return FRSDG_ERR_INVALID;
+    if (d->gas_kind < 0 || d->gas_kind > 1 || d->riemann < 0 || d->riemann > 1) return FRSDG_ERR_INVALID;
+    if (d->geometry < 0 || d->geometry > 4 || (d->dims == 1 && d->geometry != 0)) return FRSDG_ERR_INVALID;
+    for (int k = 0; k < (d->dims == 2 ? 4 : 2); ++k)
+    {
+        const int t = d->bc_type[k];
+        if (!(t == 0 || t == 1 || t == 2 || t == 5 || (t == 6 && d->dims == 2))) return FRSDG_ERR_INVALID;
+    }
+    if (d->apply_filter && (d->filt_cutoff < 0 || d->filt_cutoff >= d->p)) return FRSDG_ERR_INVALID;
+    auto* c = new (std::nothrow) frsdg_ctx{};
+    if (!c) return FRSDG_ERR_INVALID;
+    c->d = *d;
+    if (!make_tables(d->p, d->c_vcjh, d->filt_cutoff, d->filt_strength, d->filt_alpha, c->tb))
+    {
+        delete c;
+        return FRSDG_ERR_INVALID;
+    }
+    const int n = c->n = d->p + 1;
+    GasP gas{d->gas_kind, d->gamma, d->vdw_a, d->vdw_b};
+    BCP  bc{};
+    for (int k = 0; k < 4; ++k)
+    {
+        bc.type[k] = d->bc_type[k];
+        for (int q = 0; q < 4; ++q) bc.state[k][q] = d->bc_state[k][q];
+    }
+    bc.freeze = d->tangent_bc_mode;
+    std::vector<double> tab;
+    tab.insert(tab.end(), c->tb.D.begin(), c->tb.D.end());
+    tab.insert(tab.end(), c->tb.dL.begin(), c->tb.dL.end());
+    tab.insert(tab.end(), c->tb.dR.begin(), c->tb.dR.end());
+    tab.insert(tab.end(), c->tb.filt.begin(), c->tb.filt.end());
+    int rc = FRSDG_OK;
+    auto fail = [&](int code)
+    {
+        frsdg_destroy(c);
+        return code;
+    };
+    if (hipMalloc(&c->d_tab, sizeof(double) * tab.size()) != hipSuccess) return fail(FRSDG_ERR_HIP);
+    if (hipMemcpy(c->d_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(FRSDG_ERR_HIP);
+    if (hipMalloc(&c->d_csi, sizeof(double) * 2 * n) != hipSuccess) return fail(FRSDG_ERR_HIP);
+    {
+        std::vector<double> cw(c->tb.csi);
+        cw.insert(cw.end(), c->tb.w.begin(), c->tb.w.end());
+        if (hipMemcpy(c->d_csi, cw.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(FRSDG_ERR_HIP);
+    }
+    if (d->dims == 1)
+    {
+        if (d->nx < 2 && !(d->bc_type[0] == 5 && d->bc_type[1] == 5)) return fail(FRSDG_ERR_INVALID);
+        c->E  = d->nx;
+        c->nd = 3LL * n * d->nx;
+        c->p1 = Res1DP{d->nx, d->bc_type[0] == 5 && d->bc_type[1] == 5, d->riemann,
+                       2.0 / (d->lx / d->nx), c->d_tab, gas, bc};
+        *out  = c;
+        return FRSDG_OK;
+    }
+    c->E  = static_cast<long long>(d->nx) * d->ny;
+    c->nd = 4LL * n * n * c->E;
+    Res2DP& P = c->p2;
+    P.nx      = d->nx;
+    P.ny      = d->ny;
+    P.E       = c->E;
+    P.per_x   = d->bc_type[0] == 5 && d->bc_type[1] == 5;
+    P.per_y   = d->bc_type[2] == 5 && d->bc_type[3] == 5;
+    P.riemann = d->riemann;
+    P.tab     = c->d_tab;
+    P.gas     = gas;
+    P.bc      = bc;
+    P.curved  = d->geometry != 0;
+    const GeomP g = geom_of(*d);
+    {
+        // affine constants exactly as the reference composes them
+        const VolM m = composite_metrics(g, d->nx, d->ny, 0.0, 0.0);
+        P.xi_x  = m.xi_x;
+        P.eta_y = m.eta_y;
+        P.sc_x  = std::sqrt(m.xi_x * m.xi_x + m.xi_y * m.xi_y);
+        P.sc_y  = std::sqrt(m.eta_x * m.eta_x + m.eta_y * m.eta_y);
+    }
+    if (P.curved)
+    {
+        const long long nv  = c->E * n * n;
+        const long long nfx = static_cast<long long>(d->ny) * (d->nx + 1) * n;
+        const long long nfy = static_cast<long long>(d->ny + 1) * d->nx * n;
+        if (hipMalloc(&c->d_vmet, sizeof(double) * 4 * nv) != hipSuccess) return fail(FRSDG_ERR_HIP);
+        if (hipMalloc(&c->d_fxm, sizeof(double) * 4 * nfx) != hipSuccess) return fail(FRSDG_ERR_HIP);
+        if (hipMalloc(&c->d_fym, sizeof(double) * 4 * nfy) != hipSuccess) return fail(FRSDG_ERR_HIP);
+        hipLaunchKernelGGL(k_geom_vol, dim3(static_cast<unsigned>((nv + 255) / 256)), dim3(256), 0, nullptr, g,
+                           d->nx, d->ny, n, c->d_csi, c->E, c->d_vmet);
+        hipLaunchKernelGGL(k_geom_face, dim3(static_cast<unsigned>((nfx + 255) / 256)), dim3(256), 0, nullptr, g,
+                           d->nx, d->ny, n, c->d_csi, 1, c->d_fxm);
+        hipLaunchKernelGGL(k_geom_face, dim3(static_cast<unsigned>((nfy + 255) / 256)), dim3(256), 0, nullptr, g,
+                           d->nx, d->ny, n, c->d_csi, 0, c->d_fym);
+        if (hipDeviceSynchronize() != hipSuccess) return fail(FRSDG_ERR_HIP);
+        P.vmet = c->d_vmet;
+        P.fxm  = c->d_fxm;
+        P.fym  = c->d_fym;
+    }
+    (void) rc;
+    *out = c;
+    return FRSDG_OK;
+}
+
+int frsdg_destroy(frsdg_ctx* c)
+{
+    if (!c) return FRSDG_OK;
+    if (c->gexec) (void) hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void) hipGraphDestroy(c->graph);
+    for (double* p : {c->d_tab, c->d_vmet, c->d_fxm, c->d_fym, c->d_csi, c->w1, c->w2, c->red})
+        if (p) (void) hipFree(p);
+    delete c;
+    return FRSDG_OK;
+}
+
+long long frsdg_ndofs(const frsdg_ctx* c) { return c ? c->nd : -1; }
+
+int frsdg_get_tables(const frsdg_ctx* c, double* csi, double* w, double* D, double* dL, double* dR,
+                     double* filt)
+{
+    if (!c) return FRSDG_ERR_INVALID;
+    const int n = c->n;
+    if (csi) std::memcpy(csi, c->tb.csi.data(), sizeof(double) * n);
+    if (w) std::memcpy(w, c->tb.w.data(), sizeof(double) * n);
+    if (D) std::memcpy(D, c->tb.D.data(), sizeof(double) * n * n);
+    if (dL) std::memcpy(dL, c->tb.dL.data(), sizeof(double) * n);
+    if (dR) std::memcpy(dR, c->tb.dR.data(), sizeof(double) * n);
+    if (filt) std::memcpy(filt, c->tb.filt.data(), sizeof(double) * n * n);
+    return FRSDG_OK;
+}
+
+int frsdg_residual(frsdg_ctx* c, double, const double* u, double* R, void* stream)
+{
+    if (!c || !u || !R) return FRSDG_ERR_INVALID;
+    if (!aligned(u) || !aligned(R)) return FRSDG_ERR_ALIGN;
+    IORes io{u, R, nullptr, 1.0};
+    return launch_res(c, io, static_cast<hipStream_t>(stream));
+}
+
+int frsdg_rhs(frsdg_ctx* c, double, const double* u, double* rhs, void* stream)
+{
+    if (!c || !u || !rhs) return FRSDG_ERR_INVALID;
+    if (!aligned(u) || !aligned(rhs)) return FRSDG_ERR_ALIGN;
+    IORes io{u, rhs, nullptr, -1.0};
+    return launch_res(c, io, static_cast<hipStream_t>(stream));
+}
+
+int frsdg_ifunction(frsdg_ctx* c, double, const double* u, const double* udot, double* F, void* stream)
+{
+    if (!c || !u || !udot || !F) return FRSDG_ERR_INVALID;
+    if (!aligned(u) || !aligned(udot) || !aligned(F)) return FRSDG_ERR_ALIGN;
+    IORes io{u, F, udot, 1.0};
+    return launch_res(c, io, static_cast<hipStream_t>(stream));
+}
+
+int frsdg_jvp(frsdg_ctx* c, double, const double* u, const double* v, double sigma, double* out,
+              void* stream)
+{
+    if (!c || !u || !v || !out) return FRSDG_ERR_INVALID;
+    if (!aligned(u) || !aligned(v) || !aligned(out)) return FRSDG_ERR_ALIGN;
+    IOTan io{u, v, out, sigma};
+    return launch_res(c, io, static_cast<hipStream_t>(stream));
+}
+
+int frsdg_jvp_cstep(frsdg_ctx* c, double, const double* u, const double* v, double sigma, double h,
+                    double* out, void* stream)
+{
+    if (!c || !u || !v || !out || !(h > 0.0)) return FRSDG_ERR_INVALID;
+    if (!aligned(u) || !aligned(v) || !aligned(out)) return FRSDG_ERR_ALIGN;
+    // the complex residual sees lambda-bound BCs frozen (boundary_conditions.hpp:270-294)
+    const int keep      = c->p2.bc.freeze;
+    c->p2.bc.freeze     = 1;
+    c->p1.bc.freeze     = 1;
+    IOCs io{u, v, out, sigma, h, 1.0 / h};
+    const int rc    = launch_res(c, io, static_cast<hipStream_t>(stream));
+    c->p2.bc.freeze = keep;
+    c->p1.bc.freeze = keep;
+    return rc;
+}
+
+int frsdg_vjp(frsdg_ctx*, double, const double*, const double*, double, double*, void*)
+{
+    return FRSDG_ERR_UNSUPPORTED;
+}
+
+int frsdg_pre_jv_eval(frsdg_ctx* c, double t, const double* u, double sigma)
+{
+    if (!c || !u) return FRSDG_ERR_INVALID;
+    c->jv_u     = u;
+    c->jv_sigma = sigma;
+    c->jv_t     = t;
+    return FRSDG_OK;
+}
+
+int frsdg_matmult(frsdg_ctx* c, const double* v, double* out, void* stream)
+{
+    if (!c || !c->jv_u) return FRSDG_ERR_INVALID;
+    if (c->d.jac_type == 1) return frsdg_jvp_cstep(c, c->jv_t, c->jv_u, v, c->jv_sigma, 1e-7, out, stream);
+    return frsdg_jvp(c, c->jv_t, c->jv_u, v, c->jv_sigma, out, stream);
+}
+
+int frsdg_filter(frsdg_ctx* c, const double* u, double* out, void* stream)
+{
+    if (!c || !u || !out) return FRSDG_ERR_INVALID;
+    if (!aligned(u) || !aligned(out)) return FRSDG_ERR_ALIGN;
+    return do_filter(c, u, out, static_cast<hipStream_t>(stream));
+}
+
+int frsdg_apply_stabilization(frsdg_ctx* c, double* u, void* stream)
+{
+    if (!c || !u) return FRSDG_ERR_INVALID;
+    if (!c->d.apply_filter) return FRSDG_OK;
+    return frsdg_filter(c, u, u, stream);
+}
+
+int frsdg_ssprk_step(frsdg_ctx* c, double, double dt, double* u, int stages, void* stream)
+{
+    if (!c || !u) return FRSDG_ERR_INVALID;
+    if (!aligned(u)) return FRSDG_ERR_ALIGN;
+    if (stages != 3 && stages != 4) return FRSDG_ERR_INVALID;
+    int rc = ensure_work(c);
+    if (rc) return rc;
+    return do_step(c, dt, u, stages, static_cast<hipStream_t>(stream));
+}
+
+int frsdg_ssprk_run(frsdg_ctx* c, double t, double dt, double* u, int stages, int nsteps, void* stream)
+{
+    if (!c || !u || nsteps < 0) return FRSDG_ERR_INVALID;
+    if (!aligned(u)) return FRSDG_ERR_ALIGN;
+    if (stages != 3 && stages != 4) return FRSDG_ERR_INVALID;
+    int rc = ensure_work(c);
+    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    constexpr int CH = 16; // steps per captured graph
+    if (nsteps < CH) // short runs: plain launches
+    {
+        for (int k = 0; k < nsteps; ++k)
+            if ((rc = frsdg_ssprk_step(c, t + k * dt, dt, u, stages, stream))) return rc;
+        return FRSDG_OK;
+    }
+    if (!c->gexec || c->g_u != u || c->g_dt != dt || c->g_st != stages)
+    {
+        if (c->gexec) (void) hipGraphExecDestroy(c->gexec);
+        if (c->graph) (void) hipGraphDestroy(c->graph);
+        c->gexec = nullptr;
+        c->graph = nullptr;
+        hipStream_t cs;
+        FR_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        {
+            (void) hipStreamDestroy(cs);
+            return FRSDG_ERR_HIP;
+        }
+        int rr = FRSDG_OK;
+        for (int k = 0; k < CH && !rr; ++k) rr = do_step(c, dt, u, stages, cs);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(cs, &g);
+        (void) hipStreamDestroy(cs);
+        if (rr || ec != hipSuccess) return rr ? rr : FRSDG_ERR_HIP;
+        c->graph = g;
+        FR_TRY(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+        c->g_u  = u;
+        c->g_dt = dt;
+        c->g_st = stages;
+    }
+    const int full = nsteps / CH, rem = nsteps % CH;
+    for (int k = 0; k < full; ++k) FR_TRY(hipGraphLaunch(c->gexec, s));
+    for (int k = 0; k < rem; ++k)
+        if ((rc = do_step(c, dt, u, stages, s))) return rc;
+    return FRSDG_OK;
+}
+
+int frsdg_impose_ic(frsdg_ctx* c, int kind, const double* prm, double* u, void* stream)
+{
+    if (!c || !prm || !u || kind < 0 || kind > 3) return FRSDG_ERR_INVALID;
+    ICP ic{};
+    ic.kind = kind;
+    for (int q = 0; q < 8; ++q) ic.prm[q] = prm[q];
+    ic.t = 0.0;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int n = c->n;
+    if (c->d.dims == 1)
+    {
+        if (kind != 0 && kind != 3) return FRSDG_ERR_INVALID;
+        const long long tot = static_cast<long long>(c->d.nx) * n;
+        hipLaunchKernelGGL(k_ic1d, dim3(static_cast<unsigned>((tot + 255) / 256)), dim3(256), 0, s, c->p1.gas, ic,
+                           c->d.nx, n, c->d.lx, c->d_csi, u);
+        return hip_rc(hipGetLastError());
+    }
+    if (kind == 3) return FRSDG_ERR_INVALID;
+    const long long tot = c->E * n * n;
+    hipLaunchKernelGGL(k_ic2d, dim3(static_cast<unsigned>((tot + 255) / 256)), dim3(256), 0, s, geom_of(c->d),
+                       c->p2.gas, ic, c->d.geometry == 0 ? 1 : 0, c->d.nx, c->d.ny, n, c->d.lx, c->d.ly,
+                       c->d_csi, c->E, u);
+    return hip_rc(hipGetLastError());
+}
+
+int frsdg_l2_error(frsdg_ctx* c, int kind, const double* prm, double t, const double* u, double* out,
+                   void* stream)
+{
+    if (!c || !prm || !u || !out || c->d.dims != 2 || kind < 0 || kind > 2) return FRSDG_ERR_INVALID;
+    ICP ic{};
+    ic.kind = kind;
+    for (int q = 0; q < 8; ++q) ic.prm[q] = prm[q];
+    ic.t = t;
+    hipStream_t     s   = static_cast<hipStream_t>(stream);
+    const int       n   = c->n;
+    const long long tot = c->E * n * n;
+    const long long nb  = (tot + 255) / 256;
+    if (!c->red) FR_TRY(hipMalloc(&c->red, sizeof(double) * 4 * (nb + 1)));
+    hipLaunchKernelGGL(k_l2_part, dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, geom_of(c->d), c->p2.gas, ic,
+                       c->d.nx, c->d.ny, n, c->d_csi, c->d_csi + n, c->E, u, c->red + 4);
+    hipLaunchKernelGGL(k_sum_blocks, dim3(1), dim3(256), 0, s, nb, c->red + 4, c->red);
+    FR_TRY(hipGetLastError());
+    double h[4];
+    FR_TRY(hipMemcpyAsync(h, c->red, sizeof(h), hipMemcpyDeviceToHost, s));
+    FR_TRY(hipStreamSynchronize(s));
+    for (int q = 0; q < 4; ++q) out[q] = std::sqrt(h[q]);
+    return FRSDG_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,295 @@
+// frsdg_physics.hpp — pointwise physics shared by every kernel, templated on
+// the scalar type (double / Dual / Cplx, frsdg_scalar.hpp).
+//
+// Formulas follow the reference so that results agree to rounding:
+//   smooth_abs/max/min          macro_and_libc.hpp:80-113 (eps = 1e-7)
+//   wrap_positive               macro_and_libc.hpp:115-138
+//   ideal gas / van der Waals   gas_models.hpp:56-158
+//   2D fluxes                   riemann_solver.hpp:48-77
+//   rotated Rusanov / HLLC      riemann_solver.hpp:189-349
+//   1D flux / Rusanov / HLLC    riemann_solver.hpp:39-46, 87-184
+// Arithmetic is regrouped where that saves divides on the GPU (one
+// reciprocal of rho per state, one of (sL-sM) per HLLC star state); the
+// differences are at rounding level.
+#pragma once
+#include "frsdg_scalar.hpp"
+
+namespace frsdg
+{
+    struct GasP
+    {
+        int    kind; // 0 ideal, 1 van der Waals
+        double gamma, a, b;
+    };
+
+    template <class S>
+    FR_HD S smooth_abs(S x)
+    {
+        const double eps = 1e-7;
+        return sqrt_(x * x + eps * eps) - eps;
+    }
+    template <class S>
+    FR_HD S smooth_max(S a, S b)
+    {
+        return 0.5 * (a + b + smooth_abs(a - b));
+    }
+    template <class S>
+    FR_HD S smooth_min(S a, S b)
+    {
+        return 0.5 * (a + b - smooth_abs(a - b));
+    }
+
+    // pressure from density and specific internal energy
+    template <class S>
+    FR_HD S gas_pressure(const GasP& g, S rho, S e)
+    {
+        if (g.kind == 0) return (g.gamma - 1.0) * rho * e;
+        const S covol = 1.0 - g.b * rho;
+        const S cvT   = e + g.a * rho;
+        return ((g.gamma - 1.0) * rho * cvT) / covol - g.a * rho * rho;
+    }
+    // sound speed (ideal: sqrt(gamma p / rho) with p passed in)
+    template <class S>
+    FR_HD S gas_sound(const GasP& g, S rho, S e, S p, S inv_rho)
+    {
+        if (g.kind == 0) return sqrt_(g.gamma * p * inv_rho);
+        const S covol = 1.0 - g.b * rho;
+        const S inv_d = 1.0 / (covol * covol);
+        const S th    = e + g.a * rho;
+        const S t1    = (g.gamma * (g.gamma - 1.0) * th) * inv_d;
+        const S t2    = ((g.gamma - 1.0) * g.a * rho * (2.0 - g.b * rho)) * inv_d;
+        const S t3    = 2.0 * g.a * rho;
+        return sqrt_(t1 + t2 - t3);
+    }
+
+    // primitive state of a 2D node: floored rho, 1/rho, u, v, e (make_state), p
+    template <class S>
+    struct Prim2
+    {
+        S rho, inv_rho, u, v, e, p;
+    };
+    template <class S>
+    FR_HD Prim2<S> prim2(const GasP& g, const S U[4])
+    {
+        Prim2<S> q;
+        q.rho        = Sc<S>::wrap_positive(U[0]);
+        q.inv_rho    = 1.0 / q.rho;
+        q.u          = U[1] * q.inv_rho;
+        q.v          = U[2] * q.inv_rho;
+        const S totE = Sc<S>::wrap_positive(U[3]);
+        q.e          = totE * q.inv_rho - 0.5 * (q.u * q.u + q.v * q.v);
+        q.p          = gas_pressure(g, q.rho, q.e);
+        return q;
+    }
+    // euler_fluxes_2d_w_p: F, G at a node
+    template <class S>
+    FR_HD void flux2(const GasP& g, const S U[4], S F[4], S G[4])
+    {
+        const Prim2<S> q  = prim2(g, U);
+        const S        Ep = U[3] + q.p;
+        const S        uv = U[1] * q.v;
+        F[0]              = U[1];
+        F[1]              = U[1] * q.u + q.p;
+        F[2]              = uv;
+        F[3]              = q.u * Ep;
+        G[0]              = U[2];
+        G[1]              = uv;
+        G[2]              = U[2] * q.v + q.p;
+        G[3]              = q.v * Ep;
+    }
+
+    // rotated 2D numerical flux; kind 0 Rusanov, 1 HLLC
+    template <class S>
+    FR_HD void riemann2(const GasP& g, int kind, const S UL[4], const S UR[4], double nx, double ny,
+                        S out[4])
+    {
+        const Prim2<S> L   = prim2(g, UL);
+        const Prim2<S> R   = prim2(g, UR);
+        const S        unL = L.u * nx + L.v * ny;
+        const S        utL = -L.u * ny + L.v * nx;
+        const S        unR = R.u * nx + R.v * ny;
+        const S        utR = -R.u * ny + R.v * nx;
+        const S        cL  = gas_sound(g, L.rho, L.e, L.p, L.inv_rho);
+        const S        cR  = gas_sound(g, R.rho, R.e, R.p, R.inv_rho);
+        // normal-frame physical fluxes
+        const S mL = UL[0] * unL, mR = UR[0] * unR;
+        S       FL[4] = {mL, mL * unL + L.p, mL * utL, (UL[3] + L.p) * unL};
+        S       FR[4] = {mR, mR * unR + R.p, mR * utR, (UR[3] + R.p) * unR};
+        S       fs[4];
+        if (kind == 0)
+        {
+            const S smax = smooth_max(smooth_abs(unL) + cL, smooth_abs(unR) + cR);
+            fs[0]        = 0.5 * (FL[0] + FR[0]) - 0.5 * smax * (UR[0] - UL[0]);
+            fs[1]        = 0.5 * (FL[1] + FR[1]) - 0.5 * smax * (UR[0] * unR - UL[0] * unL);
+            fs[2]        = 0.5 * (FL[2] + FR[2]) - 0.5 * smax * (UR[0] * utR - UL[0] * utL);
+            fs[3]        = 0.5 * (FL[3] + FR[3]) - 0.5 * smax * (UR[3] - UL[3]);
+        }
+        else
+        {
+            const S sL   = smooth_min(unL - cL, unR - cR);
+            const S sR   = smooth_max(unL + cL, unR + cR);
+            const S rLd  = UL[0] * (sL - unL);
+            const S rRd  = UR[0] * (sR - unR);
+            const S sM   = (R.p - L.p + unL * rLd - unR * rRd) / (rLd - rRd);
+            if (Sc<S>::branch(sL) >= 0.0)
+            {
+                for (int q = 0; q < 4; ++q) fs[q] = FL[q];
+            }
+            else if (Sc<S>::branch(sM) >= 0.0)
+            {
+                const S ps  = L.p + UL[0] * (sL - unL) * (sM - unL);
+                const S inv = 1.0 / (sL - sM);
+                fs[0]       = (sM * (sL * UL[0] - FL[0])) * inv;
+                fs[1]       = (sM * (sL * mL - FL[1]) + sL * ps) * inv;
+                fs[2]       = utL * fs[0];
+                fs[3]       = (sM * (sL * UL[3] - FL[3]) + sL * ps * sM) * inv;
+            }
+            else if (Sc<S>::branch(sR) >= 0.0)
+            {
+                const S ps  = R.p + UR[0] * (sR - unR) * (sM - unR);
+                const S inv = 1.0 / (sR - sM);
+                fs[0]       = (sM * (sR * UR[0] - FR[0])) * inv;
+                fs[1]       = (sM * (sR * mR - FR[1]) + sR * ps) * inv;
+                fs[2]       = utR * fs[0];
+                fs[3]       = (sM * (sR * UR[3] - FR[3]) + sR * ps * sM) * inv;
+            }
+            else
+            {
+                for (int q = 0; q < 4; ++q) fs[q] = FR[q];
+            }
+        }
+        out[0] = fs[0];
+        out[1] = fs[1] * nx - fs[2] * ny;
+        out[2] = fs[1] * ny + fs[2] * nx;
+        out[3] = fs[3];
+    }
+
+    // ---------------- 1D (riemann_solver.hpp:19-46, 87-184) ----------------
+    template <class S>
+    FR_HD void flux1(const GasP& g, const S U[3], S F[3])
+    {
+        const S rho  = Sc<S>::wrap_positive(U[0]);
+        const S ir   = 1.0 / rho;
+        const S u    = U[1] * ir;
+        const S totE = Sc<S>::wrap_positive(U[2]);
+        const S e    = totE * ir - 0.5 * u * u;
+        const S p    = gas_pressure(g, rho, e);
+        F[0]         = U[1];
+        F[1]         = U[1] * u + p;
+        F[2]         = (U[2] + p) * u;
+    }
+    template <class S>
+    FR_HD void riemann1(const GasP& g, int kind, const S UL[3], const S UR[3], S out[3])
+    {
+        S       FL[3], FR[3];
+        flux1(g, UL, FL);
+        flux1(g, UR, FR);
+        // make_state (floored) for the sound speeds; velocity with the unfloored
+        // density as in the reference 1D solvers (quirk: riemann_solver.hpp:102,131)
+        const S rhoL = Sc<S>::wrap_positive(UL[0]), rhoR = Sc<S>::wrap_positive(UR[0]);
+        const S irL = 1.0 / rhoL, irR = 1.0 / rhoR;
+        const S uLf = UL[1] * irL, uRf = UR[1] * irR;
+        const S eL  = Sc<S>::wrap_positive(UL[2]) * irL - 0.5 * uLf * uLf;
+        const S eR  = Sc<S>::wrap_positive(UR[2]) * irR - 0.5 * uRf * uRf;
+        const S pL_ = gas_pressure(g, rhoL, eL), pR_ = gas_pressure(g, rhoR, eR);
+        const S aL  = gas_sound(g, rhoL, eL, pL_, irL);
+        const S aR  = gas_sound(g, rhoR, eR, pR_, irR);
+        const S uL  = UL[1] / UL[0];
+        const S uR  = UR[1] / UR[0];
+        if (kind == 0)
+        {
+            const S sL   = smooth_abs(uL) + aL;
+            const S sR   = smooth_abs(uR) + aR;
+            const S smax = smooth_max(sL, sR);
+            for (int q = 0; q < 3; ++q) out[q] = 0.5 * (FL[q] + FR[q]) - 0.5 * smax * (UR[q] - UL[q]);
+            return;
+        }
+        const S pL  = FL[1] - uL * UL[1];
+        const S pR  = FR[1] - uR * UR[1];
+        const S sL  = smooth_min(uL - aL, uR - aR);
+        const S sR  = smooth_max(uL + aL, uR + aR);
+        const S rLd = UL[0] * (sL - uL);
+        const S rRd = UR[0] * (sR - uR);
+        const S sM  = (pR - pL + uL * rLd - uR * rRd) / (rLd - rRd);
+        if (Sc<S>::branch(sL) >= 0.0)
+        {
+            for (int q = 0; q < 3; ++q) out[q] = FL[q];
+        }
+        else if (Sc<S>::branch(sM) >= 0.0)
+        {
+            const S ps  = pL + UL[0] * (sL - uL) * (sM - uL);
+            const S inv = 1.0 / (sL - sM);
+            out[0]      = (sM * (sL * UL[0] - FL[0])) * inv;
+            out[1]      = (sM * (sL * UL[1] - FL[1]) + sL * ps) * inv;
+            out[2]      = (sM * (sL * UL[2] - FL[2]) + sL * ps * sM) * inv;
+        }
+        else if (Sc<S>::branch(sR) >= 0.0)
+        {
+            const S ps  = pR + UR[0] * (sR - uR) * (sM - uR);
+            const S inv = 1.0 / (sR - sM);
+            out[0]      = (sM * (sR * UR[0] - FR[0])) * inv;
+            out[1]      = (sM * (sR * UR[1] - FR[1]) + sR * ps) * inv;
+            out[2]      = (sM * (sR * UR[2] - FR[2]) + sR * ps * sM) * inv;
+        }
+        else
+        {
+            for (int q = 0; q < 3; ++q) out[q] = FR[q];
+        }
+    }
+
+    // ---------------- boundary states (boundary_conditions.hpp:214-324) ----------------
+    struct BCP
+    {
+        int    type[4];     // Left, Right, Top, Bottom
+        double state[4][4]; // Dirichlet states
+        int    freeze;      // 1: lambda-bound BCs carry no derivative (complex step semantics)
+    };
+    enum : int
+    {
+        BC_WALL = 0,
+        BC_OUTFLOW = 1,
+        BC_DIRICHLET = 2,
+        BC_PERIODIC = 5,
+        BC_SLIPWALL = 6
+    };
+    // exterior state for boundary `dir` (0 L, 1 R, 2 T, 3 B) given the interior
+    // trace and the face normal.
+    template <class S>
+    FR_HD void bc_exterior(const BCP& bc, int dir, const S in[4], double nx, double ny, S ex[4])
+    {
+        const int t = bc.type[dir];
+        for (int q = 0; q < 4; ++q) ex[q] = in[q];
+        if (t == BC_WALL)
+        {
+            const int axis = dir / 2; // :300-308
+            ex[1 + axis]   = -in[1 + axis];
+        }
+        else if (t == BC_DIRICHLET)
+        {
+            for (int q = 0; q < 4; ++q) ex[q] = Sc<S>::c(bc.state[dir][q]);
+        }
+        else if (t == BC_SLIPWALL)
+        {
+            // tests/bump.cpp:250-272: reflect the velocity about the face normal
+            S src[4];
+            for (int q = 0; q < 4; ++q) src[q] = bc.freeze ? Sc<S>::frozen(in[q]) : in[q];
+            const S uu = src[1] / src[0];
+            const S vv = src[2] / src[0];
+            const S vn = uu * nx + vv * ny;
+            ex[0]      = src[0];
+            ex[1]      = src[0] * (uu - 2.0 * vn * nx);
+            ex[2]      = src[0] * (vv - 2.0 * vn * ny);
+            ex[3]      = src[3];
+        }
+    }
+    template <class S>
+    FR_HD void bc_exterior1(const BCP& bc, int side, const S in[3], S ex[3])
+    {
+        const int t = bc.type[side];
+        for (int q = 0; q < 3; ++q) ex[q] = in[q];
+        if (t == BC_WALL)
+            ex[1] = -in[1];
+        else if (t == BC_DIRICHLET)
+            for (int q = 0; q < 3; ++q) ex[q] = Sc<S>::c(bc.state[side][q]);
+    }
+} // namespace frsdg

@@ -1,0 +1,127 @@
+// frsdg_scalar.hpp — the three scalar types the residual kernels are
+// instantiated on:
+//   double  — the plain residual;
+//   Dual    — forward tangent (value + one directional derivative); branches
+//             are taken on the value (primal) part;
+//   Cplx    — the complex step of petsc_solver_impl.hpp:270-359, with the
+//             reference's branch rule re+im (riemann_solver.hpp:299-305) and
+//             the CSD-safe wrap_positive of macro_and_libc.hpp:115-138.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cfloat>
+
+#define FR_HD __host__ __device__ __forceinline__
+
+namespace frsdg
+{
+    struct Dual
+    {
+        double v, d;
+    };
+    struct Cplx
+    {
+        double re, im;
+    };
+
+    // ---------------- Dual ----------------
+    FR_HD Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+    FR_HD Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+    FR_HD Dual operator-(Dual a) { return {-a.v, -a.d}; }
+    FR_HD Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+    FR_HD Dual operator/(Dual a, Dual b)
+    {
+        const double q = a.v / b.v;
+        return {q, (a.d - q * b.d) / b.v};
+    }
+    FR_HD Dual operator+(Dual a, double b) { return {a.v + b, a.d}; }
+    FR_HD Dual operator+(double a, Dual b) { return {a + b.v, b.d}; }
+    FR_HD Dual operator-(Dual a, double b) { return {a.v - b, a.d}; }
+    FR_HD Dual operator-(double a, Dual b) { return {a - b.v, -b.d}; }
+    FR_HD Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
+    FR_HD Dual operator*(double a, Dual b) { return {a * b.v, a * b.d}; }
+    FR_HD Dual operator/(Dual a, double b) { return {a.v / b, a.d / b}; }
+    FR_HD Dual operator/(double a, Dual b)
+    {
+        const double q = a / b.v;
+        return {q, -q * b.d / b.v};
+    }
+    FR_HD Dual sqrt(Dual a)
+    {
+        const double s = ::sqrt(a.v);
+        return {s, a.d / (2.0 * s)};
+    }
+
+    // ---------------- Cplx ----------------
+    FR_HD Cplx operator+(Cplx a, Cplx b) { return {a.re + b.re, a.im + b.im}; }
+    FR_HD Cplx operator-(Cplx a, Cplx b) { return {a.re - b.re, a.im - b.im}; }
+    FR_HD Cplx operator-(Cplx a) { return {-a.re, -a.im}; }
+    FR_HD Cplx operator*(Cplx a, Cplx b)
+    {
+        return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+    }
+    FR_HD Cplx operator/(Cplx a, Cplx b)
+    {
+        // Smith's algorithm: scale by the larger component of the divisor.
+        if (::fabs(b.re) >= ::fabs(b.im))
+        {
+            const double r = b.im / b.re, den = b.re + b.im * r;
+            return {(a.re + a.im * r) / den, (a.im - a.re * r) / den};
+        }
+        const double r = b.re / b.im, den = b.re * r + b.im;
+        return {(a.re * r + a.im) / den, (a.im * r - a.re) / den};
+    }
+    FR_HD Cplx operator+(Cplx a, double b) { return {a.re + b, a.im}; }
+    FR_HD Cplx operator+(double a, Cplx b) { return {a + b.re, b.im}; }
+    FR_HD Cplx operator-(Cplx a, double b) { return {a.re - b, a.im}; }
+    FR_HD Cplx operator-(double a, Cplx b) { return {a - b.re, -b.im}; }
+    FR_HD Cplx operator*(Cplx a, double b) { return {a.re * b, a.im * b}; }
+    FR_HD Cplx operator*(double a, Cplx b) { return {a * b.re, a * b.im}; }
+    FR_HD Cplx operator/(Cplx a, double b) { return {a.re / b, a.im / b}; }
+    FR_HD Cplx operator/(double a, Cplx b) { return Cplx{a, 0.0} / b; }
+    FR_HD Cplx sqrt(Cplx z)
+    {
+        // principal square root, cancellation-free in both half planes
+        const double m = ::sqrt(z.re * z.re + z.im * z.im);
+        if (z.re >= 0.0)
+        {
+            const double t = ::sqrt(0.5 * (m + z.re));
+            return {t, (t == 0.0) ? 0.0 : z.im / (2.0 * t)};
+        }
+        const double t = ::sqrt(0.5 * (m - z.re));
+        return {::fabs(z.im) / (2.0 * t), ::copysign(t, z.im)};
+    }
+
+    // ------------- uniform helpers over the three types -------------
+    template <class S>
+    struct Sc;
+    template <>
+    struct Sc<double>
+    {
+        static FR_HD double c(double x) { return x; }
+        static FR_HD double branch(double x) { return x; }
+        static FR_HD double val(double x) { return x; }
+        static FR_HD double wrap_positive(double x) { return (DBL_MIN < x) ? x : DBL_MIN; }
+        static FR_HD double frozen(double x) { return x; }
+    };
+    template <>
+    struct Sc<Dual>
+    {
+        static FR_HD Dual c(double x) { return {x, 0.0}; }
+        static FR_HD double branch(Dual x) { return x.v; }
+        static FR_HD double val(Dual x) { return x.v; }
+        static FR_HD Dual wrap_positive(Dual x) { return (DBL_MIN < x.v) ? x : Dual{DBL_MIN, 0.0}; }
+        static FR_HD Dual frozen(Dual x) { return {x.v, 0.0}; }
+    };
+    template <>
+    struct Sc<Cplx>
+    {
+        static FR_HD Cplx c(double x) { return {x, 0.0}; }
+        static FR_HD double branch(Cplx x) { return x.re + x.im; }
+        static FR_HD double val(Cplx x) { return x.re; }
+        static FR_HD Cplx wrap_positive(Cplx x) { return (x.re < DBL_MIN) ? Cplx{DBL_MIN, 0.0} : x; }
+        static FR_HD Cplx frozen(Cplx x) { return {x.re, 0.0}; }
+    };
+    FR_HD double sqrt_(double x) { return ::sqrt(x); }
+    FR_HD Dual sqrt_(Dual x) { return sqrt(x); }
+    FR_HD Cplx sqrt_(Cplx x) { return sqrt(x); }
+} // namespace frsdg

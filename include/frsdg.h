@@ -1,0 +1,141 @@
+/* frsdg.h — C-ABI of the MI355X FR residual library (libfrsdg.so).
+ *
+ * Drop-in replacement for the residual callbacks that the reference
+ * (mvtteoleo/fr_sdg) registers with PETSc TS/SNES.  All state arrays are
+ * DEVICE pointers (HBM-resident), in the reference's natural DMDA layout:
+ *   2D: element (i,j) is the contiguous block starting at (j*nx + i)*4*n*n,
+ *       node (k,l) and equation eq at offset ((l*n + k)*4 + eq)
+ *       (include/physics_impl_2D.hpp:27-35 Indexer);
+ *   1D: element e at e*3*n, equation/node at (eq*n + node)
+ *       (include/physics_impl_1D.hpp:19-24).
+ * n = p+1.  Arrays must be 16-byte aligned.  Owned elements only (no ghosts):
+ * periodic wrap and boundary states are resolved inside.
+ *
+ * Every entry point returns 0 (FRSDG_OK) on success and a nonzero code
+ * otherwise (mirrors PetscErrorCode; nothing throws across the ABI).  One
+ * context is not reentrant: do not call it concurrently from two threads.
+ * `stream` is a hipStream_t (NULL = default stream); calls are asynchronous
+ * on that stream unless stated otherwise.
+ */
+#ifndef FRSDG_H
+#define FRSDG_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FRSDG_OK 0
+#define FRSDG_ERR_INVALID 1     /* bad descriptor / argument */
+#define FRSDG_ERR_HIP 2         /* HIP runtime error */
+#define FRSDG_ERR_UNSUPPORTED 3 /* valid but not implemented on this path */
+#define FRSDG_ERR_ALIGN 4       /* pointer not 16-byte aligned */
+
+/* boundary types: values of numPDE::BCType (boundary_conditions.hpp:8-16) plus
+ * FRSDG_BC_SLIPWALL, the curvilinear slip wall of tests/bump.cpp:250-272 */
+#define FRSDG_BC_WALL 0
+#define FRSDG_BC_OUTFLOW 1
+#define FRSDG_BC_DIRICHLET 2
+#define FRSDG_BC_PERIODIC 5
+#define FRSDG_BC_SLIPWALL 6
+
+/* geometry models (geometry_models.hpp:72-195, tests/bump.cpp:13-171) */
+#define FRSDG_GEOM_CARTESIAN 0
+#define FRSDG_GEOM_DELAVAL 1 /* GeometryDeLavalNozzle2D with StraightLineProfile */
+#define FRSDG_GEOM_WEDGE 2   /* GeometryCurvedWedge2D (tests/bump.cpp) */
+#define FRSDG_GEOM_BUMP 3    /* channel bump: param {A, x0, xc, w} */
+#define FRSDG_GEOM_WARPED 4  /* sine warp: param {A, x0, y0, -} */
+
+typedef struct frsdg_ctx frsdg_ctx;
+
+/* Problem description: the ConfigOptions / ProblemTraits / BoundaryConditions
+ * content of the reference (parser.hpp:45-191, physics_evaluator.hpp:37-52,
+ * boundary_conditions.hpp:214-324). */
+typedef struct frsdg_desc
+{
+    int    dims;       /* 1 or 2 */
+    int    nx, ny;     /* -n_elems per axis */
+    int    p;          /* -poly_deg, 1..8 (reference_element.hpp:24, MaxNumElem = 9) */
+    double lx, ly;     /* -box_l per axis */
+    int    gas_kind;   /* -gas_mod_law: 0 ideal, 1 van der Waals */
+    double gamma, R_spec, vdw_a, vdw_b;
+    int    riemann;    /* 0 Rusanov, 1 HLLC (riemann_solver.hpp:13-17) */
+    int    geometry;   /* FRSDG_GEOM_* */
+    double geom_param[4];
+    int    bc_type[4]; /* Left, Right, Top, Bottom (boundary_conditions.hpp:17-23) */
+    double bc_state[4][4]; /* Dirichlet states (rho, rhoU, rhoV, rhoE) */
+    double c_vcjh;     /* -radau_corr */
+    int    apply_filter, filt_cutoff, filt_strength; /* -apply_filter -filt_cut_off_deg -filt_strength */
+    double filt_alpha;
+    int    jac_type;   /* MatShell J*v: 0 forward tangent, 1 complex step (-jac_type complexfd) */
+    int    tangent_bc_mode; /* 0 exact linearisation of lambda BCs; 1 reference-compatible:
+                               lambda-bound BCs frozen as complex step sees them
+                               (boundary_conditions.hpp:270-294) */
+} frsdg_desc;
+
+/* PhysicsEvaluator ctor (physics_evaluator.hpp:188-201) + the once-only
+ * precompute of GLL/D/VCJH/filter tables and curved-mesh metrics in HBM. */
+int  frsdg_create(const frsdg_desc* d, frsdg_ctx** out);
+int  frsdg_destroy(frsdg_ctx* ctx);
+long long frsdg_ndofs(const frsdg_ctx* ctx);
+/* host copies of the setup tables (GLL csi/w, D (n*n row-major), dL, dR, filter (n*n)) */
+int  frsdg_get_tables(const frsdg_ctx* ctx, double* csi, double* w, double* D, double* dL,
+                      double* dR, double* filt);
+
+/* residual_kernel<double> (physics_impl_2D.hpp:644-902, physics_impl_1D.hpp:8-94):
+ * R = +div F (reference sign). */
+int frsdg_residual(frsdg_ctx* ctx, double t, const double* u, double* R, void* stream);
+/* rhs_func (petsc_solver_impl.hpp:657-670): rhs = -R(u). */
+int frsdg_rhs(frsdg_ctx* ctx, double t, const double* u, double* rhs, void* stream);
+/* fun_implicit (petsc_solver_impl.hpp:626-655): F = udot + R(u). */
+int frsdg_ifunction(frsdg_ctx* ctx, double t, const double* u, const double* udot, double* F,
+                    void* stream);
+
+/* Forward tangent: out = sigma*v + (dR/du)(u) v.  Replaces the complex-step
+ * MatShell product (petsc_solver_impl.hpp:270-359) with an exact J*v. */
+int frsdg_jvp(frsdg_ctx* ctx, double t, const double* u, const double* v, double sigma, double* out,
+              void* stream);
+/* Complex-step product of petsc_solver_impl.hpp:270-359 on the GPU:
+ * out = sigma*Im(u+ihv)/h + Im R(u+ihv)/h (h = 1e-7 in the reference, :179). */
+int frsdg_jvp_cstep(frsdg_ctx* ctx, double t, const double* u, const double* v, double sigma,
+                    double h, double* out, void* stream);
+/* Adjoint: out = sigma*w + (dR/du)(u)^T w. No reference counterpart (new). */
+int frsdg_vjp(frsdg_ctx* ctx, double t, const double* u, const double* w, double sigma, double* out,
+              void* stream);
+
+/* pre_jv_eval (petsc_solver_impl.hpp:223-248): snapshot of the linearisation
+ * point (u stays owned by the caller and must stay valid), sigma and time. */
+int frsdg_pre_jv_eval(frsdg_ctx* ctx, double t, const double* u, double sigma);
+/* matmult (petsc_solver_impl.hpp:250-267): out = sigma*v + J v at the snapshot,
+ * by forward tangent (jac_type 0) or complex step with h = 1e-7 (jac_type 1). */
+int frsdg_matmult(frsdg_ctx* ctx, const double* v, double* out, void* stream);
+
+/* limit_solution with the spectral filter (physics_impl_2D.hpp:76-124,
+ * physics_impl_1D.hpp:96-121, filter.hpp:425-445,714-755).  out may equal u. */
+int frsdg_filter(frsdg_ctx* ctx, const double* u, double* out, void* stream);
+/* apply_stabilization (petsc_solver_impl.hpp:683-702): filter u in place if
+ * the filter is active, no-op otherwise. */
+int frsdg_apply_stabilization(frsdg_ctx* ctx, double* u, void* stream);
+
+/* One explicit step u <- S(u) with rhs = -R: stages 3 = Shu-Osher SSPRK(3,3),
+ * stages 4 = SSPRK(4,3) (PETSc TSSSP rks3 default), each stage fused with its
+ * residual; post-step filter (TSSetPostStep, petsc_solver_impl.hpp:604-605). */
+int frsdg_ssprk_step(frsdg_ctx* ctx, double t, double dt, double* u, int stages, void* stream);
+/* nsteps steps of frsdg_ssprk_step, captured once into a hipGraph and replayed. */
+int frsdg_ssprk_run(frsdg_ctx* ctx, double t, double dt, double* u, int stages, int nsteps,
+                    void* stream);
+
+/* impose_ic (physics_impl_2D.hpp:156-201, physics_impl_1D.hpp:207-244) for the
+ * presets of factory.hpp: kind 0 uniform (rho,u,v,p), 1 isentropic vortex
+ * (u0,v0,beta,r_v,l), 2 circular explosion (rhoL,pL,rhoR,pR,radius,xc,yc),
+ * 3 Sod 1D (rhoL,uL,pL,rhoR,uR,pR).  prm is a host array of 8 doubles. */
+int frsdg_impose_ic(frsdg_ctx* ctx, int kind, const double* prm, double* u, void* stream);
+/* compute_error (physics_impl_2D.hpp:203-265) against the same presets at time
+ * t; out is a HOST array of 4 doubles.  Synchronises the stream. */
+int frsdg_l2_error(frsdg_ctx* ctx, int kind, const double* prm, double t, const double* u,
+                   double* out, void* stream);
+
+const char* frsdg_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,294 @@
+"""Pins of the oracle's pointwise physics and residual by identities the
+mathematics fixes (the reference ships no golden vectors: SURVEY 8c).
+
+* Riemann solvers: consistency F*(U,U,n) = F(U).n, rotational invariance,
+  conservation F*(UL,UR,n) = -F*(UR,UL,-n), supersonic upwinding.
+* Gas models: van der Waals with a = b = 0 reduces to the ideal gas.
+* Residual: free-stream preservation on affine and curved maps, discrete
+  conservation on periodic meshes (sum of w_k w_l R = 0) for any VCJH c,
+  exact divergence of a polynomial flux, 1D <-> y-uniform 2D equivalence,
+  complex step <-> central differences.
+* Sod: exact Riemann star state, and the full config-1 run against the exact
+  solution.
+"""
+import numpy as np
+import pytest
+
+import cases as C
+import exact_riemann as ER
+from pyoracle import Oracle
+
+rng = np.random.default_rng(C.SEED)
+
+
+def state2(rho, u, v, p, g=1.4):
+    return np.array([rho, rho * u, rho * v, p / (g - 1) + 0.5 * rho * (u * u + v * v)])
+
+
+def flux_n(o, U, nx, ny):
+    F, G = o.flux2d(U)
+    return F * nx + G * ny
+
+
+@pytest.mark.parametrize("riemann", [0, 1])
+@pytest.mark.parametrize("gas", [0, 1])
+def test_riemann_identities(riemann, gas):
+    cfg = C.explosion_cfg(2, 2, 2, gas=gas, riemann=riemann)
+    o = Oracle(cfg)
+    for _ in range(20):
+        th = rng.uniform(0, 2 * np.pi)
+        nx, ny = np.cos(th), np.sin(th)
+        UL = state2(rng.uniform(.5, 1.5), rng.uniform(-.5, .5), rng.uniform(-.5, .5), rng.uniform(.5, 1.5))
+        UR = state2(rng.uniform(.5, 1.5), rng.uniform(-.5, .5), rng.uniform(-.5, .5), rng.uniform(.5, 1.5))
+        if gas:  # make energies consistent with the VdW gas
+            UL[3] = C.rhoE_of(cfg, UL[0], (UL[1] ** 2 + UL[2] ** 2) / UL[0] ** 2, 1.0)
+            UR[3] = C.rhoE_of(cfg, UR[0], (UR[1] ** 2 + UR[2] ** 2) / UR[0] ** 2, 0.7)
+        # consistency
+        assert np.allclose(o.riemann2d(UL, UL, nx, ny), flux_n(o, UL, nx, ny), rtol=0, atol=1e-13)
+        # conservation / antisymmetry
+        a = o.riemann2d(UL, UR, nx, ny)
+        b = o.riemann2d(UR, UL, -nx, -ny)
+        assert np.allclose(a, -b, rtol=0, atol=1e-13)
+        # rotational invariance: rotate states and normal by phi
+        ph = rng.uniform(0, 2 * np.pi)
+        c, s = np.cos(ph), np.sin(ph)
+
+        def rot(U):
+            return np.array([U[0], c * U[1] - s * U[2], s * U[1] + c * U[2], U[3]])
+        r = o.riemann2d(rot(UL), rot(UR), c * nx - s * ny, s * nx + c * ny)
+        assert np.allclose(r, rot(a), rtol=0, atol=1e-13)
+
+
+def test_hllc_supersonic_upwinding():
+    o = Oracle(C.explosion_cfg(2, 2, 2, riemann=1))
+    UL = state2(1.0, 3.0, 0.2, 1.0)   # M ~ 2.5 to the right: sL > 0
+    UR = state2(0.5, 2.5, -0.1, 0.8)
+    assert np.allclose(o.riemann2d(UL, UR, 1.0, 0.0), flux_n(o, UL, 1.0, 0.0), rtol=0, atol=1e-14)
+    UL2 = state2(1.0, -3.0, 0.2, 1.0)
+    UR2 = state2(0.5, -2.5, -0.1, 0.8)
+    assert np.allclose(o.riemann2d(UL2, UR2, 1.0, 0.0), flux_n(o, UR2, 1.0, 0.0), rtol=0, atol=1e-14)
+
+
+def test_vdw_reduces_to_ideal():
+    ideal = Oracle(C.explosion_cfg(2, 2, 2))
+    vdw0 = Oracle(dict(C.explosion_cfg(2, 2, 2), gas_kind=1, vdw_a=0.0, vdw_b=0.0))
+    UL, UR = state2(1.0, 0.3, -0.2, 1.0), state2(0.5, -0.1, 0.2, 0.4)
+    assert np.allclose(ideal.riemann2d(UL, UR, 0.6, 0.8), vdw0.riemann2d(UL, UR, 0.6, 0.8), rtol=1e-14, atol=0)
+    # closed-form VdW pressure: p = (g-1) rho (e + a rho) / (1 - b rho) - a rho^2
+    cfg = C.explosion_cfg(2, 2, 2, gas=1)
+    o = Oracle(cfg)
+    rho, e = 0.9, 2.1
+    U = np.array([rho, 0.0, 0.0, rho * e])
+    F, G = o.flux2d(U)
+    p = 0.4 * rho * (e + 0.1 * rho) / (1 - 0.02 * rho) - 0.1 * rho * rho
+    assert abs(F[1] - p) <= 1e-15 and abs(G[2] - p) <= 1e-15
+
+
+@pytest.mark.parametrize("cfg", [C.vortex_cfg(4, 3, 4), C.bump_cfg(6, 3, 3), C.warped_cfg(4, 4, 5, filt=0),
+                                 dict(C.explosion_cfg(3, 3, 2), geometry=2, lx=1.0, ly=1.0),
+                                 dict(C.explosion_cfg(3, 3, 3), geometry=1, lx=1.0, ly=1.0)],
+                         ids=["affine", "bump", "warped", "wedge", "delaval"])
+def test_free_stream_preservation(cfg):
+    o = Oracle(dict(cfg, bc_type=[1, 1, 1, 1]))
+    u = o.impose_ic(0, [1.2, 0.3, -0.2, 0.9])
+    R = o.residual(u)
+    assert np.max(np.abs(R)) <= 1e-10 * max(cfg["nx"], cfg["ny"]) * (cfg["p"] + 1) ** 2
+
+
+@pytest.mark.parametrize("c_vcjh", [0.0, 0.01])
+@pytest.mark.parametrize("riemann", [0, 1])
+def test_discrete_conservation_periodic(c_vcjh, riemann):
+    cfg = dict(C.vortex_cfg(5, 4, 3), c_vcjh=c_vcjh, riemann=riemann)
+    o = Oracle(cfg)
+    u = C.random_state(cfg)
+    R = o.residual(u).reshape(4, 5, 4, 4, 4)  # [j][i][l][k][eq]
+    w = o.tables()["w"]
+    W = np.outer(w, w)
+    tot = np.einsum("jilke,lk->e", R, W)
+    assert np.max(np.abs(tot)) <= 1e-12 * np.abs(R).sum() / R.size * 100
+
+
+def test_polynomial_flux_divergence():
+    """rho, u, v constant and p linear in x: F is linear in x, so R = dF/dx exactly,
+    with vanishing jumps at element faces (Riemann consistency)."""
+    cfg = dict(C.vortex_cfg(3, 2, 3), lx=3.0, ly=2.0, bc_type=[1, 1, 1, 1])
+    o = Oracle(cfg)
+    n = 4
+    csi = o.tables()["csi"]
+    rho, uu, vv, g = 1.1, 0.4, 0.2, 1.4
+    a, b = 1.0, 0.05
+    u = np.zeros(o.ndofs).reshape(2, 3, n, n, 4)
+    for j in range(2):
+        for i in range(3):
+            for l in range(n):
+                for k in range(n):
+                    x = 1.0 * (i + 0.5 * (1 + csi[k]))
+                    u[j, i, l, k] = state2(rho, uu, vv, a + b * x)
+    R = o.residual(u.reshape(-1)).reshape(2, 3, n, n, 4)
+    # dF/dx: mass 0, x-mom dp/dx = b, y-mom 0, energy u (dE/dx + dp/dx) = u b (1/(g-1) + 1)
+    exact = np.array([0.0, b, 0.0, uu * b * (1.0 / (g - 1) + 1.0)])
+    # interior elements only (outflow faces are consistent too, so all elements)
+    assert np.max(np.abs(R - exact)) <= 1e-12
+
+
+def test_1d_vs_y_uniform_2d():
+    """A y-uniform 2D state on a periodic-in-y mesh has the 1D residual in x."""
+    p, nx = 3, 6
+    c1 = dict(dims=1, nx=nx, p=p, lx=2.0, riemann=1, bc_type=[1, 1, 1, 1])
+    c2 = dict(dims=2, nx=nx, ny=3, p=p, lx=2.0, ly=1.0, riemann=1, bc_type=[1, 1, 5, 5])
+    o1, o2 = Oracle(c1), Oracle(c2)
+    n = p + 1
+    u1 = C.random_state(c1).reshape(nx, 3, n)  # [e][eq][node]
+    u2 = np.zeros((3, nx, n, n, 4))
+    for l in range(n):
+        u2[:, :, l, :, 0] = u1[None, :, 0, :]
+        u2[:, :, l, :, 1] = u1[None, :, 1, :]
+        u2[:, :, l, :, 3] = u1[None, :, 2, :]
+    R1 = o1.residual(u1.reshape(-1)).reshape(nx, 3, n)
+    R2 = o2.residual(u2.reshape(-1)).reshape(3, nx, n, n, 4)
+    for l in range(n):
+        assert np.max(np.abs(R2[:, :, l, :, 0] - R1[None, :, 0, :])) <= 1e-12
+        assert np.max(np.abs(R2[:, :, l, :, 1] - R1[None, :, 1, :])) <= 1e-12
+        assert np.max(np.abs(R2[:, :, l, :, 3] - R1[None, :, 2, :])) <= 1e-12
+        assert np.max(np.abs(R2[:, :, l, :, 2])) <= 1e-12
+
+
+@pytest.mark.parametrize("cfg", [C.vortex_cfg(3, 3, 3), C.warped_cfg(3, 3, 3, filt=0), C.sod_cfg(12, 3, 1)],
+                         ids=["vortex", "warped", "sod1d"])
+def test_complex_step_vs_central_difference(cfg):
+    o = Oracle(cfg)
+    u = C.random_state(cfg)
+    v = C.random_state(cfg, seed=3) - C.random_state(cfg, seed=4)
+    jv = o.jvp_cstep(u, v)
+    eps = 1e-6
+    fd = (o.residual(u + eps * v) - o.residual(u - eps * v)) / (2 * eps)
+    assert C.rel_l2(jv, fd) <= 1e-6
+    # sigma term: out = sigma v + J v
+    jv2 = o.jvp_cstep(u, v, sigma=2.0)
+    assert C.rel_l2(jv2 - jv, 2.0 * v) <= 1e-12
+
+
+def test_sod_star_state():
+    ps, us, rsL, rsR = ER.star_state(1.0, 0.0, 1.0, 0.125, 0.0, 0.1)
+    assert abs(ps - 0.30313) < 1e-5 and abs(us - 0.92745) < 1e-5
+    assert abs(rsL - 0.42632) < 1e-5 and abs(rsR - 0.26557) < 1e-5
+
+
+def sod_run(o, steps_dt):
+    u = o.impose_ic(3, [1.0, 0.0, 1.0, 0.125, 0.0, 0.1])
+    t = 0.0
+    for dt in steps_dt:
+        u = o.ssprk_step(u, dt, t=t, stages=3)
+        t += dt
+    return u, t
+
+
+def sod_steps(t_end=0.2, h=0.005):
+    dt = 0.1 * h / np.sqrt(1.4)
+    n_full = int(np.floor(t_end / dt))
+    steps = [dt] * n_full
+    rem = t_end - n_full * dt
+    if rem > 1e-14:
+        steps.append(rem)  # TS_EXACTFINALTIME_MATCHSTEP
+    return steps
+
+
+def sod_l2_vs_exact(u, cfg, t):
+    n = cfg["p"] + 1
+    o = Oracle(cfg)
+    tb = o.tables()
+    h = cfg["lx"] / cfg["nx"]
+    x = np.array([[h * (e + 0.5 * (1 + c)) for c in tb["csi"]] for e in range(cfg["nx"])])
+    r, uu, p = ER.sample(1.0, 0.0, 1.0, 0.125, 0.0, 0.1, t, x)
+    U = u.reshape(cfg["nx"], 3, n)
+    wq = 0.5 * h * tb["w"][None, :]
+    return np.sqrt(np.sum(wq * (U[:, 0, :] - r) ** 2))
+
+
+SOD_FILTERED = dict(apply_filter=1, filt_cutoff=0, filt_strength=8, filt_alpha=36.0)
+
+
+def test_sod_plain_method_diverges():
+    """The reference's Sod set-up (factory.hpp:196-214, pos < L/2) puts the jump
+    on the last node of element 99; the unstabilised scheme then produces a
+    negative pressure within a few steps for both fluxes and both RK forms.
+    The oracle reproduces that (the restatement has no positivity guard)."""
+    for riemann in (0, 1):
+        for stages in (3, 4):
+            o = Oracle(C.sod_cfg(riemann=riemann))
+            u = o.impose_ic(3, [1.0, 0.0, 1.0, 0.125, 0.0, 0.1])
+            for k, dt in enumerate(sod_steps()[:10]):
+                u = o.ssprk_step(u, dt, stages=stages)
+                if not np.isfinite(u).all():
+                    break
+            assert not np.isfinite(u).all()
+
+
+def test_sod_full_run_vs_exact():
+    """Config 1 geometry/time stepping (200 elements, p=3, CFL 0.1, t=0.2 with
+    MATCHSTEP, 474 steps, SSPRK(4,3) = PETSc rks3) with the spectral filter as
+    post-step stabiliser (s=8, alpha=36, n_c=0) and HLLC: runs to the end and
+    its density error against the exact Riemann solution is small."""
+    cfg = dict(C.sod_cfg(riemann=1), **SOD_FILTERED)
+    o = Oracle(cfg)
+    steps = sod_steps()
+    assert len(steps) == 474
+    u = o.impose_ic(3, [1.0, 0.0, 1.0, 0.125, 0.0, 0.1])
+    t = 0.0
+    for dt in steps:
+        u = o.ssprk_step(u, dt, t=t, stages=4)
+        t += dt
+    assert abs(t - 0.2) < 1e-14
+    assert np.isfinite(u).all()
+    err = sod_l2_vs_exact(u, cfg, t)
+    assert err < 0.01, err
+
+
+def entropy_wave_error(N, p, t_end=0.1):
+    """1D entropy wave rho = 1 + 0.2 sin(2 pi x), u = 1, p = 1 on a periodic unit
+    interval: the exact solution is a translation."""
+    cfg = dict(dims=1, nx=N, p=p, lx=1.0, riemann=1, bc_type=[5, 5, 5, 5])
+    o = Oracle(cfg)
+    tb = o.tables()
+    h = 1.0 / N
+    x = np.array([[h * (e + 0.5 * (1 + c)) for c in tb["csi"]] for e in range(N)])
+
+    def exact(t):
+        rho = 1 + 0.2 * np.sin(2 * np.pi * (x - t))
+        return np.stack([rho, rho * 1.0, 1.0 / 0.4 + 0.5 * rho], axis=1)
+    u = exact(0.0).reshape(-1)
+    nst = int(np.ceil(t_end / (0.05 * h)))
+    dt = t_end / nst
+    for _ in range(nst):
+        u = o.ssprk_step(u, dt, stages=3)
+    ex = exact(t_end)
+    U = u.reshape(N, 3, p + 1)
+    return np.sqrt(np.sum(0.5 * h * tb["w"][None, :] * (U[:, 0, :] - ex[:, 0, :]) ** 2))
+
+
+@pytest.mark.parametrize("p", [2, 3])
+def test_entropy_wave_convergence(p):
+    e1, e2 = entropy_wave_error(8, p), entropy_wave_error(16, p)
+    rate = np.log2(e1 / e2)
+    assert rate > p + 0.5, (e1, e2, rate)
+
+
+def vortex_error(N, p, t_end=0.5):
+    cfg = dict(C.vortex_cfg(N, N, p), riemann=1)
+    o = Oracle(cfg)
+    prm = [1.0, 1.0, 5.0, 1.0, 10.0]
+    u = o.impose_ic(1, prm)
+    h = 10.0 / N
+    nst = int(np.ceil(t_end / (0.1 * h / 3.0)))
+    dt = t_end / nst
+    for _ in range(nst):
+        u = o.ssprk_step(u, dt, stages=3)
+    return o.l2_error(1, prm, u, t=t_end)[0]
+
+
+def test_vortex_convergence():
+    """Isentropic vortex (factory.hpp:311-352) against its exact translation
+    (the reference's accuracy test, tests/test_2d.cpp): the density
+    error drops at least like h^3 from 8^2 to 16^2 elements at p = 3."""
+    e1, e2 = vortex_error(8, 3), vortex_error(16, 3)
+    assert np.log2(e1 / e2) > 3.0, (e1, e2)
