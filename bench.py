@@ -36,6 +36,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-large", action="store_true", help="skip the large curved residual")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--kernel-only", choices=["stage", "residual", "tangent", "cstep", "adjoint", "large",
+                                                "copy"], default=None,
+                    help="profiling mode: launch only this kernel `--steps` times, print nothing else")
     return ap.parse_args()
 
 
@@ -75,6 +78,8 @@ def main():
     import fr_sdg_amd as fr
 
     torch.cuda.set_device(0)
+    if args.kernel_only:
+        return kernel_only(torch, fr, args)
     cfg = vortex_cfg()
     ev = fr.PhysicsEvaluator(cfg)
     n = cfg["p"] + 1
@@ -169,6 +174,36 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(line))
+
+
+def kernel_only(torch, fr, args):
+    """Launch one kernel kind back to back (for rocprofv3 counter / trace runs)."""
+    if args.kernel_only == "large":
+        cfg = dict(dims=2, nx=1024, ny=1024, p=7, lx=2.0, ly=2.0, riemann=1, geometry=fr.GEOM_WARPED,
+                   geom_param=[0.05, -1.0, -1.0, 0.0], bc_type=[1, 1, 1, 1], gas_kind=1, vdw_a=0.1,
+                   vdw_b=0.02)
+        prm, kind = [1.0, 1.0, 0.125, 0.1, 0.4, 0.0, 0.0], fr.IC_EXPLOSION
+    else:
+        cfg, prm, kind = vortex_cfg(), VORTEX_PRM, fr.IC_VORTEX
+    ev = fr.PhysicsEvaluator(cfg)
+    u = ev.impose_ic(kind, prm)
+    out = ev.empty()
+    v = torch.randn(ev.ndofs, dtype=torch.float64, device="cuda")
+    k = args.kernel_only
+    fn = {
+        "stage": lambda: ev.ssprk_step(u, 1e-4),
+        "residual": lambda: ev.residual_kernel(u, out=out),
+        "large": lambda: ev.residual_kernel(u, out=out),
+        "tangent": lambda: ev.jvp(u, v, 1.0, out=out),
+        "cstep": lambda: ev.jvp_cstep(u, v, 1.0, 1e-7, out=out),
+        "adjoint": lambda: ev.vjp(u, v, 1.0, out=out),
+        "copy": lambda: out.copy_(u),
+    }[k]
+    for _ in range(args.steps):
+        fn()
+    torch.cuda.synchronize()
+    print(json.dumps({"kernel_only": k, "launches": args.steps, "ndofs": ev.ndofs,
+                      "bytes_per_copy": 16 * ev.ndofs if k == "copy" else None}))
 
 
 def bench_large(torch, fr):

@@ -53,11 +53,33 @@ class FrsdgDesc(ctypes.Structure):
     ]
 
 
+class NewtonOpts(ctypes.Structure):
+    _fields_ = [("jac_type", ctypes.c_int), ("precond", ctypes.c_int), ("snes_rtol", ctypes.c_double),
+                ("snes_atol", ctypes.c_double), ("snes_max_it", ctypes.c_int), ("ksp_restart", ctypes.c_int),
+                ("ksp_max_it", ctypes.c_int), ("ksp_rtol", ctypes.c_double)]
+
+
+class NewtonStats(ctypes.Structure):
+    _fields_ = [("newton_its", ctypes.c_int), ("ksp_its", ctypes.c_int), ("jv_evals", ctypes.c_int),
+                ("res_evals", ctypes.c_int), ("converged", ctypes.c_int), ("fnorm0", ctypes.c_double),
+                ("fnorm", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# PETSc defaults: SNES rtol 1e-8, atol 1e-50, max_it 50; KSP GMRES restart 30, rtol 1e-5, max_it 10000
+NEWTON_DEFAULTS = dict(jac_type=0, precond=0, snes_rtol=1e-8, snes_atol=1e-50, snes_max_it=50, ksp_restart=30,
+                       ksp_max_it=10000, ksp_rtol=1e-5)
+
 EXPORTS = [
     "frsdg_create", "frsdg_destroy", "frsdg_ndofs", "frsdg_get_tables", "frsdg_residual",
     "frsdg_rhs", "frsdg_ifunction", "frsdg_jvp", "frsdg_jvp_cstep", "frsdg_vjp",
     "frsdg_pre_jv_eval", "frsdg_matmult", "frsdg_filter", "frsdg_apply_stabilization",
     "frsdg_ssprk_step", "frsdg_ssprk_run", "frsdg_impose_ic", "frsdg_l2_error", "frsdg_strerror",
+    "frsdg_beuler_step", "frsdg_batch_create", "frsdg_batch_destroy", "frsdg_batch_ndofs",
+    "frsdg_batch_offsets", "frsdg_batch_instance", "frsdg_batch_set_dt", "frsdg_batch_set_tuning",
+    "frsdg_batch_ssprk_step", "frsdg_batch_ssprk_run", "frsdg_batch_ssprk_step_each",
 ]
 
 _lib = None
@@ -92,6 +114,18 @@ def load_library(path=LIB_PATH):
     L.frsdg_impose_ic.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double), dp, vp]
     L.frsdg_l2_error.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double), d, dp,
                                  ctypes.POINTER(ctypes.c_double), vp]
+    L.frsdg_beuler_step.argtypes = [vp, d, d, dp, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats), vp]
+    L.frsdg_batch_create.argtypes = [ctypes.POINTER(FrsdgDesc), i, ctypes.POINTER(ctypes.c_void_p)]
+    L.frsdg_batch_destroy.argtypes = [vp]
+    L.frsdg_batch_ndofs.argtypes = [vp]
+    L.frsdg_batch_ndofs.restype = ctypes.c_longlong
+    L.frsdg_batch_offsets.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
+    L.frsdg_batch_instance.argtypes = [vp, i, ctypes.POINTER(ctypes.c_void_p)]
+    L.frsdg_batch_set_dt.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    L.frsdg_batch_set_tuning.argtypes = [vp, i, i]
+    L.frsdg_batch_ssprk_step.argtypes = [vp, dp, vp]
+    L.frsdg_batch_ssprk_run.argtypes = [vp, dp, i, vp]
+    L.frsdg_batch_ssprk_step_each.argtypes = [vp, ctypes.POINTER(ctypes.c_double), dp, vp]
     L.frsdg_strerror.argtypes = [i]
     L.frsdg_strerror.restype = ctypes.c_char_p
     _lib = L
@@ -249,6 +283,15 @@ class PhysicsEvaluator:
         _check(self.lib.frsdg_ssprk_run(self.h, t, dt, _ptr(u), stages, nsteps, _stream()))
         return u
 
+    def beuler_step(self, u, dt, t=0.0, **opts):
+        """One backward-Euler step solved by JFNK (TSSetIFunction path); u updated in place."""
+        o = dict(NEWTON_DEFAULTS)
+        o.update(opts)
+        no = NewtonOpts(**o)
+        st = NewtonStats()
+        _check(self.lib.frsdg_beuler_step(self.h, t, dt, _ptr(u), ctypes.byref(no), ctypes.byref(st), _stream()))
+        return st.as_dict()
+
     def impose_ic(self, kind, prm, out=None):
         out = self.empty() if out is None else out
         prm = (ctypes.c_double * 8)(*(list(prm) + [0.0] * (8 - len(prm))))
@@ -260,6 +303,82 @@ class PhysicsEvaluator:
         out = (ctypes.c_double * 4)()
         _check(self.lib.frsdg_l2_error(self.h, kind, prm, t, _ptr(u), out, _stream()))
         return list(out)
+
+
+class _InstanceView(PhysicsEvaluator):
+    """Non-owning evaluator over one instance of a Batch."""
+
+    def __init__(self, lib, h, cfg, device):
+        self.lib, self.h, self.cfg, self.device = lib, h, dict(cfg), device
+        self.n = cfg["p"] + 1
+        self.ndofs = int(lib.frsdg_ndofs(h))
+
+    def close(self):
+        self.h = None
+
+    __del__ = close
+
+
+class Batch:
+    """Many independent problems advanced by persistent batched kernels."""
+
+    def __init__(self, cfgs, device="cuda"):
+        if not torch.cuda.is_available():
+            raise FrsdgError("fr_sdg_amd needs a ROCm GPU (no CPU fallback)")
+        self.lib = load_library()
+        self.cfgs = [dict(c) for c in cfgs]
+        self.device = torch.device(device)
+        torch.cuda.init()
+        arr = (FrsdgDesc * len(cfgs))(*[make_desc(c) for c in cfgs])
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _check(self.lib.frsdg_batch_create(arr, len(cfgs), ctypes.byref(h)))
+        self.h = h
+        self.count = len(cfgs)
+        self.ndofs = int(self.lib.frsdg_batch_ndofs(h))
+        off = (ctypes.c_longlong * (self.count + 1))()
+        _check(self.lib.frsdg_batch_offsets(h, off))
+        self.offsets = list(off)
+        self.instances = []
+        for k in range(self.count):
+            ck = ctypes.c_void_p()
+            _check(self.lib.frsdg_batch_instance(h, k, ctypes.byref(ck)))
+            self.instances.append(_InstanceView(self.lib, ck, cfgs[k], self.device))
+
+    def close(self):
+        if getattr(self, "h", None):
+            for iv in self.instances:
+                iv.close()
+            self.lib.frsdg_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def empty(self):
+        return torch.empty(self.ndofs, dtype=torch.float64, device=self.device)
+
+    def view(self, u, k):
+        return u[self.offsets[k]:self.offsets[k + 1]]
+
+    def set_dt(self, dts):
+        self.dts = list(dts)
+        _check(self.lib.frsdg_batch_set_dt(self.h, (ctypes.c_double * self.count)(*self.dts)))
+
+    def set_tuning(self, grid, epb_div):
+        _check(self.lib.frsdg_batch_set_tuning(self.h, grid, epb_div))
+
+    def ssprk_step(self, u):
+        _check(self.lib.frsdg_batch_ssprk_step(self.h, _ptr(u), _stream()))
+        return u
+
+    def ssprk_run(self, u, nsteps):
+        _check(self.lib.frsdg_batch_ssprk_run(self.h, _ptr(u), nsteps, _stream()))
+        return u
+
+    def ssprk_step_each(self, u):
+        _check(self.lib.frsdg_batch_ssprk_step_each(self.h, (ctypes.c_double * self.count)(*self.dts),
+                                                    _ptr(u), _stream()))
+        return u
 
 
 class FRResidual(torch.autograd.Function):

@@ -12,11 +12,11 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libfrsdg.so")
-SOURCES = [os.path.join(HERE, "csrc", "frsdg_capi.hip")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("frsdg_capi.hip", "frsdg_newton.hip", "frsdg_batch.hip")]
 HEADERS = [
     os.path.join(HERE, "csrc", f)
     for f in ("frsdg_scalar.hpp", "frsdg_physics.hpp", "frsdg_geometry.hpp", "frsdg_kernels.hpp",
-              "frsdg_tables.hpp", "frsdg_adjoint.hpp")
+              "frsdg_tables.hpp", "frsdg_adjoint.hpp", "frsdg_ctx.hpp")
 ] + [os.path.join(ROOT, "include", "frsdg.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -32,8 +32,25 @@ FLAGS = [
 
 
 def build(verbose=False):
+    """Compile each translation unit in parallel, then link libfrsdg.so."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    cflags = [f for f in FLAGS if f != "-shared"]
+
+    def compile_one(src):
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        cmd = [HIPCC, *cflags, "-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        return obj
+
+    with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
+        objs = list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, *SOURCES, "-o", tmp]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -1,0 +1,304 @@
+// frsdg_batch.hip — many independent FR problems (uneven mesh size, order and
+// gas model) advanced together: one persistent kernel per SSP-RK stage walks
+// the concatenated (instance, element-tile) work list; each tile dispatches on
+// its instance's n = p+1 to the same tile function the single-problem kernels
+// use.  The last stage carries the post-step spectral filter.  A whole step is
+// 3 launches regardless of the number of instances (vs 3 per instance), and
+// frsdg_batch_ssprk_run replays captured steps from a hipGraph.
+#include "frsdg_ctx.hpp"
+
+#include <vector>
+
+using namespace frsdg;
+
+namespace
+{
+    struct BatchInst
+    {
+        Res2DP    P;
+        long long off;    // first DOF of the instance in the concatenated state
+        long long tile0;  // first tile index of the instance
+        long long ntiles;
+        int       n;
+        int       epb;    // elements per tile
+        int       filt;   // post-step filter active
+        int       pad;
+    };
+
+    union BatchSmem
+    {
+        TileSmem<2, double> s2;
+        TileSmem<3, double> s3;
+        TileSmem<4, double> s4;
+        TileSmem<5, double> s5;
+        TileSmem<6, double> s6;
+        TileSmem<7, double> s7;
+        TileSmem<8, double> s8;
+        TileSmem<9, double> s9;
+    };
+
+    struct StageArgs
+    {
+        const double* us;
+        const double* u0;
+        double*       out;
+        const double* dt; // per-instance time step (device)
+        double        a0, b, cfac;
+        int           last;
+    };
+
+    __global__ __launch_bounds__(256) void k_batch_stage(const BatchInst* __restrict__ tab, int ninst,
+                                                         long long total, StageArgs A)
+    {
+        __shared__ BatchSmem sm;
+        const long long slot = xcd_tile(blockIdx.x, gridDim.x);
+        for (long long t = slot; t < total; t += gridDim.x)
+        {
+            int lo = 0, hi = ninst - 1;
+            while (lo < hi)
+            {
+                const int mid = (lo + hi + 1) >> 1;
+                if (tab[mid].tile0 <= t)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const BatchInst& I   = tab[lo];
+            const IOStage    io  = {A.us + I.off, A.u0 + I.off, A.out + I.off, A.a0, A.b, A.cfac * A.dt[lo]};
+            const long long  blk = t - I.tile0;
+            const bool       flt = A.last && I.filt;
+            switch (I.n)
+            {
+                case 2: res2d_tile<2, IOStage>(I.P, io, blk, I.epb, flt, sm.s2); break;
+                case 3: res2d_tile<3, IOStage>(I.P, io, blk, I.epb, flt, sm.s3); break;
+                case 4: res2d_tile<4, IOStage>(I.P, io, blk, I.epb, flt, sm.s4); break;
+                case 5: res2d_tile<5, IOStage>(I.P, io, blk, I.epb, flt, sm.s5); break;
+                case 6: res2d_tile<6, IOStage>(I.P, io, blk, I.epb, flt, sm.s6); break;
+                case 7: res2d_tile<7, IOStage>(I.P, io, blk, I.epb, flt, sm.s7); break;
+                case 8: res2d_tile<8, IOStage>(I.P, io, blk, I.epb, flt, sm.s8); break;
+                case 9: res2d_tile<9, IOStage>(I.P, io, blk, I.epb, flt, sm.s9); break;
+            }
+            __syncthreads(); // LDS is reused by the next tile
+        }
+    }
+
+    int max_epb(int n)
+    {
+        switch (n)
+        {
+            case 2: return Tile2D<2, double>::EPB;
+            case 3: return Tile2D<3, double>::EPB;
+            case 4: return Tile2D<4, double>::EPB;
+            case 5: return Tile2D<5, double>::EPB;
+            case 6: return Tile2D<6, double>::EPB;
+            case 7: return Tile2D<7, double>::EPB;
+            case 8: return Tile2D<8, double>::EPB;
+            case 9: return Tile2D<9, double>::EPB;
+        }
+        return 1;
+    }
+} // namespace
+
+struct frsdg_batch
+{
+    std::vector<frsdg_ctx*> ctx;
+    std::vector<long long>  off;   // count + 1
+    std::vector<BatchInst>  inst;
+    BatchInst*              d_tab = nullptr;
+    double*                 d_dt  = nullptr;
+    double*                 w1    = nullptr;
+    double*                 w2    = nullptr;
+    long long               total_tiles = 0;
+    long long               nd          = 0;
+    int                     grid        = 0; // persistent workgroups
+    int                     epb_div     = 1;
+    hipGraph_t              graph = nullptr;
+    hipGraphExec_t          gexec = nullptr;
+    double*                 g_u   = nullptr;
+    int                     g_ok  = 0;
+};
+
+namespace
+{
+    int rebuild_tiles(frsdg_batch* b)
+    {
+        long long t = 0;
+        for (auto& I : b->inst)
+        {
+            I.epb    = std::max(1, max_epb(I.n) / b->epb_div);
+            I.tile0  = t;
+            I.ntiles = (I.P.E + I.epb - 1) / I.epb;
+            t += I.ntiles;
+        }
+        b->total_tiles = t;
+        FR_TRY(hipMemcpy(b->d_tab, b->inst.data(), sizeof(BatchInst) * b->inst.size(), hipMemcpyHostToDevice));
+        if (b->gexec) (void) hipGraphExecDestroy(b->gexec);
+        if (b->graph) (void) hipGraphDestroy(b->graph);
+        b->gexec = nullptr;
+        b->graph = nullptr;
+        return FRSDG_OK;
+    }
+
+    int batch_step(frsdg_batch* b, double* u, hipStream_t s)
+    {
+        const int ninst = static_cast<int>(b->inst.size());
+        auto launch     = [&](const double* us, double a0, double bb, double* out, int last)
+        {
+            StageArgs A{us, u, out, b->d_dt, a0, bb, 1.0, last};
+            hipLaunchKernelGGL(k_batch_stage, dim3(b->grid), dim3(256), 0, s, b->d_tab, ninst, b->total_tiles, A);
+            return hip_rc(hipGetLastError());
+        };
+        int rc;
+        if ((rc = launch(u, 0.0, 1.0, b->w1, 0))) return rc;
+        if ((rc = launch(b->w1, 0.75, 0.25, b->w2, 0))) return rc;
+        return launch(b->w2, 1.0 / 3.0, 2.0 / 3.0, u, 1);
+    }
+} // namespace
+
+extern "C" {
+
+int frsdg_batch_create(const frsdg_desc* descs, int count, frsdg_batch** out)
+{
+    if (!descs || count < 1 || !out) return FRSDG_ERR_INVALID;
+    *out   = nullptr;
+    auto* b = new frsdg_batch{};
+    auto fail = [&](int rc)
+    {
+        frsdg_batch_destroy(b);
+        return rc;
+    };
+    b->off.push_back(0);
+    for (int k = 0; k < count; ++k)
+    {
+        if (descs[k].dims != 2) return fail(FRSDG_ERR_UNSUPPORTED);
+        frsdg_ctx* c = nullptr;
+        int rc = frsdg_create(&descs[k], &c);
+        if (rc) return fail(rc);
+        b->ctx.push_back(c);
+        BatchInst I{};
+        I.P    = c->p2;
+        I.off  = b->off.back();
+        I.n    = c->n;
+        I.filt = descs[k].apply_filter;
+        b->inst.push_back(I);
+        b->off.push_back(b->off.back() + c->nd);
+    }
+    b->nd = b->off.back();
+    if (hipMalloc(&b->d_tab, sizeof(BatchInst) * count) != hipSuccess) return fail(FRSDG_ERR_HIP);
+    if (hipMalloc(&b->d_dt, sizeof(double) * count) != hipSuccess) return fail(FRSDG_ERR_HIP);
+    if (hipMalloc(&b->w1, sizeof(double) * b->nd) != hipSuccess) return fail(FRSDG_ERR_HIP);
+    if (hipMalloc(&b->w2, sizeof(double) * b->nd) != hipSuccess) return fail(FRSDG_ERR_HIP);
+    int dev = 0, ncu = 256;
+    (void) hipGetDevice(&dev);
+    (void) hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    b->grid = 4 * ncu;
+    int rc = rebuild_tiles(b);
+    if (rc) return fail(rc);
+    *out = b;
+    return FRSDG_OK;
+}
+
+int frsdg_batch_destroy(frsdg_batch* b)
+{
+    if (!b) return FRSDG_OK;
+    if (b->gexec) (void) hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void) hipGraphDestroy(b->graph);
+    for (auto* c : b->ctx) frsdg_destroy(c);
+    for (void* p : {static_cast<void*>(b->d_tab), static_cast<void*>(b->d_dt), static_cast<void*>(b->w1),
+                    static_cast<void*>(b->w2)})
+        if (p) (void) hipFree(p);
+    delete b;
+    return FRSDG_OK;
+}
+
+long long frsdg_batch_ndofs(const frsdg_batch* b) { return b ? b->nd : -1; }
+
+int frsdg_batch_offsets(const frsdg_batch* b, long long* off)
+{
+    if (!b || !off) return FRSDG_ERR_INVALID;
+    for (size_t k = 0; k < b->off.size(); ++k) off[k] = b->off[k];
+    return FRSDG_OK;
+}
+
+int frsdg_batch_instance(frsdg_batch* b, int k, frsdg_ctx** out)
+{
+    if (!b || !out || k < 0 || k >= static_cast<int>(b->ctx.size())) return FRSDG_ERR_INVALID;
+    *out = b->ctx[k];
+    return FRSDG_OK;
+}
+
+int frsdg_batch_set_tuning(frsdg_batch* b, int grid, int epb_div)
+{
+    if (!b || grid < 1 || epb_div < 1 || epb_div > 64) return FRSDG_ERR_INVALID;
+    b->grid    = grid;
+    b->epb_div = epb_div;
+    return rebuild_tiles(b);
+}
+
+int frsdg_batch_set_dt(frsdg_batch* b, const double* dt)
+{
+    if (!b || !dt) return FRSDG_ERR_INVALID;
+    FR_TRY(hipMemcpy(b->d_dt, dt, sizeof(double) * b->inst.size(), hipMemcpyHostToDevice));
+    return FRSDG_OK;
+}
+
+int frsdg_batch_ssprk_step(frsdg_batch* b, double* u, void* stream)
+{
+    if (!b || !u) return FRSDG_ERR_INVALID;
+    if (!aligned(u)) return FRSDG_ERR_ALIGN;
+    return batch_step(b, u, static_cast<hipStream_t>(stream));
+}
+
+int frsdg_batch_ssprk_run(frsdg_batch* b, double* u, int nsteps, void* stream)
+{
+    if (!b || !u || nsteps < 0) return FRSDG_ERR_INVALID;
+    if (!aligned(u)) return FRSDG_ERR_ALIGN;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    constexpr int CH = 8; // steps per captured graph
+    if (!b->gexec || b->g_u != u)
+    {
+        if (b->gexec) (void) hipGraphExecDestroy(b->gexec);
+        if (b->graph) (void) hipGraphDestroy(b->graph);
+        b->gexec = nullptr;
+        b->graph = nullptr;
+        hipStream_t cs;
+        FR_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        {
+            (void) hipStreamDestroy(cs);
+            return FRSDG_ERR_HIP;
+        }
+        int rr = FRSDG_OK;
+        for (int k = 0; k < CH && !rr; ++k) rr = batch_step(b, u, cs);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(cs, &g);
+        (void) hipStreamDestroy(cs);
+        if (rr || ec != hipSuccess) return rr ? rr : FRSDG_ERR_HIP;
+        b->graph = g;
+        FR_TRY(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+        b->g_u = u;
+    }
+    for (int k = 0; k < nsteps / CH; ++k) FR_TRY(hipGraphLaunch(b->gexec, s));
+    for (int k = 0; k < nsteps % CH; ++k)
+    {
+        int rc = batch_step(b, u, s);
+        if (rc) return rc;
+    }
+    return FRSDG_OK;
+}
+
+// Reference point for the batched sweep: the same step with one set of
+// launches per instance (3 stage launches each), using each instance's own
+// context and work buffers.
+int frsdg_batch_ssprk_step_each(frsdg_batch* b, const double* dt, double* u, void* stream)
+{
+    if (!b || !u || !dt) return FRSDG_ERR_INVALID;
+    for (size_t k = 0; k < b->ctx.size(); ++k)
+    {
+        int rc = frsdg_ssprk_step(b->ctx[k], 0.0, dt[k], u + b->off[k], 3, stream);
+        if (rc) return rc;
+    }
+    return FRSDG_OK;
+}
+
+} // extern "C"

@@ -1,8 +1,7 @@
 // frsdg_capi.hip — context, setup kernels and the extern "C" entry points of
 // libfrsdg.so (declared in include/frsdg.h).
-#include "../../include/frsdg.h"
-#include "frsdg_kernels.hpp"
-#include "frsdg_tables.hpp"
+#include "frsdg_adjoint.hpp"
+#include "frsdg_ctx.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -13,48 +12,8 @@
 
 using namespace frsdg;
 
-struct frsdg_ctx
-{
-    frsdg_desc d;
-    int        n   = 0;
-    long long  E   = 0;
-    long long  nd  = 0; // number of DOFs
-    Tables     tb;
-    double*    d_tab  = nullptr; // D, dL, dR, filt
-    double*    d_vmet = nullptr;
-    double*    d_fxm  = nullptr;
-    double*    d_fym  = nullptr;
-    double*    d_csi  = nullptr;
-    Res2DP     p2{};
-    Res1DP     p1{};
-    double*    w1 = nullptr; // RK work states
-    double*    w2 = nullptr;
-    double*    red = nullptr; // reduction partials
-    // pre_jv_eval snapshot
-    const double* jv_u     = nullptr;
-    double        jv_sigma = 0.0;
-    double        jv_t     = 0.0;
-    // graph cache for ssprk_run
-    hipGraphExec_t gexec  = nullptr;
-    hipGraph_t     graph  = nullptr;
-    double*        g_u    = nullptr;
-    double         g_dt   = 0.0;
-    int            g_st   = 0;
-    int            g_n    = 0;
-};
-
 namespace
 {
-    inline int hip_rc(hipError_t e) { return e == hipSuccess ? FRSDG_OK : FRSDG_ERR_HIP; }
-#define FR_TRY(x)                                                                                  \
-    do                                                                                             \
-    {                                                                                              \
-        hipError_t _e = (x);                                                                       \
-        if (_e != hipSuccess) return FRSDG_ERR_HIP;                                                \
-    } while (0)
-
-    inline bool aligned(const void* p) { return (reinterpret_cast<std::uintptr_t>(p) & 15u) == 0; }
-
     // ---------------- setup kernels ----------------
     // per-node composite metrics (physics_evaluator.hpp:235-313)
     __global__ void k_geom_vol(GeomP g, int nx, int ny, int n, const double* __restrict__ csi,
@@ -364,6 +323,47 @@ namespace
         }
     };
 
+    template <int N>
+    struct Adj2
+    {
+        static int run(const frsdg_ctx* c, const double* u, const double* w, double sigma, double* out,
+                       hipStream_t s)
+        {
+            const long long nl = c->E * 4 * N;
+            hipLaunchKernelGGL((k_adj_lift<N>), dim3(static_cast<unsigned>((nl + 255) / 256)), dim3(256), 0, s,
+                               c->E, c->d_tab, w, c->wb);
+            constexpr int EPB = AdjTile<N>::EPB;
+            const long long nb = (c->E + EPB - 1) / EPB;
+            Res2DP P = c->p2;
+            hipLaunchKernelGGL((k_adj2d<N>), dim3(static_cast<unsigned>(nb)), dim3(ADJ_BS), 0, s, P, u, w,
+                               static_cast<const double*>(c->wb), sigma, out);
+            return hip_rc(hipGetLastError());
+        }
+    };
+    template <int N>
+    struct Adj1
+    {
+        static int run(const frsdg_ctx* c, const double* u, const double* w, double sigma, double* out,
+                       hipStream_t s)
+        {
+            const int nb = (c->p1.ne + 63) / 64;
+            hipLaunchKernelGGL((k_adj1d<N>), dim3(nb), dim3(64), 0, s, c->p1, u, w, sigma, out);
+            return hip_rc(hipGetLastError());
+        }
+    };
+
+    template <int N>
+    struct StageFilt
+    {
+        static int run(const frsdg_ctx* c, const IOStage& io, hipStream_t s)
+        {
+            constexpr int EPB = Tile2D<N, double>::EPB;
+            const long long nb = (c->E + EPB - 1) / EPB;
+            hipLaunchKernelGGL((k_res2d_stage_filter<N>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, c->p2, io);
+            return hip_rc(hipGetLastError());
+        }
+    };
+
     int ensure_work(frsdg_ctx* c)
     {
         if (!c->w1) FR_TRY(hipMalloc(&c->w1, sizeof(double) * c->nd));
@@ -394,6 +394,12 @@ namespace
         {
             if ((rc = stage(u, 0.0, 1.0, dt, c->w1))) return rc;
             if ((rc = stage(c->w1, 0.75, 0.25, dt, c->w2))) return rc;
+            if (c->d.apply_filter && c->d.dims == 2)
+            {
+                // last stage fused with the post-step filter
+                IOStage io{c->w2, u, u, 1.0 / 3.0, 2.0 / 3.0, dt};
+                return by_n<StageFilt>(c->n, c, io, s);
+            }
             if ((rc = stage(c->w2, 1.0 / 3.0, 2.0 / 3.0, dt, u))) return rc;
         }
         else if (stages == 4)
@@ -549,9 +555,10 @@ int frsdg_create(const frsdg_desc* d, frsdg_ctx** out)
 int frsdg_destroy(frsdg_ctx* c)
 {
     if (!c) return FRSDG_OK;
+    free_solver(c);
     if (c->gexec) (void) hipGraphExecDestroy(c->gexec);
     if (c->graph) (void) hipGraphDestroy(c->graph);
-    for (double* p : {c->d_tab, c->d_vmet, c->d_fxm, c->d_fym, c->d_csi, c->w1, c->w2, c->red})
+    for (double* p : {c->d_tab, c->d_vmet, c->d_fxm, c->d_fym, c->d_csi, c->w1, c->w2, c->red, c->wb})
         if (p) (void) hipFree(p);
     delete c;
     return FRSDG_OK;
@@ -622,9 +629,16 @@ int frsdg_jvp_cstep(frsdg_ctx* c, double, const double* u, const double* v, doub
     return rc;
 }
 
-int frsdg_vjp(frsdg_ctx*, double, const double*, const double*, double, double*, void*)
+int frsdg_vjp(frsdg_ctx* c, double, const double* u, const double* w, double sigma, double* out,
+              void* stream)
 {
-    return FRSDG_ERR_UNSUPPORTED;
+    if (!c || !u || !w || !out) return FRSDG_ERR_INVALID;
+    if (!aligned(u) || !aligned(w) || !aligned(out)) return FRSDG_ERR_ALIGN;
+    if (out == u || out == w) return FRSDG_ERR_INVALID; // neighbours' entries are still being read
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (c->d.dims == 1) return by_n<Adj1>(c->n, c, u, w, sigma, out, s);
+    if (!c->wb) FR_TRY(hipMalloc(&c->wb, sizeof(double) * 16 * c->n * c->E));
+    return by_n<Adj2>(c->n, c, u, w, sigma, out, s);
 }
 
 int frsdg_pre_jv_eval(frsdg_ctx* c, double t, const double* u, double sigma)

@@ -1,0 +1,64 @@
+// frsdg_ctx.hpp — the library context shared by the translation units of
+// libfrsdg.so, and small host helpers.
+#pragma once
+#include "../../include/frsdg.h"
+#include "frsdg_kernels.hpp"
+#include "frsdg_tables.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+struct frsdg_ctx
+{
+    using Tables = frsdg::Tables;
+    using Res2DP = frsdg::Res2DP;
+    using Res1DP = frsdg::Res1DP;
+    // problem and setup tables
+    frsdg_desc d;
+    int        n   = 0;
+    long long  E   = 0;
+    long long  nd  = 0; // number of DOFs
+    Tables     tb;
+    double*    d_tab  = nullptr; // D, dL, dR, filt
+    double*    d_vmet = nullptr;
+    double*    d_fxm  = nullptr;
+    double*    d_fym  = nullptr;
+    double*    d_csi  = nullptr;
+    Res2DP     p2{};
+    Res1DP     p1{};
+    // lazily allocated scratch
+    double* w1  = nullptr; // RK work states
+    double* w2  = nullptr;
+    double* red = nullptr; // reduction partials
+    double* wb  = nullptr; // adjoint lifted face cotangents [E][4][n][4]
+    // pre_jv_eval snapshot
+    const double* jv_u     = nullptr;
+    double        jv_sigma = 0.0;
+    double        jv_t     = 0.0;
+    // graph cache for ssprk_run
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t     graph = nullptr;
+    double*        g_u   = nullptr;
+    double         g_dt  = 0.0;
+    int            g_st  = 0;
+    int            g_n   = 0;
+    // Newton-Krylov workspace (frsdg_newton.hip)
+    struct SolverWs* ws = nullptr;
+};
+
+namespace frsdg
+{
+    inline int hip_rc(hipError_t e) { return e == hipSuccess ? FRSDG_OK : FRSDG_ERR_HIP; }
+#define FR_TRY(x)                                                                                  \
+    do                                                                                             \
+    {                                                                                              \
+        hipError_t _e = (x);                                                                       \
+        if (_e != hipSuccess) return FRSDG_ERR_HIP;                                                \
+    } while (0)
+
+    inline bool aligned(const void* p) { return (reinterpret_cast<std::uintptr_t>(p) & 15u) == 0; }
+
+    // defined in frsdg_newton.hip
+    void free_solver(frsdg_ctx* c);
+} // namespace frsdg

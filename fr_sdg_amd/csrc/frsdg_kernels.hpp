@@ -72,6 +72,7 @@ namespace frsdg
     struct IORes
     {
         using S = double;
+        static constexpr bool can_filter = false;
         const double* __restrict__ u;
         double* __restrict__ out;
         const double* __restrict__ add;
@@ -100,8 +101,9 @@ namespace frsdg
     struct IOStage
     {
         using S = double;
+        static constexpr bool can_filter = true;
         const double* __restrict__ u;  // us
-        const double* __restrict__ u0; // may alias out (same-thread read then write)
+        const double* u0; // may alias out (each node read by its own thread before any write)
         double* out;
         double  a0, b, cdt;
         __device__ double ld(long long i) const { return u[i]; }
@@ -111,9 +113,8 @@ namespace frsdg
             const double s = own + cdt * (-r);
             out[i]         = (a0 != 0.0) ? a0 * u0[i] + b * s : b * s;
         }
-        __device__ void st4(long long i, const double own[4], const double r[4]) const
+        __device__ void val4(long long i, const double own[4], const double r[4], double o[4]) const
         {
-            double o[4];
             for (int q = 0; q < 4; ++q) o[q] = own[q] + cdt * (-r[q]);
             if (a0 != 0.0)
             {
@@ -123,6 +124,12 @@ namespace frsdg
             }
             else
                 for (int q = 0; q < 4; ++q) o[q] = b * o[q];
+        }
+        __device__ void put4(long long i, const double o[4]) const { st4d(out + i, o); }
+        __device__ void st4(long long i, const double own[4], const double r[4]) const
+        {
+            double o[4];
+            val4(i, own, r, o);
             st4d(out + i, o);
         }
     };
@@ -130,6 +137,7 @@ namespace frsdg
     struct IOTan
     {
         using S = Dual;
+        static constexpr bool can_filter = false;
         const double* __restrict__ u;
         const double* __restrict__ v;
         double* __restrict__ out;
@@ -154,6 +162,7 @@ namespace frsdg
     struct IOCs
     {
         using S = Cplx;
+        static constexpr bool can_filter = false;
         const double* __restrict__ u;
         const double* __restrict__ v;
         double* __restrict__ out;
@@ -204,34 +213,46 @@ namespace frsdg
     // ------------------------------------------------------------------
     // 2D residual: physics_impl_2D.hpp:267-357 (volume) + :689-899 (faces)
     // ------------------------------------------------------------------
-    template <int N, class IO>
-    __global__ __launch_bounds__(BlockSize<typename IO::S>::value) void k_res2d(const Res2DP P,
-                                                                                 const IO io)
+    // LDS of one residual tile
+    template <int N, class S>
+    struct TileSmem
     {
-        using S             = typename IO::S;
-        constexpr int BS    = Tile2D<N, S>::BS;
-        constexpr int NN    = N * N;
-        constexpr int EPB   = Tile2D<N, S>::EPB;
-        __shared__ S      sU[EPB][NN][4];
-        __shared__ S      sF[EPB][NN][4];
-        __shared__ S      sG[EPB][NN][4];
-        __shared__ S      sDel[EPB][4][N][4];
-        __shared__ double sD[NN], sdL[N], sdR[N];
+        static constexpr int NN = N * N, EPB = Tile2D<N, S>::EPB;
+        S      U[EPB][NN][4];
+        S      F[EPB][NN][4];
+        S      G[EPB][NN][4];
+        S      Del[EPB][4][N][4];
+        double D[NN], dL[N], dR[N], Fm[NN];
+    };
 
-        const int tid = threadIdx.x;
-        for (int q = tid; q < NN; q += BS) sD[q] = P.tab[q];
+    // One tile = EPB consecutive elements of one mesh, processed by one
+    // workgroup.  `epb` <= EPB elements are active (the batched sweep may run
+    // smaller tiles).  With `filt` (stage outputs only) the element-local
+    // spectral filter of the post-step hook is applied in LDS before the store.
+    template <int N, class IO>
+    __device__ __forceinline__ void res2d_tile(const Res2DP& P, const IO& io, long long blk, int epb, bool filt,
+                                               TileSmem<N, typename IO::S>& sm)
+    {
+        using S           = typename IO::S;
+        constexpr int BS  = Tile2D<N, S>::BS;
+        constexpr int NN  = N * N;
+        const int     tid = threadIdx.x;
+        for (int q = tid; q < NN; q += BS)
+        {
+            sm.D[q] = P.tab[q];
+            if (filt) sm.Fm[q] = P.tab[NN + 2 * N + q];
+        }
         if (tid < N)
         {
-            sdL[tid] = P.tab[NN + tid];
-            sdR[tid] = P.tab[NN + N + tid];
+            sm.dL[tid] = P.tab[NN + tid];
+            sm.dR[tid] = P.tab[NN + N + tid];
         }
-        const long long blk  = xcd_tile(blockIdx.x, gridDim.x);
         const int       el   = tid / NN;
         const int       node = tid - el * NN;
         const int       k    = node % N;
         const int       l    = node / N;
-        const long long e    = blk * EPB + el;
-        const bool      act  = (el < EPB) && (e < P.E);
+        const long long e    = blk * epb + el;
+        const bool      act  = (el < epb) && (e < P.E);
 
         S own[4];
         if (act)
@@ -241,19 +262,19 @@ namespace frsdg
             flux2(P.gas, own, F, G);
             for (int q = 0; q < 4; ++q)
             {
-                sU[el][node][q] = own[q];
-                sF[el][node][q] = F[q];
-                sG[el][node][q] = G[q];
+                sm.U[el][node][q] = own[q];
+                sm.F[el][node][q] = F[q];
+                sm.G[el][node][q] = G[q];
             }
         }
         __syncthreads();
 
         // ---- face tasks: 4 faces x N nodes per element ----
-        for (int t = tid; t < EPB * 4 * N; t += BS)
+        for (int t = tid; t < epb * 4 * N; t += BS)
         {
             const int       fe = t / (4 * N);
             const int       ft = t - fe * 4 * N;
-            const long long eg = blk * EPB + fe;
+            const long long eg = blk * epb + fe;
             if (eg >= P.E) continue;
             const int face = ft / N, idx = ft - (ft / N) * N;
             const int i = static_cast<int>(eg % P.nx), j = static_cast<int>(eg / P.nx);
@@ -320,7 +341,7 @@ namespace frsdg
                 }
             }
             S Uo[4], Un[4], Fs[4];
-            for (int q = 0; q < 4; ++q) Uo[q] = sU[fe][on][q];
+            for (int q = 0; q < 4; ++q) Uo[q] = sm.U[fe][on][q];
             if (bnd)
                 bc_exterior(P.bc, dir, Uo, fnx, fny, Un);
             else
@@ -331,60 +352,111 @@ namespace frsdg
                 riemann2(P.gas, P.riemann, Uo, Un, fnx, fny, Fs);
             for (int q = 0; q < 4; ++q)
             {
-                const S fn          = sF[fe][on][q] * fnx + sG[fe][on][q] * fny;
-                sDel[fe][face][idx][q] = sc * (Fs[q] - fn);
+                const S fn               = sm.F[fe][on][q] * fnx + sm.G[fe][on][q] * fny;
+                sm.Del[fe][face][idx][q] = sc * (Fs[q] - fn);
             }
         }
         __syncthreads();
 
-        if (!act) return;
         S r[4];
-        if (!P.curved)
+        if (act)
         {
+            if (!P.curved)
+            {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                {
+                    S dFx = Sc<S>::c(0.0), dGy = Sc<S>::c(0.0);
+#pragma unroll
+                    for (int m = 0; m < N; ++m)
+                    {
+                        dFx = dFx + sm.D[k * N + m] * sm.F[el][l * N + m][q];
+                        dGy = dGy + sm.D[l * N + m] * sm.G[el][m * N + k][q];
+                    }
+                    r[q] = P.xi_x * dFx + P.eta_y * dGy;
+                }
+            }
+            else
+            {
+                double mt[4];
+                ld4d(P.vmet + (e * NN + node) * 4, mt);
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                {
+                    S dFx = Sc<S>::c(0.0), dGx = Sc<S>::c(0.0), dFy = Sc<S>::c(0.0), dGy = Sc<S>::c(0.0);
+#pragma unroll
+                    for (int m = 0; m < N; ++m)
+                    {
+                        const double a = sm.D[k * N + m], b = sm.D[l * N + m];
+                        dGx = dGx + a * sm.G[el][l * N + m][q];
+                        dFx = dFx + a * sm.F[el][l * N + m][q];
+                        dGy = dGy + b * sm.G[el][m * N + k][q];
+                        dFy = dFy + b * sm.F[el][m * N + k][q];
+                    }
+                    const S px = mt[0] * dFx + mt[2] * dFy;
+                    const S py = mt[1] * dGx + mt[3] * dGy;
+                    r[q]       = px + py;
+                }
+            }
 #pragma unroll
             for (int q = 0; q < 4; ++q)
             {
-                S dFx = Sc<S>::c(0.0), dGy = Sc<S>::c(0.0);
-#pragma unroll
-                for (int m = 0; m < N; ++m)
-                {
-                    dFx = dFx + sD[k * N + m] * sF[el][l * N + m][q];
-                    dGy = dGy + sD[l * N + m] * sG[el][m * N + k][q];
-                }
-                r[q] = P.xi_x * dFx + P.eta_y * dGy;
+                r[q] = r[q] + sm.Del[el][0][l][q] * sm.dL[k];
+                r[q] = r[q] + sm.Del[el][1][l][q] * sm.dR[k];
+                r[q] = r[q] + sm.Del[el][2][k][q] * sm.dL[l];
+                r[q] = r[q] + sm.Del[el][3][k][q] * sm.dR[l];
             }
         }
-        else
+        if constexpr (IO::can_filter)
         {
-            double mt[4];
-            ld4d(P.vmet + (e * NN + node) * 4, mt);
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
+            if (filt)
             {
-                S dFx = Sc<S>::c(0.0), dGx = Sc<S>::c(0.0), dFy = Sc<S>::c(0.0), dGy = Sc<S>::c(0.0);
-#pragma unroll
-                for (int m = 0; m < N; ++m)
+                // stage value -> LDS (U is free: its last readers, the face
+                // tasks, finished at the previous barrier), then
+                // out = 1/2 (F_x v + F_y v)
+                if (act)
                 {
-                    const double a = sD[k * N + m], b = sD[l * N + m];
-                    dGx = dGx + a * sG[el][l * N + m][q];
-                    dFx = dFx + a * sF[el][l * N + m][q];
-                    dGy = dGy + b * sG[el][m * N + k][q];
-                    dFy = dFy + b * sF[el][m * N + k][q];
+                    double o[4];
+                    io.val4((e * NN + node) * 4, own, r, o);
+                    for (int q = 0; q < 4; ++q) sm.U[el][node][q] = o[q];
                 }
-                const S px = mt[0] * dFx + mt[2] * dFy;
-                const S py = mt[1] * dGx + mt[3] * dGy;
-                r[q]       = px + py;
+                __syncthreads();
+                if (act)
+                {
+                    double o[4];
+                    for (int q = 0; q < 4; ++q)
+                    {
+                        double sx = 0.0, sy = 0.0;
+                        for (int c = 0; c < N; ++c)
+                        {
+                            sx += sm.Fm[k * N + c] * sm.U[el][l * N + c][q];
+                            sy += sm.Fm[l * N + c] * sm.U[el][c * N + k][q];
+                        }
+                        o[q] = 0.5 * (sx + sy);
+                    }
+                    io.put4((e * NN + node) * 4, o);
+                }
+                return;
             }
         }
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-        {
-            r[q] = r[q] + sDel[el][0][l][q] * sdL[k];
-            r[q] = r[q] + sDel[el][1][l][q] * sdR[k];
-            r[q] = r[q] + sDel[el][2][k][q] * sdL[l];
-            r[q] = r[q] + sDel[el][3][k][q] * sdR[l];
-        }
-        io.st4((e * NN + node) * 4, own, r);
+        if (act) io.st4((e * NN + node) * 4, own, r);
+    }
+
+    // ------------------------------------------------------------------
+    // 2D residual: physics_impl_2D.hpp:267-357 (volume) + :689-899 (faces)
+    // ------------------------------------------------------------------
+    template <int N, class IO>
+    __global__ __launch_bounds__(BlockSize<typename IO::S>::value) void k_res2d(const Res2DP P, const IO io)
+    {
+        __shared__ TileSmem<N, typename IO::S> sm;
+        res2d_tile<N, IO>(P, io, xcd_tile(blockIdx.x, gridDim.x), Tile2D<N, typename IO::S>::EPB, false, sm);
+    }
+    // last SSP-RK stage fused with the post-step spectral filter
+    template <int N>
+    __global__ __launch_bounds__(256) void k_res2d_stage_filter(const Res2DP P, const IOStage io)
+    {
+        __shared__ TileSmem<N, double> sm;
+        res2d_tile<N, IOStage>(P, io, xcd_tile(blockIdx.x, gridDim.x), Tile2D<N, double>::EPB, true, sm);
     }
 
     // ------------------------------------------------------------------

@@ -252,21 +252,42 @@ namespace frsdg
         BC_PERIODIC = 5,
         BC_SLIPWALL = 6
     };
+    // static-index selection (keeps the kernel-argument copy of BCP out of
+    // scratch memory: no run-time indexing into it)
+    FR_HD int bc_type_of(const BCP& bc, int dir)
+    {
+        return dir == 0 ? bc.type[0] : dir == 1 ? bc.type[1] : dir == 2 ? bc.type[2] : bc.type[3];
+    }
+    FR_HD double bc_state_of(const BCP& bc, int dir, int q)
+    {
+        double r = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                if (k == dir && m == q) r = bc.state[k][m];
+        return r;
+    }
     // exterior state for boundary `dir` (0 L, 1 R, 2 T, 3 B) given the interior
     // trace and the face normal.
     template <class S>
     FR_HD void bc_exterior(const BCP& bc, int dir, const S in[4], double nx, double ny, S ex[4])
     {
-        const int t = bc.type[dir];
+        const int t = bc_type_of(bc, dir);
+#pragma unroll
         for (int q = 0; q < 4; ++q) ex[q] = in[q];
         if (t == BC_WALL)
         {
-            const int axis = dir / 2; // :300-308
-            ex[1 + axis]   = -in[1 + axis];
+            // :300-308, axis = dir / 2
+            if (dir < 2)
+                ex[1] = -in[1];
+            else
+                ex[2] = -in[2];
         }
         else if (t == BC_DIRICHLET)
         {
-            for (int q = 0; q < 4; ++q) ex[q] = Sc<S>::c(bc.state[dir][q]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) ex[q] = Sc<S>::c(bc_state_of(bc, dir, q));
         }
         else if (t == BC_SLIPWALL)
         {
@@ -285,11 +306,11 @@ namespace frsdg
     template <class S>
     FR_HD void bc_exterior1(const BCP& bc, int side, const S in[3], S ex[3])
     {
-        const int t = bc.type[side];
+        const int t = bc_type_of(bc, side);
         for (int q = 0; q < 3; ++q) ex[q] = in[q];
         if (t == BC_WALL)
             ex[1] = -in[1];
         else if (t == BC_DIRICHLET)
-            for (int q = 0; q < 3; ++q) ex[q] = Sc<S>::c(bc.state[side][q]);
+            for (int q = 0; q < 3; ++q) ex[q] = Sc<S>::c(bc_state_of(bc, side, q));
     }
 } // namespace frsdg

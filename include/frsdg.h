@@ -133,6 +133,61 @@ int frsdg_impose_ic(frsdg_ctx* ctx, int kind, const double* prm, double* u, void
 int frsdg_l2_error(frsdg_ctx* ctx, int kind, const double* prm, double t, const double* u,
                    double* out, void* stream);
 
+/* ---------------- implicit steps: Jacobian-free Newton-Krylov ----------------
+ * The TSSetIFunction / TSSetIJacobian path of the reference
+ * (petsc_solver_impl.hpp:572-579, 626-655, 168-267) with PETSc's SNES newtonls
+ * + KSP GMRES replaced by a device solver: Newton with backtracking line
+ * search, restarted GMRES (classical Gram-Schmidt, right preconditioning) whose
+ * operator is the MatShell product sigma*v + J v from the forward-tangent or
+ * the complex-step kernel. */
+typedef struct frsdg_newton_opts
+{
+    int    jac_type;    /* 0 forward tangent, 1 complex step (h = 1e-7) */
+    int    precond;     /* 0 none (-pc_type none), 1 element block-Jacobi */
+    double snes_rtol;   /* stop when ||F|| <= snes_rtol * ||F_0|| (PETSc default 1e-8) */
+    double snes_atol;   /* or ||F|| <= snes_atol (PETSc default 1e-50) */
+    int    snes_max_it; /* PETSc default 50 */
+    int    ksp_restart; /* PETSc default 30 */
+    int    ksp_max_it;  /* PETSc default 10000 */
+    double ksp_rtol;    /* PETSc default 1e-5 */
+} frsdg_newton_opts;
+
+typedef struct frsdg_newton_stats
+{
+    int    newton_its, ksp_its, jv_evals, res_evals, converged;
+    double fnorm0, fnorm;
+    double seconds; /* wall time of the solve, stream-synchronised */
+} frsdg_newton_stats;
+
+/* One backward-Euler step: solves F(u) = (u - u^n)/dt + R(u) = 0 for u^{n+1}.
+ * u holds u^n on entry and u^{n+1} on return.  Synchronises the stream. */
+int frsdg_beuler_step(frsdg_ctx* ctx, double t, double dt, double* u, const frsdg_newton_opts* opts,
+                      frsdg_newton_stats* stats, void* stream);
+
+/* ---------------- batched sweep of independent problems ----------------
+ * No reference counterpart (the reference runs one problem per process).
+ * Instances may differ in mesh size, order, gas model and BCs (2D only).  Their
+ * states are concatenated in one device array; frsdg_batch_offsets gives the
+ * first DOF of each (count+1 entries).  One SSP-RK3 step = 3 persistent-kernel
+ * launches over all (instance, element-tile) work items, the post-step filter
+ * fused into the last one. */
+typedef struct frsdg_batch frsdg_batch;
+int       frsdg_batch_create(const frsdg_desc* descs, int count, frsdg_batch** out);
+int       frsdg_batch_destroy(frsdg_batch* b);
+long long frsdg_batch_ndofs(const frsdg_batch* b);
+int       frsdg_batch_offsets(const frsdg_batch* b, long long* off);
+/* the per-instance context (owned by the batch; do not destroy) */
+int       frsdg_batch_instance(frsdg_batch* b, int k, frsdg_ctx** out);
+/* per-instance time steps (host array of count doubles), copied to the device; synchronous */
+int       frsdg_batch_set_dt(frsdg_batch* b, const double* dt);
+/* launch shape: persistent workgroups, and tile = (max elements per tile)/epb_div */
+int       frsdg_batch_set_tuning(frsdg_batch* b, int grid, int epb_div);
+int       frsdg_batch_ssprk_step(frsdg_batch* b, double* u, void* stream);
+/* nsteps steps replayed from a captured hipGraph */
+int       frsdg_batch_ssprk_run(frsdg_batch* b, double* u, int nsteps, void* stream);
+/* the same step with one set of launches per instance (comparison point) */
+int       frsdg_batch_ssprk_step_each(frsdg_batch* b, const double* dt, double* u, void* stream);
+
 const char* frsdg_strerror(int code);
 
 #ifdef __cplusplus
