@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-large", action="store_true", help="skip the large curved residual")
+    ap.add_argument("--no-newton", action="store_true", help="skip the implicit (JFNK) comparison")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep comparison")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--kernel-only", choices=["stage", "residual", "tangent", "cstep", "adjoint", "large",
                                                 "copy"], default=None,
@@ -170,6 +172,16 @@ def main():
             line["large_curved"] = bench_large(torch, fr)
         except Exception as exc:  # report, keep the headline line valid
             line["large_curved"] = {"error": repr(exc)}
+    if not args.no_newton:
+        try:
+            line["newton"] = bench_newton(torch, fr)
+        except Exception as exc:
+            line["newton"] = {"error": repr(exc)}
+    if not args.no_batch:
+        try:
+            line["batched"] = bench_batch(torch, fr)
+        except Exception as exc:
+            line["batched"] = {"error": repr(exc)}
 
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
@@ -227,6 +239,79 @@ def bench_large(torch, fr):
             "dof_rhs_per_s": ndof / t, "gbs_alg": bytes_alg / t / 1e9,
             "hbm_frac": bytes_alg / t / 1e9 / HBM_PEAK_GBS,
             "fp64_tflops": (32 * n ** 3 + 89 * n ** 2 + 424 * n) * E / t / 1e12, "finite": ok}
+
+
+def bench_newton(torch, fr):
+    """Channel bump 256x64 p=5 (2.36 M DOFs), backward Euler with a CFL ramp
+    10 -> 100 -> 1000 from free stream; each step solved by JFNK to
+    ||F|| <= 1e-8 ||F_0|| with GMRES(30) + element block-Jacobi, J*v from
+    the forward-tangent kernel vs the complex-step kernel (same GPU)."""
+    from fr_sdg_amd import configs as CF
+    out = {"workload": "bump 256x64 p=5 HLLC, backward Euler CFL 10->100->1000, JFNK GMRES(30)+block-Jacobi"}
+    for jt, name in ((0, "tangent"), (1, "complex_step")):
+        cfg = CF.bump(tangent_bc_mode=1)
+        ev = fr.PhysicsEvaluator(cfg)
+        u = ev.impose_ic(fr.IC_UNIFORM, CF.bump_freestream())
+        smax = CF.sound_speed_max(u, cfg)
+        h = min(cfg["lx"] / cfg["nx"], cfg["ly"] / cfg["ny"])
+        steps = []
+        for cfl in (10.0, 100.0, 1000.0):
+            st = ev.beuler_step(u, cfl * h / smax, jac_type=jt, precond=1)
+            steps.append(st)
+        tot = sum(x["seconds"] for x in steps)
+        nits = sum(x["newton_its"] for x in steps)
+        out[name] = {"seconds": tot, "newton_its": nits, "ksp_its": sum(x["ksp_its"] for x in steps),
+                     "jv_evals": sum(x["jv_evals"] for x in steps),
+                     "s_per_newton_it": tot / max(1, nits),
+                     "fnorm_reduction": [x["fnorm"] / x["fnorm0"] for x in steps],
+                     "converged": all(x["converged"] for x in steps)}
+        del ev, u
+        torch.cuda.empty_cache()
+    out["speedup_total"] = out["complex_step"]["seconds"] / out["tangent"]["seconds"]
+    out["speedup_per_newton_it"] = out["complex_step"]["s_per_newton_it"] / out["tangent"]["s_per_newton_it"]
+    return out
+
+
+def bench_batch(torch, fr, steps=20):
+    """64 radial shock tubes (N in {32..192}, p in {2..6}, seeded), SSP-RK3 at
+    CFL 0.1 per instance: 3 persistent batched launches per step (graph
+    replay, autotuned launch shape) vs one set of launches per instance."""
+    from fr_sdg_amd import autotune
+    from fr_sdg_amd import configs as CF
+    cfgs = CF.batch64()
+    b = fr.Batch(cfgs)
+    u = b.empty()
+    dts = []
+    for k, iv in enumerate(b.instances):
+        uk = iv.impose_ic(fr.IC_EXPLOSION, CF.EXPLOSION_PRM, out=b.view(u, k))
+        h = cfgs[k]["lx"] / cfgs[k]["nx"]
+        dts.append(0.1 * h / CF.sound_speed_max(uk, cfgs[k]))
+    b.set_dt(dts)
+    best, table = autotune.tune_batch(b, u, use_cache=False)
+    s = torch.cuda.current_stream()
+
+    def timed(fn):
+        w = u.clone()
+        fn(w, 2)
+        torch.cuda.synchronize()
+        a, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        fn(w, steps)
+        e.record(s)
+        e.synchronize()
+        return a.elapsed_time(e) / 1e3 / steps, w
+
+    t_b, wb = timed(lambda w, n: b.ssprk_run(w, n))
+
+    def each(w, n):
+        for _ in range(n):
+            b.ssprk_step_each(w)
+    t_e, we = timed(each)
+    same = bool(torch.equal(wb, we))
+    return {"workload": "64 radial shock tubes, seeded N/p/gas, Rusanov, filter s=16 a=36, SSP-RK3",
+            "ndofs": b.ndofs, "steps": steps, "batched_s_per_step": t_b, "per_instance_s_per_step": t_e,
+            "speedup": t_e / t_b, "dof_rhs_per_s_batched": 3 * b.ndofs / t_b, "tuning": list(best),
+            "tuning_table": table, "bitwise_equal": same}
 
 
 def cpu_baseline(seconds):

@@ -80,6 +80,7 @@ EXPORTS = [
     "frsdg_beuler_step", "frsdg_batch_create", "frsdg_batch_destroy", "frsdg_batch_ndofs",
     "frsdg_batch_offsets", "frsdg_batch_instance", "frsdg_batch_set_dt", "frsdg_batch_set_tuning",
     "frsdg_batch_ssprk_step", "frsdg_batch_ssprk_run", "frsdg_batch_ssprk_step_each",
+    "frsdg_filter_transpose", "frsdg_mass", "frsdg_boundary_weights", "frsdg_adjoint_solve",
 ]
 
 _lib = None
@@ -126,6 +127,11 @@ def load_library(path=LIB_PATH):
     L.frsdg_batch_ssprk_step.argtypes = [vp, dp, vp]
     L.frsdg_batch_ssprk_run.argtypes = [vp, dp, i, vp]
     L.frsdg_batch_ssprk_step_each.argtypes = [vp, ctypes.POINTER(ctypes.c_double), dp, vp]
+    L.frsdg_filter_transpose.argtypes = [vp, dp, dp, vp]
+    L.frsdg_mass.argtypes = [vp, dp, vp]
+    L.frsdg_boundary_weights.argtypes = [vp, i, dp, vp]
+    L.frsdg_adjoint_solve.argtypes = [vp, d, d, dp, dp, dp, ctypes.POINTER(NewtonOpts),
+                                      ctypes.POINTER(NewtonStats), vp]
     L.frsdg_strerror.argtypes = [i]
     L.frsdg_strerror.restype = ctypes.c_char_p
     _lib = L
@@ -291,6 +297,33 @@ class PhysicsEvaluator:
         st = NewtonStats()
         _check(self.lib.frsdg_beuler_step(self.h, t, dt, _ptr(u), ctypes.byref(no), ctypes.byref(st), _stream()))
         return st.as_dict()
+
+    def adjoint_solve(self, u, rhs, sigma, t=0.0, out=None, **opts):
+        """(sigma I + J(u))^T lambda = rhs by GMRES on the adjoint kernel."""
+        out = self.empty() if out is None else out
+        o = dict(NEWTON_DEFAULTS)
+        o.update(opts)
+        no = NewtonOpts(**o)
+        st = NewtonStats()
+        _check(self.lib.frsdg_adjoint_solve(self.h, t, sigma, _ptr(u), _ptr(rhs), _ptr(out), ctypes.byref(no),
+                                            ctypes.byref(st), _stream()))
+        return out, st.as_dict()
+
+    def filter_transpose(self, w, out=None):
+        out = self.empty() if out is None else out
+        _check(self.lib.frsdg_filter_transpose(self.h, _ptr(w), _ptr(out), _stream()))
+        return out
+
+    def mass(self):
+        m = self.empty()
+        _check(self.lib.frsdg_mass(self.h, _ptr(m), _stream()))
+        return m
+
+    def boundary_weights(self, direction):
+        along = self.cfg["ny"] if direction < 2 else self.cfg["nx"]
+        out = torch.empty(along * self.n * 2, dtype=torch.float64, device=self.device)
+        _check(self.lib.frsdg_boundary_weights(self.h, direction, _ptr(out), _stream()))
+        return out.view(along, self.n, 2)
 
     def impose_ic(self, kind, prm, out=None):
         out = self.empty() if out is None else out

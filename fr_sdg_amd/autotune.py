@@ -1,0 +1,77 @@
+"""Launch-shape autotuner for the batched sweep.
+
+Knobs (frsdg_batch_set_tuning): the number of persistent workgroups of the
+stage kernel, and the tile size as a divisor of the largest elements-per-
+workgroup tile that fits 256 threads (smaller tiles = more, shorter work
+items, better balance across uneven instances).  Each candidate is timed over
+a few graph-free batched steps on a scratch copy of the state; the winner is
+cached in a JSON file keyed by the batch signature (per-instance mesh size,
+order, gas model) and the device name.
+"""
+import hashlib
+import json
+import os
+
+import torch
+
+CACHE = os.environ.get("FRSDG_TUNING_CACHE",
+                       os.path.join(os.path.expanduser("~"), ".cache", "fr_sdg_amd", "tuning.json"))
+
+
+def signature(batch):
+    items = [(c["nx"], c["ny"], c["p"], c.get("gas_kind", 0), c.get("geometry", 0), c.get("apply_filter", 0))
+             for c in batch.cfgs]
+    dev = torch.cuda.get_device_name(batch.device)
+    return hashlib.sha1(json.dumps([dev, items]).encode()).hexdigest()
+
+
+def _load():
+    try:
+        with open(CACHE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(db):
+    try:
+        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+        with open(CACHE, "w") as f:
+            json.dump(db, f, indent=1)
+    except OSError:
+        pass
+
+
+def candidates():
+    ncu = torch.cuda.get_device_properties(0).multi_processor_count
+    return [(g * ncu, d) for g in (1, 2, 4, 8) for d in (1, 2)]
+
+
+def tune_batch(batch, u, steps=3, use_cache=True):
+    """Pick (grid, epb_div) for `batch` by timing; returns (best, table)."""
+    key = signature(batch)
+    db = _load() if use_cache else {}
+    if use_cache and key in db:
+        g, d = db[key]["best"]
+        batch.set_tuning(g, d)
+        return (g, d), db[key]["table"]
+    scratch = u.clone()
+    table = []
+    s = torch.cuda.current_stream()
+    for g, d in candidates():
+        batch.set_tuning(g, d)
+        scratch.copy_(u)
+        batch.ssprk_step(scratch)  # warm
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        for _ in range(steps):
+            batch.ssprk_step(scratch)
+        b.record(s)
+        b.synchronize()
+        table.append([g, d, a.elapsed_time(b) / steps])
+    best = min(table, key=lambda r: r[2])
+    batch.set_tuning(best[0], best[1])
+    db[key] = {"best": best[:2], "table": table}
+    if use_cache:
+        _store(db)
+    return (best[0], best[1]), table

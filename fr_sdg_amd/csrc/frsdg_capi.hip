@@ -80,6 +80,68 @@ namespace
         fm[t * 4 + 3] = 0.0;
     }
 
+    // quadrature weight of every DOF: w_k w_l detJ (2D, compute_error's
+    // physics_impl_2D.hpp:253) or (h/2) w_k (1D, physics_impl_1D.hpp:260-312)
+    __global__ void k_mass2d(GeomP g, int nx, int ny, int n, const double* __restrict__ csi,
+                             const double* __restrict__ w, long long E, double* __restrict__ m)
+    {
+        const long long t  = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+        const long long NN = static_cast<long long>(n) * n;
+        if (t >= E * NN) return;
+        const long long e = t / NN;
+        const int node = static_cast<int>(t - e * NN), k = node % n, l = node / n;
+        const int i = static_cast<int>(e % nx), j = static_cast<int>(e / nx);
+        const double cx = (1.0 / static_cast<double>(nx)) * (static_cast<double>(i) + 0.5 * (1.0 + csi[k]));
+        const double cy = (1.0 / static_cast<double>(ny)) * (static_cast<double>(j) + 0.5 * (1.0 + csi[l]));
+        const double v  = w[k] * w[l] * composite_metrics(g, nx, ny, cx, cy).detJ;
+        for (int q = 0; q < 4; ++q) m[t * 4 + q] = v;
+    }
+    __global__ void k_mass1d(int ne, int n, double h, const double* __restrict__ w, double* __restrict__ m)
+    {
+        const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+        if (t >= static_cast<long long>(ne) * 3 * n) return;
+        m[t] = 0.5 * h * w[t % n];
+    }
+    // boundary quadrature of face direction dir (0 L, 1 R, 2 T, 3 B) per face
+    // node: (w_idx ds nx, w_idx ds ny), ds = |d(x,y)/d(face coordinate)|
+    __global__ void k_wall2d(GeomP g, int nx, int ny, int n, const double* __restrict__ csi,
+                             const double* __restrict__ w, int dir, double* __restrict__ out)
+    {
+        const int  along = (dir < 2) ? ny : nx;
+        const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+        if (t >= static_cast<long long>(along) * n) return;
+        const int el = static_cast<int>(t / n), idx = static_cast<int>(t % n);
+        const double hx = 1.0 / static_cast<double>(nx), hy = 1.0 / static_cast<double>(ny);
+        double cx, cy;
+        if (dir < 2)
+        {
+            cx = (dir == 0) ? 0.0 : 1.0;
+            cy = hy * (static_cast<double>(el) + 0.5 * (1.0 + csi[idx]));
+        }
+        else
+        {
+            cx = hx * (static_cast<double>(el) + 0.5 * (1.0 + csi[idx]));
+            cy = (dir == 3) ? 0.0 : 1.0;
+        }
+        const Jac2 J = geom_jac(g, cx, cy);
+        double a, b, ds;
+        if (dir < 2)
+        {
+            a  = J.y_e;
+            b  = 0.0 - J.x_e;
+            ds = 0.5 * hy * ::sqrt(J.x_e * J.x_e + J.y_e * J.y_e);
+        }
+        else
+        {
+            a  = 0.0 - J.y_c;
+            b  = J.x_c;
+            ds = 0.5 * hx * ::sqrt(J.x_c * J.x_c + J.y_c * J.y_c);
+        }
+        const double inv = 1.0 / ::sqrt(a * a + b * b);
+        out[t * 2 + 0] = w[idx] * ds * a * inv;
+        out[t * 2 + 1] = w[idx] * ds * b * inv;
+    }
+
     // ---------------- initial conditions / exact solutions ----------------
     // factory.hpp:311-352 (isentropic vortex, gamma hard-wired 1.4),
     // factory.hpp:423-454 (circular explosion), uniform state.
@@ -305,19 +367,19 @@ namespace
     template <int N>
     struct Filt
     {
-        static int run(const frsdg_ctx* c, const double* u, double* o, hipStream_t s)
+        static int run(const frsdg_ctx* c, const double* u, double* o, hipStream_t s, int transpose = 0)
         {
             if (c->d.dims == 2)
             {
                 constexpr int EPB = (256 / (N * N)) > 0 ? 256 / (N * N) : 1;
                 const long long nb = (c->E + EPB - 1) / EPB;
                 hipLaunchKernelGGL((k_filter2d<N>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, c->E,
-                                   c->d_tab, u, o);
+                                   c->d_tab, u, o, transpose);
             }
             else
             {
                 const int nb = (c->p1.ne + 63) / 64;
-                hipLaunchKernelGGL((k_filter1d<N>), dim3(nb), dim3(64), 0, s, c->p1.ne, c->d_tab, u, o);
+                hipLaunchKernelGGL((k_filter1d<N>), dim3(nb), dim3(64), 0, s, c->p1.ne, c->d_tab, u, o, transpose);
             }
             return hip_rc(hipGetLastError());
         }
@@ -662,6 +724,44 @@ int frsdg_filter(frsdg_ctx* c, const double* u, double* out, void* stream)
     if (!c || !u || !out) return FRSDG_ERR_INVALID;
     if (!aligned(u) || !aligned(out)) return FRSDG_ERR_ALIGN;
     return do_filter(c, u, out, static_cast<hipStream_t>(stream));
+}
+
+int frsdg_filter_transpose(frsdg_ctx* c, const double* w, double* out, void* stream)
+{
+    if (!c || !w || !out) return FRSDG_ERR_INVALID;
+    if (!aligned(w) || !aligned(out)) return FRSDG_ERR_ALIGN;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!c->d.apply_filter)
+    {
+        if (out != w) FR_TRY(hipMemcpyAsync(out, w, sizeof(double) * c->nd, hipMemcpyDeviceToDevice, s));
+        return FRSDG_OK;
+    }
+    return by_n<Filt>(c->n, c, w, out, s, 1);
+}
+
+int frsdg_mass(frsdg_ctx* c, double* m, void* stream)
+{
+    if (!c || !m) return FRSDG_ERR_INVALID;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int n = c->n;
+    if (c->d.dims == 2)
+        hipLaunchKernelGGL(k_mass2d, dim3(static_cast<unsigned>((c->E * n * n + 255) / 256)), dim3(256), 0, s,
+                           geom_of(c->d), c->d.nx, c->d.ny, n, c->d_csi, c->d_csi + n, c->E, m);
+    else
+        hipLaunchKernelGGL(k_mass1d, dim3(static_cast<unsigned>((c->nd + 255) / 256)), dim3(256), 0, s, c->d.nx, n,
+                           c->d.lx / c->d.nx, c->d_csi + n, m);
+    return hip_rc(hipGetLastError());
+}
+
+int frsdg_boundary_weights(frsdg_ctx* c, int dir, double* out, void* stream)
+{
+    if (!c || !out || c->d.dims != 2 || dir < 0 || dir > 3) return FRSDG_ERR_INVALID;
+    const int along = (dir < 2) ? c->d.ny : c->d.nx;
+    const long long tot = static_cast<long long>(along) * c->n;
+    hipLaunchKernelGGL(k_wall2d, dim3(static_cast<unsigned>((tot + 255) / 256)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), geom_of(c->d), c->d.nx, c->d.ny, c->n, c->d_csi,
+                       c->d_csi + c->n, dir, out);
+    return hip_rc(hipGetLastError());
 }
 
 int frsdg_apply_stabilization(frsdg_ctx* c, double* u, void* stream)

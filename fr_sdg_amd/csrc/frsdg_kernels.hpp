@@ -533,13 +533,17 @@ namespace frsdg
     // ------------------------------------------------------------------
     template <int N>
     __global__ __launch_bounds__(256) void k_filter2d(long long E, const double* __restrict__ tab,
-                                                       const double* u, double* out)
+                                                       const double* u, double* out, int transpose)
     {
         constexpr int NN = N * N, EPB = (256 / NN) > 0 ? 256 / NN : 1;
         __shared__ double sU[EPB][NN][4];
         __shared__ double sFm[NN];
         const int tid = threadIdx.x;
-        for (int q = tid; q < NN; q += 256) sFm[q] = tab[NN + 2 * N + q];
+        for (int q = tid; q < NN; q += 256)
+        {
+            const int r = q / N, c = q - (q / N) * N; // transposed filter for the adjoint
+            sFm[q] = tab[NN + 2 * N + (transpose ? c * N + r : q)];
+        }
         const long long blk = xcd_tile(blockIdx.x, gridDim.x);
         const int el = tid / NN, node = tid - el * NN, k = node % N, l = node / N;
         const long long e   = blk * EPB + el;
@@ -567,7 +571,7 @@ namespace frsdg
     }
     template <int N>
     __global__ __launch_bounds__(64) void k_filter1d(int ne, const double* __restrict__ tab,
-                                                      const double* u, double* out)
+                                                      const double* u, double* out, int transpose)
     {
         const int e = blockIdx.x * blockDim.x + threadIdx.x;
         if (e >= ne) return;
@@ -578,7 +582,7 @@ namespace frsdg
             for (int r = 0; r < N; ++r)
             {
                 double s = 0.0;
-                for (int c = 0; c < N; ++c) s += Fm[r * N + c] * x[q * N + c];
+                for (int c = 0; c < N; ++c) s += Fm[transpose ? c * N + r : r * N + c] * x[q * N + c];
                 out[static_cast<long long>(e) * 3 * N + q * N + r] = s;
             }
     }

@@ -270,9 +270,9 @@ namespace
             __syncthreads();
         }
     }
-    // z_e = Binv_e x_e (column-major block: coalesced over rows)
+    // z_e = Binv_e x_e (column-major block: coalesced over rows), or Binv_e^T x_e
     __global__ void k_block_apply(long long E, int nd_e, const double* __restrict__ Binv, const double* __restrict__ x,
-                                  double* __restrict__ z)
+                                  double* __restrict__ z, int transpose)
     {
         const long long e = blockIdx.x;
         const double* A = Binv + e * nd_e * nd_e;
@@ -280,7 +280,10 @@ namespace
         for (int r = threadIdx.x; r < nd_e; r += blockDim.x)
         {
             double s = 0.0;
-            for (int c = 0; c < nd_e; ++c) s += A[static_cast<long long>(c) * nd_e + r] * xe[c];
+            if (transpose)
+                for (int c = 0; c < nd_e; ++c) s += A[static_cast<long long>(r) * nd_e + c] * xe[c];
+            else
+                for (int c = 0; c < nd_e; ++c) s += A[static_cast<long long>(c) * nd_e + r] * xe[c];
             z[e * nd_e + r] = s;
         }
         (void) E;
@@ -295,12 +298,14 @@ namespace
         double t, sigma;
         const double* u; // linearisation point
         frsdg_newton_stats* stats;
+        int transpose = 0; // solve with J^T (adjoint kernel) instead of J
 
         unsigned grid(long long n) const { return static_cast<unsigned>((n + 255) / 256); }
 
         int jv(const double* v, double* out)
         {
             stats->jv_evals++;
+            if (transpose) return frsdg_vjp(c, t, u, v, sigma, out, st);
             if (o->jac_type == 1) return frsdg_jvp_cstep(c, t, u, v, sigma, 1e-7, out, st);
             return frsdg_jvp(c, t, u, v, sigma, out, st);
         }
@@ -309,7 +314,7 @@ namespace
             if (o->precond == 1)
             {
                 const int nd_e = static_cast<int>(c->nd / c->E);
-                hipLaunchKernelGGL(k_block_apply, dim3(static_cast<unsigned>(c->E)), dim3(nd_e <= 64 ? 64 : (nd_e <= 128 ? 128 : 256)), 0, st, c->E, nd_e, s->B, x, z);
+                hipLaunchKernelGGL(k_block_apply, dim3(static_cast<unsigned>(c->E)), dim3(nd_e <= 64 ? 64 : (nd_e <= 128 ? 128 : 256)), 0, st, c->E, nd_e, s->B, x, z, transpose);
                 return hip_rc(hipGetLastError());
             }
             if (z != x) FR_TRY(hipMemcpyAsync(z, x, sizeof(double) * s->N, hipMemcpyDeviceToDevice, st));
@@ -332,12 +337,17 @@ namespace
         }
         int build_block_jacobi()
         {
+            // a 2-colour checkerboard separates face neighbours unless a
+            // periodic direction has an odd element count
+            if ((c->p2.per_x && (c->d.nx & 1)) || (c->p2.per_y && (c->d.ny & 1))) return FRSDG_ERR_UNSUPPORTED;
             const int nd_e = static_cast<int>(c->nd / c->E);
             const long long nb = static_cast<long long>(nd_e) * nd_e * c->E;
             if (!s->B) FR_TRY(hipMalloc(&s->B, sizeof(double) * nb));
             if (!s->pv) FR_TRY(hipMalloc(&s->pv, sizeof(double) * s->N));
             if (!s->pjv) FR_TRY(hipMalloc(&s->pjv, sizeof(double) * s->N));
             if (!s->info) FR_TRY(hipMalloc(&s->info, sizeof(int) * c->E));
+            const int keep = transpose;
+            transpose      = 0; // blocks are probed with forward products
             for (int col = 0; col < 2; ++col)
                 for (int m = 0; m < nd_e; ++m)
                 {
@@ -347,6 +357,7 @@ namespace
                     hipLaunchKernelGGL(k_probe_store, dim3(grid(s->N)), dim3(256), 0, st, c->E, c->d.nx, nd_e, m, col,
                                        s->pjv, s->B);
                 }
+            transpose = keep;
             hipLaunchKernelGGL(k_block_inverse, dim3(static_cast<unsigned>(c->E)), dim3(256), 0, st, nd_e, s->B, s->info);
             return hip_rc(hipGetLastError());
         }
@@ -526,6 +537,42 @@ extern "C" int frsdg_beuler_step(frsdg_ctx* c, double t, double dt, double* u, c
     FR_TRY(hipStreamSynchronize(st));
     S->fnorm     = fn;
     S->converged = fn <= target;
+    S->seconds   = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return FRSDG_OK;
+}
+
+// Adjoint linear solve (sigma I + dR/du(u))^T lambda = rhs by GMRES with the
+// adjoint kernel as operator (and the transposed block-Jacobi preconditioner).
+extern "C" int frsdg_adjoint_solve(frsdg_ctx* c, double t, double sigma, const double* u, const double* rhs,
+                                   double* lambda, const frsdg_newton_opts* o, frsdg_newton_stats* stats,
+                                   void* stream)
+{
+    if (!c || !u || !rhs || !lambda || !o) return FRSDG_ERR_INVALID;
+    if (!aligned(u) || !aligned(rhs) || !aligned(lambda)) return FRSDG_ERR_ALIGN;
+    if (o->precond == 1 && c->d.dims != 2) return FRSDG_ERR_UNSUPPORTED;
+    frsdg_newton_stats local{};
+    frsdg_newton_stats* S = stats ? stats : &local;
+    *S = frsdg_newton_stats{};
+    int rc = ensure_ws(c, o->ksp_restart);
+    if (rc) return rc;
+    SolverWs* s = c->ws;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    FR_TRY(hipStreamSynchronize(st));
+    const auto t0 = std::chrono::steady_clock::now();
+    Ctx K{c, s, st, o, t, sigma, u, S};
+    if (o->precond == 1)
+        if ((rc = K.build_block_jacobi())) return rc;
+    K.transpose = 1;
+    FR_TRY(hipMemcpyAsync(s->F, rhs, sizeof(double) * s->N, hipMemcpyDeviceToDevice, st));
+    S->fnorm0 = K.norm(s->F);
+    int its = 0;
+    if ((rc = K.gmres(s->F, lambda, its))) return rc;
+    S->ksp_its = its;
+    // true residual of the adjoint system
+    if ((rc = K.jv(lambda, s->w))) return rc;
+    hipLaunchKernelGGL(k_waxpby, dim3(K.grid(s->N)), dim3(256), 0, st, s->N, 1.0, s->F, -1.0, s->w, s->w);
+    S->fnorm     = K.norm(s->w);
+    S->converged = S->fnorm <= o->ksp_rtol * S->fnorm0 * 10.0;
     S->seconds   = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return FRSDG_OK;
 }

@@ -115,6 +115,16 @@ int frsdg_filter(frsdg_ctx* ctx, const double* u, double* out, void* stream);
  * the filter is active, no-op otherwise. */
 int frsdg_apply_stabilization(frsdg_ctx* ctx, double* u, void* stream);
 
+/* Transpose of the (linear) filter map, for adjoints: out = F^T w. */
+int frsdg_filter_transpose(frsdg_ctx* ctx, const double* w, double* out, void* stream);
+/* Quadrature weight of every DOF (device array of ndofs): w_k w_l detJ in 2D
+ * (compute_error, physics_impl_2D.hpp:253), (h/2) w_k in 1D. */
+int frsdg_mass(frsdg_ctx* ctx, double* m, void* stream);
+/* Boundary quadrature of face direction dir (0 Left, 1 Right, 2 Top, 3 Bottom):
+ * per face node along that boundary (element-major, then node), the pair
+ * (w ds n_x, w ds n_y) with n the face normal of the geometry model. */
+int frsdg_boundary_weights(frsdg_ctx* ctx, int dir, double* out, void* stream);
+
 /* One explicit step u <- S(u) with rhs = -R: stages 3 = Shu-Osher SSPRK(3,3),
  * stages 4 = SSPRK(4,3) (PETSc TSSSP rks3 default), each stage fused with its
  * residual; post-step filter (TSSetPostStep, petsc_solver_impl.hpp:604-605). */
@@ -163,6 +173,13 @@ typedef struct frsdg_newton_stats
  * u holds u^n on entry and u^{n+1} on return.  Synchronises the stream. */
 int frsdg_beuler_step(frsdg_ctx* ctx, double t, double dt, double* u, const frsdg_newton_opts* opts,
                       frsdg_newton_stats* stats, void* stream);
+
+/* Adjoint linear solve (sigma I + dR/du(u))^T lambda = rhs by GMRES on the
+ * adjoint kernel (opts: ksp_* fields, precond, jac_type for the block probes).
+ * No reference counterpart.  Synchronises the stream. */
+int frsdg_adjoint_solve(frsdg_ctx* ctx, double t, double sigma, const double* u, const double* rhs,
+                        double* lambda, const frsdg_newton_opts* opts, frsdg_newton_stats* stats,
+                        void* stream);
 
 /* ---------------- batched sweep of independent problems ----------------
  * No reference counterpart (the reference runs one problem per process).
