@@ -30,7 +30,7 @@ LEFT, RIGHT, TOP, BOTTOM = 0, 1, 2, 3
 GEOM_CARTESIAN, GEOM_DELAVAL, GEOM_WEDGE, GEOM_BUMP, GEOM_WARPED = 0, 1, 2, 3, 4
 RUSANOV, HLLC = 0, 1
 IDEAL, VAN_DER_WAALS = 0, 1
-IC_UNIFORM, IC_VORTEX, IC_EXPLOSION, IC_SOD = 0, 1, 2, 3
+IC_UNIFORM, IC_VORTEX, IC_EXPLOSION, IC_SOD, IC_PULSE = 0, 1, 2, 3, 4
 
 
 class FrsdgError(RuntimeError):

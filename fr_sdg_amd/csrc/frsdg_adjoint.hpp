@@ -155,7 +155,8 @@ namespace frsdg
             const long long eg = blk * EPB + fe;
             if (eg >= P.E) continue;
             const int face = ft / N, idx = ft - (ft / N) * N;
-            const int i = static_cast<int>(eg % P.nx), j = static_cast<int>(eg / P.nx);
+            const int eg32 = static_cast<int>(eg);
+            const int j = eg32 / P.nx, i = eg32 - j * P.nx;
             int       ok, ol, nk, nl, ni = i, nj = j, dir, nface;
             bool      bnd;
             if (face == 0)

@@ -168,6 +168,19 @@ namespace
             st[3] = gas_rhoE(g, rho, ::sqrt(u * u + v * v), p);
             return;
         }
+        if (c.kind == 4)
+        {
+            // smooth pressure pulse at rest: rho = rho0 (1 + A g), p = p0 (1 + A g)^gamma,
+            // g = exp(-|x - x_c|^2 / w^2)
+            const double dx = x - c.prm[2], dy = y - c.prm[3];
+            const double gg = ::exp(-(dx * dx + dy * dy) / (c.prm[1] * c.prm[1]));
+            const double rho = c.prm[4] * (1.0 + c.prm[0] * gg);
+            st[0] = rho;
+            st[1] = 0.0;
+            st[2] = 0.0;
+            st[3] = gas_rhoE(g, rho, 0.0, c.prm[5] * ::pow(1.0 + c.prm[0] * gg, g.gamma));
+            return;
+        }
         if (c.kind == 1)
         {
             const double u_0 = c.prm[0], v_0 = c.prm[1], beta = c.prm[2], r_v = c.prm[3], L = c.prm[4];
@@ -569,6 +582,7 @@ int frsdg_create(const frsdg_desc* d, frsdg_ctx** out)
         return FRSDG_OK;
     }
     c->E  = static_cast<long long>(d->nx) * d->ny;
+    if (c->E >= (1LL << 31) - 1024) return fail(FRSDG_ERR_UNSUPPORTED); // 32-bit element indices in kernels
     c->nd = 4LL * n * n * c->E;
     Res2DP& P = c->p2;
     P.nx      = d->nx;
@@ -830,7 +844,7 @@ int frsdg_ssprk_run(frsdg_ctx* c, double t, double dt, double* u, int stages, in
 
 int frsdg_impose_ic(frsdg_ctx* c, int kind, const double* prm, double* u, void* stream)
 {
-    if (!c || !prm || !u || kind < 0 || kind > 3) return FRSDG_ERR_INVALID;
+    if (!c || !prm || !u || kind < 0 || kind > 4) return FRSDG_ERR_INVALID;
     ICP ic{};
     ic.kind = kind;
     for (int q = 0; q < 8; ++q) ic.prm[q] = prm[q];
@@ -856,7 +870,7 @@ int frsdg_impose_ic(frsdg_ctx* c, int kind, const double* prm, double* u, void* 
 int frsdg_l2_error(frsdg_ctx* c, int kind, const double* prm, double t, const double* u, double* out,
                    void* stream)
 {
-    if (!c || !prm || !u || !out || c->d.dims != 2 || kind < 0 || kind > 2) return FRSDG_ERR_INVALID;
+    if (!c || !prm || !u || !out || c->d.dims != 2 || kind < 0 || kind > 4 || kind == 3) return FRSDG_ERR_INVALID;
     ICP ic{};
     ic.kind = kind;
     for (int q = 0; q < 8; ++q) ic.prm[q] = prm[q];

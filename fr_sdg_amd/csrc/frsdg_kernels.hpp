@@ -223,6 +223,7 @@ namespace frsdg
         S      G[EPB][NN][4];
         S      Del[EPB][4][N][4];
         double D[NN], dL[N], dR[N], Fm[NN];
+        int    I[EPB], J[EPB]; // element coordinates of the tile
     };
 
     // One tile = EPB consecutive elements of one mesh, processed by one
@@ -253,6 +254,13 @@ namespace frsdg
         const int       l    = node / N;
         const long long e    = blk * epb + el;
         const bool      act  = (el < epb) && (e < P.E);
+        if (tid < epb)
+        {
+            // element coordinates once per element (32-bit: E < 2^31 is checked at setup)
+            const int eg = static_cast<int>(blk * epb + tid);
+            sm.J[tid]    = eg / P.nx;
+            sm.I[tid]    = eg - sm.J[tid] * P.nx;
+        }
 
         S own[4];
         if (act)
@@ -277,7 +285,7 @@ namespace frsdg
             const long long eg = blk * epb + fe;
             if (eg >= P.E) continue;
             const int face = ft / N, idx = ft - (ft / N) * N;
-            const int i = static_cast<int>(eg % P.nx), j = static_cast<int>(eg / P.nx);
+            const int i = sm.I[fe], j = sm.J[fe];
             int       ok, ol, nk, nl, ni = i, nj = j, dir;
             bool      bnd;
             if (face == 0)

@@ -521,14 +521,20 @@ extern "C" int frsdg_beuler_step(frsdg_ctx* c, double t, double dt, double* u, c
         S->ksp_its += its;
         // backtracking line search on ||F||
         double lam = 1.0, fnew = 0.0;
+        bool   decreased = false;
         for (int ls = 0; ls < 8; ++ls)
         {
             hipLaunchKernelGGL(k_waxpby, dim3(g), dim3(256), 0, st, N, 1.0, u, lam, s->du, s->ut);
             if ((rc = eval_F(s->ut, s->w))) return rc;
             fnew = K.norm(s->w);
-            if (std::isfinite(fnew) && fnew <= (1.0 - 1e-4 * lam) * fn) break;
+            if (std::isfinite(fnew) && fnew <= (1.0 - 1e-4 * lam) * fn)
+            {
+                decreased = true;
+                break;
+            }
             lam *= 0.5;
         }
+        if (!decreased) break; // line search failed (SNES_DIVERGED_LINE_SEARCH): keep u
         FR_TRY(hipMemcpyAsync(u, s->ut, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
         FR_TRY(hipMemcpyAsync(s->F, s->w, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
         fn = fnew;

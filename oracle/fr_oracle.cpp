@@ -656,6 +656,16 @@ namespace fro
             return {rho, rho * u, rho * v, g.rhoE(rho, std::sqrt(u * u + v * v), p)};
         }
         if (kind == 1) return vortex(t, x, y, prm);
+        if (kind == 4)
+        {
+            // smooth pressure pulse at rest (test state): rho = rho0 (1 + A g), p = p0 (1 + A g)^gamma,
+            // g = exp(-((x-xc)^2 + (y-yc)^2) / w^2); prm = {A, w, xc, yc, rho0, p0}
+            const double dx = x - prm[2], dy = y - prm[3];
+            const double gg = std::exp(-(dx * dx + dy * dy) / (prm[1] * prm[1]));
+            const double rho = prm[4] * (1.0 + prm[0] * gg);
+            const double pr  = prm[5] * std::pow(1.0 + prm[0] * gg, g.gamma);
+            return {rho, 0.0, 0.0, g.rhoE(rho, 0.0, pr)};
+        }
         // factory.hpp:423-454 circular explosion
         const double dx = x - prm[5], dy = y - prm[6];
         const double r  = std::sqrt(dx * dx + dy * dy);

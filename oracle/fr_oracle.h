@@ -51,7 +51,7 @@ int fro_filter(void* h, const double* u, double* out);
 /* one explicit step (stages 3: Shu-Osher SSPRK(3,3); 4: SSPRK(4,3)) + post-step filter */
 int fro_ssprk_step(void* h, double t, double dt, double* u, int stages);
 /* impose_ic: kind 0 uniform(rho,u,v,p) 1 vortex(u0,v0,beta,rv,l) 2 explosion(rhoL,pL,rhoR,pR,r,xc,yc)
- * 3 sod1d(rhoL,uL,pL,rhoR,uR,pR) */
+ * 3 sod1d(rhoL,uL,pL,rhoR,uR,pR) 4 smooth pulse(A,w,xc,yc,rho0,p0) */
 int fro_impose_ic(void* h, int kind, const double* prm, double* u);
 /* compute_error against the same kinds as exact solution at time t */
 int fro_l2_error(void* h, int kind, const double* prm, double t, const double* u, double* out);

@@ -151,16 +151,20 @@ def test_ssprk_parity(name, stages):
     ev = fr.PhysicsEvaluator(cfg)
     o = Oracle(cfg)
     u = C.random_state(cfg)
-    dt = 1e-3
+    dt = 1e-3 if cfg["dims"] == 2 else 1e-4  # random nodal states: keep the 1D CFL small
     ud = _dev(torch, u)
     uo = u.copy()
     for k in range(3):
         ev.ssprk_step(ud, dt, stages=stages)
         uo = o.ssprk_step(uo, dt, stages=stages)
     assert C.rel_l2(_host(ud), uo) <= TOL
-    # graph-captured run == stepped run (bitwise); small dt keeps the rough
-    # random state stable over 20 steps
-    dt2 = 1e-5
+    # graph-captured run == stepped run (bitwise), on a smooth state (filtered
+    # rough random states lose positivity within a few steps, as in the oracle)
+    dt2 = 1e-4
+    if cfg["dims"] == 2:
+        u = o.impose_ic(4, [0.2, 0.3 * cfg["lx"], 0.5 * cfg["lx"], 0.5 * cfg["ly"], 1.0, 1.0])
+        if cfg.get("geometry", 0) == 4:
+            u = o.impose_ic(4, [0.2, 0.6, 0.0, 0.0, 1.0, 1.0])
     ud2 = _dev(torch, u)
     ev.ssprk_run(ud2, dt2, 20, stages=stages)
     ud3 = _dev(torch, u)
@@ -320,11 +324,12 @@ def test_beuler_jfnk_vs_dense_newton(jac_type, precond):
     """One implicit step on a tiny bump mesh (reference-compatible BCs so complex
     step and tangent linearise the same map) against a dense Newton solve."""
     torch, fr, Oracle = _mods()
-    cfg = dict(C.bump_cfg(3, 2, 2), tangent_bc_mode=1)
+    cfg = dict(C.bump_cfg(4, 2, 3), tangent_bc_mode=1)
     ev = fr.PhysicsEvaluator(cfg)
     o = Oracle(cfg)
     g = 1.4
     u0 = o.impose_ic(0, [1.0, 0.5 * g ** 0.5, 0.0, 1.0])
+    u0 = u0 * (1.0 + 0.02 * np.sin(0.37 * np.arange(u0.size)))  # away from the free-stream fixed point
     dt = 0.05
     ref = dense_beuler_oracle(o, u0, dt)
     u = _dev(torch, u0)
@@ -359,7 +364,8 @@ def test_batched_sweep_matches_per_instance_and_oracle():
         b.ssprk_step(ua)
         ub.copy_(u)
         b.ssprk_step_each(ub)
-        assert np.array_equal(_host(ua), _host(ub)), (grid, div)
+        # same tile code in a different kernel: equal up to FMA-contraction choices
+        assert C.rel_l2(_host(ua), _host(ub)) <= 1e-14, (grid, div)
     for k, cfg in enumerate(cfgs):
         o = Oracle(cfg)
         ref = o.ssprk_step(u0[b.offsets[k]:b.offsets[k + 1]], dts[k])
