@@ -78,6 +78,7 @@ def main():
     import torch
 
     import fr_sdg_amd as fr
+    from fr_sdg_amd import configs as CF
 
     torch.cuda.set_device(0)
     if args.kernel_only:
@@ -97,7 +98,7 @@ def main():
     c = torch.sqrt(1.4 * pres / rho)
     smax = float(torch.max(torch.sqrt(uu * uu + vv * vv) + c))
     h = cfg["lx"] / cfg["nx"]
-    dt = 0.1 * h / smax
+    dt = CF.explicit_cfl(cfg["p"]) * h / smax  # CFL 0.05 at p = 4 (0.1 diverges within ~30 steps)
 
     u = u0.clone()
     ev.ssprk_run(u, dt, args.warmup)
@@ -152,7 +153,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "2D isentropic vortex 128x128 p=4 HLLC ideal, SSP-RK3, CFL 0.1",
+        "config": {"workload": "2D isentropic vortex 128x128 p=4 HLLC ideal, SSP-RK3, CFL 0.05",
                    "ndofs": ndof, "elements": E, "p": cfg["p"], "dt": dt,
                    "residual_evals_per_step": 3, "graph_captured": True},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -190,11 +191,12 @@ def main():
 
 def kernel_only(torch, fr, args):
     """Launch one kernel kind back to back (for rocprofv3 counter / trace runs)."""
+    from fr_sdg_amd import configs as CF
     if args.kernel_only == "large":
         cfg = dict(dims=2, nx=1024, ny=1024, p=7, lx=2.0, ly=2.0, riemann=1, geometry=fr.GEOM_WARPED,
                    geom_param=[0.05, -1.0, -1.0, 0.0], bc_type=[1, 1, 1, 1], gas_kind=1, vdw_a=0.1,
                    vdw_b=0.02)
-        prm, kind = [1.0, 1.0, 0.125, 0.1, 0.4, 0.0, 0.0], fr.IC_EXPLOSION
+        prm, kind = CF.WARPED_PRM, fr.IC_EXPLOSION
     else:
         cfg, prm, kind = vortex_cfg(), VORTEX_PRM, fr.IC_VORTEX
     ev = fr.PhysicsEvaluator(cfg)
@@ -224,8 +226,7 @@ def bench_large(torch, fr):
                geom_param=[0.05, -1.0, -1.0, 0.0], bc_type=[1, 1, 1, 1], gas_kind=1, vdw_a=0.1,
                vdw_b=0.02)
     ev = fr.PhysicsEvaluator(cfg)
-    u = ev.impose_ic(fr.IC_EXPLOSION, [1.0, 1.0, 0.125, 0.1, 0.4, 0.0, 0.0])
-    u.add_(1e-3 * torch.sin(torch.arange(u.numel(), device="cuda", dtype=torch.float64)) * u)
+    u = ev.impose_ic(fr.IC_EXPLOSION, CF.WARPED_PRM)
     R = ev.empty()
     t = time_kernel(torch, lambda: ev.residual_kernel(u, out=R), 20)
     n = 8
@@ -283,9 +284,9 @@ def bench_batch(torch, fr, steps=20):
     u = b.empty()
     dts = []
     for k, iv in enumerate(b.instances):
-        uk = iv.impose_ic(fr.IC_EXPLOSION, CF.EXPLOSION_PRM, out=b.view(u, k))
+        uk = iv.impose_ic(fr.IC_EXPLOSION, CF.EXPLOSION_SMOOTH_PRM, out=b.view(u, k))
         h = cfgs[k]["lx"] / cfgs[k]["nx"]
-        dts.append(0.1 * h / CF.sound_speed_max(uk, cfgs[k]))
+        dts.append(CF.explicit_cfl(cfgs[k]["p"]) * h / CF.sound_speed_max(uk, cfgs[k]))
     b.set_dt(dts)
     best, table = autotune.tune_batch(b, u, use_cache=False)
     s = torch.cuda.current_stream()
@@ -308,7 +309,9 @@ def bench_batch(torch, fr, steps=20):
             b.ssprk_step_each(w)
     t_e, we = timed(each)
     same = bool(torch.equal(wb, we))
-    return {"workload": "64 radial shock tubes, seeded N/p/gas, Rusanov, filter s=16 a=36, SSP-RK3",
+    ok = bool(torch.isfinite(wb).all())
+    return {"workload": "64 radial shock tubes (tanh-smoothed jump, width 0.1), seeded N/p/gas, Rusanov, "
+                        "filter s=16 a=36, SSP-RK3 at CFL 0.4/(2p+1)", "finite": ok,
             "ndofs": b.ndofs, "steps": steps, "batched_s_per_step": t_b, "per_instance_s_per_step": t_e,
             "speedup": t_e / t_b, "dof_rhs_per_s_batched": 3 * b.ndofs / t_b, "tuning": list(best),
             "tuning_table": table, "bitwise_equal": same}

@@ -204,8 +204,15 @@ namespace
         }
         const double dx = x - c.prm[5], dy = y - c.prm[6];
         const double r  = ::sqrt(dx * dx + dy * dy);
-        const double rho = (r < c.prm[4]) ? c.prm[0] : c.prm[2];
-        const double p   = (r < c.prm[4]) ? c.prm[1] : c.prm[3];
+        double       rho = (r < c.prm[4]) ? c.prm[0] : c.prm[2];
+        double       p   = (r < c.prm[4]) ? c.prm[1] : c.prm[3];
+        if (c.prm[7] > 0.0)
+        {
+            // jump smoothed over a tanh layer of width prm[7]
+            const double s = 0.5 * (1.0 - ::tanh((r - c.prm[4]) / c.prm[7]));
+            rho = c.prm[2] + (c.prm[0] - c.prm[2]) * s;
+            p   = c.prm[3] + (c.prm[1] - c.prm[3]) * s;
+        }
         st[0] = rho;
         st[1] = rho * 0.0;
         st[2] = rho * 0.0;

@@ -135,7 +135,8 @@ int frsdg_ssprk_run(frsdg_ctx* ctx, double t, double dt, double* u, int stages, 
 
 /* impose_ic (physics_impl_2D.hpp:156-201, physics_impl_1D.hpp:207-244) for the
  * presets of factory.hpp: kind 0 uniform (rho,u,v,p), 1 isentropic vortex
- * (u0,v0,beta,r_v,l), 2 circular explosion (rhoL,pL,rhoR,pR,radius,xc,yc),
+ * (u0,v0,beta,r_v,l), 2 circular explosion (rhoL,pL,rhoR,pR,radius,xc,yc,width: a
+ * width > 0 smooths the jump over a tanh layer),
  * 3 Sod 1D (rhoL,uL,pL,rhoR,uR,pR), and 4 a smooth pressure pulse at rest
  * (A,w,xc,yc,rho0,p0: rho = rho0(1+A g), p = p0(1+A g)^gamma, g = exp(-r^2/w^2)).
  * prm is a host array of 8 doubles. */

@@ -666,11 +666,18 @@ namespace fro
             const double pr  = prm[5] * std::pow(1.0 + prm[0] * gg, g.gamma);
             return {rho, 0.0, 0.0, g.rhoE(rho, 0.0, pr)};
         }
-        // factory.hpp:423-454 circular explosion
+        // factory.hpp:423-454 circular explosion; prm[7] > 0 smooths the jump over
+        // a tanh layer of that width (test/benchmark variant, not in the reference)
         const double dx = x - prm[5], dy = y - prm[6];
         const double r  = std::sqrt(dx * dx + dy * dy);
         double       rho, pres;
-        if (r < prm[4])
+        if (prm[7] > 0.0)
+        {
+            const double s = 0.5 * (1.0 - std::tanh((r - prm[4]) / prm[7]));
+            rho  = prm[2] + (prm[0] - prm[2]) * s;
+            pres = prm[3] + (prm[1] - prm[3]) * s;
+        }
+        else if (r < prm[4])
         {
             rho  = prm[0];
             pres = prm[1];
