@@ -56,7 +56,7 @@ class FrsdgDesc(ctypes.Structure):
 class NewtonOpts(ctypes.Structure):
     _fields_ = [("jac_type", ctypes.c_int), ("precond", ctypes.c_int), ("snes_rtol", ctypes.c_double),
                 ("snes_atol", ctypes.c_double), ("snes_max_it", ctypes.c_int), ("ksp_restart", ctypes.c_int),
-                ("ksp_max_it", ctypes.c_int), ("ksp_rtol", ctypes.c_double)]
+                ("ksp_max_it", ctypes.c_int), ("ksp_rtol", ctypes.c_double), ("ksp_cgs_refine", ctypes.c_int)]
 
 
 class NewtonStats(ctypes.Structure):
@@ -70,7 +70,7 @@ class NewtonStats(ctypes.Structure):
 
 # PETSc defaults: SNES rtol 1e-8, atol 1e-50, max_it 50; KSP GMRES restart 30, rtol 1e-5, max_it 10000
 NEWTON_DEFAULTS = dict(jac_type=0, precond=0, snes_rtol=1e-8, snes_atol=1e-50, snes_max_it=50, ksp_restart=30,
-                       ksp_max_it=10000, ksp_rtol=1e-5)
+                       ksp_max_it=10000, ksp_rtol=1e-5, ksp_cgs_refine=0)
 
 EXPORTS = [
     "frsdg_create", "frsdg_destroy", "frsdg_ndofs", "frsdg_get_tables", "frsdg_residual",
@@ -123,7 +123,7 @@ def load_library(path=LIB_PATH):
     L.frsdg_batch_offsets.argtypes = [vp, ctypes.POINTER(ctypes.c_longlong)]
     L.frsdg_batch_instance.argtypes = [vp, i, ctypes.POINTER(ctypes.c_void_p)]
     L.frsdg_batch_set_dt.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
-    L.frsdg_batch_set_tuning.argtypes = [vp, i, i]
+    L.frsdg_batch_set_tuning.argtypes = [vp, i, i, i]
     L.frsdg_batch_ssprk_step.argtypes = [vp, dp, vp]
     L.frsdg_batch_ssprk_run.argtypes = [vp, dp, i, vp]
     L.frsdg_batch_ssprk_step_each.argtypes = [vp, ctypes.POINTER(ctypes.c_double), dp, vp]
@@ -397,8 +397,8 @@ class Batch:
         self.dts = list(dts)
         _check(self.lib.frsdg_batch_set_dt(self.h, (ctypes.c_double * self.count)(*self.dts)))
 
-    def set_tuning(self, grid, epb_div):
-        _check(self.lib.frsdg_batch_set_tuning(self.h, grid, epb_div))
+    def set_tuning(self, grid, epb_div, mode=1):
+        _check(self.lib.frsdg_batch_set_tuning(self.h, grid, epb_div, mode))
 
     def ssprk_step(self, u):
         _check(self.lib.frsdg_batch_ssprk_step(self.h, _ptr(u), _stream()))

@@ -1,9 +1,10 @@
 """Launch-shape autotuner for the batched sweep.
 
 Knobs (frsdg_batch_set_tuning): the number of persistent workgroups of the
-stage kernel, and the tile size as a divisor of the largest elements-per-
+stage kernels, the tile size as a divisor of the largest elements-per-
 workgroup tile that fits 256 threads (smaller tiles = more, shorter work
-items, better balance across uneven instances).  Each candidate is timed over
+items, better balance across uneven instances), and the launch mode (one
+kernel per stage over all orders, or one specialised kernel per order).  Each candidate is timed over
 a few graph-free batched steps on a scratch copy of the state; the winner is
 cached in a JSON file keyed by the batch signature (per-instance mesh size,
 order, gas model) and the device name.
@@ -44,7 +45,7 @@ def _store(db):
 
 def candidates():
     ncu = torch.cuda.get_device_properties(0).multi_processor_count
-    return [(g * ncu, d) for g in (1, 2, 4, 8) for d in (1, 2)]
+    return [(g * ncu, d, m) for m in (1, 0) for g in (1, 2, 4, 8) for d in (1, 2)]
 
 
 def tune_batch(batch, u, steps=3, use_cache=True):
@@ -52,14 +53,14 @@ def tune_batch(batch, u, steps=3, use_cache=True):
     key = signature(batch)
     db = _load() if use_cache else {}
     if use_cache and key in db:
-        g, d = db[key]["best"]
-        batch.set_tuning(g, d)
-        return (g, d), db[key]["table"]
+        g, d, m = db[key]["best"]
+        batch.set_tuning(g, d, m)
+        return (g, d, m), db[key]["table"]
     scratch = u.clone()
     table = []
     s = torch.cuda.current_stream()
-    for g, d in candidates():
-        batch.set_tuning(g, d)
+    for g, d, m in candidates():
+        batch.set_tuning(g, d, m)
         scratch.copy_(u)
         batch.ssprk_step(scratch)  # warm
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -68,10 +69,10 @@ def tune_batch(batch, u, steps=3, use_cache=True):
             batch.ssprk_step(scratch)
         b.record(s)
         b.synchronize()
-        table.append([g, d, a.elapsed_time(b) / steps])
-    best = min(table, key=lambda r: r[2])
-    batch.set_tuning(best[0], best[1])
-    db[key] = {"best": best[:2], "table": table}
+        table.append([g, d, m, a.elapsed_time(b) / steps])
+    best = min(table, key=lambda r: r[3])
+    batch.set_tuning(best[0], best[1], best[2])
+    db[key] = {"best": best[:3], "table": table}
     if use_cache:
         _store(db)
-    return (best[0], best[1]), table
+    return (best[0], best[1], best[2]), table

@@ -1,9 +1,11 @@
 // frsdg_batch.hip — many independent FR problems (uneven mesh size, order and
-// gas model) advanced together: one persistent kernel per SSP-RK stage walks
-// the concatenated (instance, element-tile) work list; each tile dispatches on
-// its instance's n = p+1 to the same tile function the single-problem kernels
-// use.  The last stage carries the post-step spectral filter.  A whole step is
-// 3 launches regardless of the number of instances (vs 3 per instance), and
+// gas model) advanced together by persistent kernels that walk the
+// concatenated (instance, element-tile) work list with the same tile function
+// the single-problem kernels use; the last stage carries the post-step
+// spectral filter.  Two launch modes: one kernel per stage over all orders
+// (each tile dispatches on its instance's n = p+1; LDS and registers sized for
+// the largest case), or one kernel per stage and order (specialised).  A step
+// is 3 (or 3 x #orders) launches regardless of the number of instances, and
 // frsdg_batch_ssprk_run replays captured steps from a hipGraph.
 #include "frsdg_ctx.hpp"
 
@@ -82,6 +84,35 @@ namespace
         }
     }
 
+    // one launch per order: the tiles of all instances with n = N, fully
+    // specialised on N (registers and LDS sized for this order only)
+    template <int N>
+    __global__ __launch_bounds__(256) void k_batch_stage_n(const BatchInst* __restrict__ tab,
+                                                           const int* __restrict__ gidx,
+                                                           const long long* __restrict__ gtile0, int ng,
+                                                           long long total, StageArgs A)
+    {
+        __shared__ TileSmem<N, double> sm;
+        const long long slot = xcd_tile(blockIdx.x, gridDim.x);
+        for (long long t = slot; t < total; t += gridDim.x)
+        {
+            int lo = 0, hi = ng - 1;
+            while (lo < hi)
+            {
+                const int mid = (lo + hi + 1) >> 1;
+                if (gtile0[mid] <= t)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const int        k  = gidx[lo];
+            const BatchInst& I  = tab[k];
+            const IOStage    io = {A.us + I.off, A.u0 + I.off, A.out + I.off, A.a0, A.b, A.cfac * A.dt[k]};
+            res2d_tile<N, IOStage>(I.P, io, t - gtile0[lo], I.epb, A.last && I.filt, sm);
+            __syncthreads();
+        }
+    }
+
     int max_epb(int n)
     {
         switch (n)
@@ -116,6 +147,17 @@ struct frsdg_batch
     hipGraphExec_t          gexec = nullptr;
     double*                 g_u   = nullptr;
     int                     g_ok  = 0;
+    int                     mode  = 1; // 0 one launch for all orders, 1 one launch per order
+    // per-order groups
+    struct Group
+    {
+        int        n;
+        int        ng;
+        long long  total;
+        int*       d_idx = nullptr;
+        long long* d_t0  = nullptr;
+    };
+    std::vector<Group> groups;
 };
 
 namespace
@@ -132,6 +174,35 @@ namespace
         }
         b->total_tiles = t;
         FR_TRY(hipMemcpy(b->d_tab, b->inst.data(), sizeof(BatchInst) * b->inst.size(), hipMemcpyHostToDevice));
+        for (auto& g : b->groups)
+        {
+            if (g.d_idx) (void) hipFree(g.d_idx);
+            if (g.d_t0) (void) hipFree(g.d_t0);
+        }
+        b->groups.clear();
+        for (int n = 2; n <= 9; ++n)
+        {
+            std::vector<int>       idx;
+            std::vector<long long> t0;
+            long long              tot = 0;
+            for (size_t k = 0; k < b->inst.size(); ++k)
+                if (b->inst[k].n == n)
+                {
+                    idx.push_back(static_cast<int>(k));
+                    t0.push_back(tot);
+                    tot += b->inst[k].ntiles;
+                }
+            if (idx.empty()) continue;
+            frsdg_batch::Group g;
+            g.n     = n;
+            g.ng    = static_cast<int>(idx.size());
+            g.total = tot;
+            FR_TRY(hipMalloc(&g.d_idx, sizeof(int) * idx.size()));
+            FR_TRY(hipMalloc(&g.d_t0, sizeof(long long) * t0.size()));
+            FR_TRY(hipMemcpy(g.d_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
+            FR_TRY(hipMemcpy(g.d_t0, t0.data(), sizeof(long long) * t0.size(), hipMemcpyHostToDevice));
+            b->groups.push_back(g);
+        }
         if (b->gexec) (void) hipGraphExecDestroy(b->gexec);
         if (b->graph) (void) hipGraphDestroy(b->graph);
         b->gexec = nullptr;
@@ -145,7 +216,24 @@ namespace
         auto launch     = [&](const double* us, double a0, double bb, double* out, int last)
         {
             StageArgs A{us, u, out, b->d_dt, a0, bb, 1.0, last};
-            hipLaunchKernelGGL(k_batch_stage, dim3(b->grid), dim3(256), 0, s, b->d_tab, ninst, b->total_tiles, A);
+            if (b->mode == 0)
+            {
+                hipLaunchKernelGGL(k_batch_stage, dim3(b->grid), dim3(256), 0, s, b->d_tab, ninst, b->total_tiles, A);
+                return hip_rc(hipGetLastError());
+            }
+            for (const auto& g : b->groups)
+            {
+                const unsigned gr = static_cast<unsigned>(std::min<long long>(b->grid, g.total));
+#define FR_BN(NN)                                                                                                  \
+    case NN:                                                                                                       \
+        hipLaunchKernelGGL(k_batch_stage_n<NN>, dim3(gr), dim3(256), 0, s, b->d_tab, g.d_idx, g.d_t0, g.ng, g.total, A); \
+        break;
+                switch (g.n)
+                {
+                    FR_BN(2) FR_BN(3) FR_BN(4) FR_BN(5) FR_BN(6) FR_BN(7) FR_BN(8) FR_BN(9)
+                }
+#undef FR_BN
+            }
             return hip_rc(hipGetLastError());
         };
         int rc;
@@ -203,6 +291,11 @@ int frsdg_batch_destroy(frsdg_batch* b)
     if (!b) return FRSDG_OK;
     if (b->gexec) (void) hipGraphExecDestroy(b->gexec);
     if (b->graph) (void) hipGraphDestroy(b->graph);
+    for (auto& g : b->groups)
+    {
+        if (g.d_idx) (void) hipFree(g.d_idx);
+        if (g.d_t0) (void) hipFree(g.d_t0);
+    }
     for (auto* c : b->ctx) frsdg_destroy(c);
     for (void* p : {static_cast<void*>(b->d_tab), static_cast<void*>(b->d_dt), static_cast<void*>(b->w1),
                     static_cast<void*>(b->w2)})
@@ -227,11 +320,12 @@ int frsdg_batch_instance(frsdg_batch* b, int k, frsdg_ctx** out)
     return FRSDG_OK;
 }
 
-int frsdg_batch_set_tuning(frsdg_batch* b, int grid, int epb_div)
+int frsdg_batch_set_tuning(frsdg_batch* b, int grid, int epb_div, int mode)
 {
-    if (!b || grid < 1 || epb_div < 1 || epb_div > 64) return FRSDG_ERR_INVALID;
+    if (!b || grid < 1 || epb_div < 1 || epb_div > 64 || mode < 0 || mode > 1) return FRSDG_ERR_INVALID;
     b->grid    = grid;
     b->epb_div = epb_div;
+    b->mode    = mode;
     return rebuild_tiles(b);
 }
 

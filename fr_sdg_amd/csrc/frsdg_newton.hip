@@ -36,6 +36,7 @@ struct SolverWs
     int       nb   = 0;       // reduction blocks
     // block-Jacobi
     double* B    = nullptr; // E blocks of (nb_e x nb_e), column-major, inverted in place
+    float*  Bf   = nullptr; // FP32 copy of the inverses used by the preconditioner apply
     double* pv   = nullptr; // probe vector
     double* pjv  = nullptr; // probe product
     int*    info = nullptr; // per-block singularity flag
@@ -50,6 +51,7 @@ namespace frsdg
         for (double* p : {s->V, s->w, s->z, s->un, s->ud, s->F, s->du, s->ut, s->part, s->red, s->B, s->pv, s->pjv})
             if (p) (void) hipFree(p);
         if (s->info) (void) hipFree(s->info);
+        if (s->Bf) (void) hipFree(s->Bf);
         delete s;
         c->ws = nullptr;
     }
@@ -270,12 +272,19 @@ namespace
             __syncthreads();
         }
     }
-    // z_e = Binv_e x_e (column-major block: coalesced over rows), or Binv_e^T x_e
-    __global__ void k_block_apply(long long E, int nd_e, const double* __restrict__ Binv, const double* __restrict__ x,
+    __global__ void k_to_float(long long n, const double* __restrict__ a, float* __restrict__ b)
+    {
+        const long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+        if (i < n) b[i] = static_cast<float>(a[i]);
+    }
+    // z_e = Binv_e x_e (column-major block: coalesced over rows), or Binv_e^T x_e.
+    // The inverses are stored in FP32 (a preconditioner; halves its HBM traffic),
+    // the products accumulate in FP64.
+    __global__ void k_block_apply(long long E, int nd_e, const float* __restrict__ Binv, const double* __restrict__ x,
                                   double* __restrict__ z, int transpose)
     {
         const long long e = blockIdx.x;
-        const double* A = Binv + e * nd_e * nd_e;
+        const float*  A  = Binv + e * nd_e * nd_e;
         const double* xe = x + e * nd_e;
         for (int r = threadIdx.x; r < nd_e; r += blockDim.x)
         {
@@ -314,7 +323,7 @@ namespace
             if (o->precond == 1)
             {
                 const int nd_e = static_cast<int>(c->nd / c->E);
-                hipLaunchKernelGGL(k_block_apply, dim3(static_cast<unsigned>(c->E)), dim3(nd_e <= 64 ? 64 : (nd_e <= 128 ? 128 : 256)), 0, st, c->E, nd_e, s->B, x, z, transpose);
+                hipLaunchKernelGGL(k_block_apply, dim3(static_cast<unsigned>(c->E)), dim3(nd_e <= 64 ? 64 : (nd_e <= 128 ? 128 : 256)), 0, st, c->E, nd_e, s->Bf, x, z, transpose);
                 return hip_rc(hipGetLastError());
             }
             if (z != x) FR_TRY(hipMemcpyAsync(z, x, sizeof(double) * s->N, hipMemcpyDeviceToDevice, st));
@@ -359,6 +368,8 @@ namespace
                 }
             transpose = keep;
             hipLaunchKernelGGL(k_block_inverse, dim3(static_cast<unsigned>(c->E)), dim3(256), 0, st, nd_e, s->B, s->info);
+            if (!s->Bf) FR_TRY(hipMalloc(&s->Bf, sizeof(float) * nb));
+            hipLaunchKernelGGL(k_to_float, dim3(grid(nb)), dim3(256), 0, st, nb, s->B, s->Bf);
             return hip_rc(hipGetLastError());
         }
 
@@ -401,16 +412,20 @@ namespace
                     int rc = prec(s->V + j * N, s->z);
                     if (rc) return rc;
                     if ((rc = jv(s->z, s->w))) return rc;
-                    // classical Gram-Schmidt: h = V^T w; w -= V h; ||w||
-                    if ((rc = mdot(s->V, j + 1, s->w))) return rc;
-                    hipLaunchKernelGGL(k_maxpy_norm, dim3(s->nb), dim3(RB), 0, st, N, s->V, j + 1, s->red, s->w, s->nb,
-                                       s->part);
-                    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RB), 0, st, s->nb, s->part, s->red + (m + 1));
-                    std::vector<double> hh(j + 1);
+                    // classical Gram-Schmidt: h = V^T w; w -= V h; ||w|| (optionally a second pass)
+                    std::vector<double> hh(j + 1, 0.0), h2(j + 1);
                     double nw2 = 0.0;
-                    FR_TRY(hipMemcpyAsync(hh.data(), s->red, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, st));
-                    FR_TRY(hipMemcpyAsync(&nw2, s->red + (m + 1), sizeof(double), hipMemcpyDeviceToHost, st));
-                    FR_TRY(hipStreamSynchronize(st));
+                    for (int pass = 0; pass < (o->ksp_cgs_refine ? 2 : 1); ++pass)
+                    {
+                        if ((rc = mdot(s->V, j + 1, s->w))) return rc;
+                        hipLaunchKernelGGL(k_maxpy_norm, dim3(s->nb), dim3(RB), 0, st, N, s->V, j + 1, s->red, s->w,
+                                           s->nb, s->part);
+                        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RB), 0, st, s->nb, s->part, s->red + (m + 1));
+                        FR_TRY(hipMemcpyAsync(h2.data(), s->red, sizeof(double) * (j + 1), hipMemcpyDeviceToHost, st));
+                        FR_TRY(hipMemcpyAsync(&nw2, s->red + (m + 1), sizeof(double), hipMemcpyDeviceToHost, st));
+                        FR_TRY(hipStreamSynchronize(st));
+                        for (int i = 0; i <= j; ++i) hh[i] += h2[i];
+                    }
                     const double hn = std::sqrt(nw2);
                     for (int i = 0; i <= j; ++i) H[i * m + j] = hh[i];
                     H[(j + 1) * m + j] = hn;

@@ -163,6 +163,7 @@ typedef struct frsdg_newton_opts
     int    ksp_restart; /* PETSc default 30 */
     int    ksp_max_it;  /* PETSc default 10000 */
     double ksp_rtol;    /* PETSc default 1e-5 */
+    int    ksp_cgs_refine; /* 0 one classical Gram-Schmidt pass (PETSc default), 1 always two */
 } frsdg_newton_opts;
 
 typedef struct frsdg_newton_stats
@@ -200,8 +201,9 @@ int       frsdg_batch_offsets(const frsdg_batch* b, long long* off);
 int       frsdg_batch_instance(frsdg_batch* b, int k, frsdg_ctx** out);
 /* per-instance time steps (host array of count doubles), copied to the device; synchronous */
 int       frsdg_batch_set_dt(frsdg_batch* b, const double* dt);
-/* launch shape: persistent workgroups, and tile = (max elements per tile)/epb_div */
-int       frsdg_batch_set_tuning(frsdg_batch* b, int grid, int epb_div);
+/* launch shape: persistent workgroups, tile = (max elements per tile)/epb_div, and
+ * mode 0 (one kernel per stage over all orders) or 1 (one kernel per stage and order) */
+int       frsdg_batch_set_tuning(frsdg_batch* b, int grid, int epb_div, int mode);
 int       frsdg_batch_ssprk_step(frsdg_batch* b, double* u, void* stream);
 /* nsteps steps replayed from a captured hipGraph */
 int       frsdg_batch_ssprk_run(frsdg_batch* b, double* u, int nsteps, void* stream);

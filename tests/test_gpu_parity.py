@@ -371,14 +371,14 @@ def test_batched_sweep_matches_per_instance_and_oracle():
     u0 = _host(u).copy()
     ua = u.clone()
     ub = u.clone()
-    for grid, div in [(1024, 1), (64, 2), (7, 1)]:
-        b.set_tuning(grid, div)
+    for grid, div, mode in [(1024, 1, 1), (64, 2, 1), (1024, 1, 0), (7, 2, 0)]:
+        b.set_tuning(grid, div, mode)
         ua.copy_(u)
         b.ssprk_step(ua)
         ub.copy_(u)
         b.ssprk_step_each(ub)
         # same tile code in a different kernel: equal up to FMA-contraction choices
-        assert C.rel_l2(_host(ua), _host(ub)) <= 1e-14, (grid, div)
+        assert C.rel_l2(_host(ua), _host(ub)) <= 1e-14, (grid, div, mode)
     for k, cfg in enumerate(cfgs):
         o = Oracle(cfg)
         ref = o.ssprk_step(u0[b.offsets[k]:b.offsets[k + 1]], dts[k])
