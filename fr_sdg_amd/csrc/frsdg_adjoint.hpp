@@ -45,8 +45,10 @@ namespace frsdg
         for (int k = 0; k < N; ++k)
         {
             const int node = (f < 2) ? (idx * N + k) : (k * N + idx); // x faces: row l=idx; y faces: column k=idx
+#pragma unroll
             for (int q = 0; q < 4; ++q) acc[q] += we[node * 4 + q] * g[k];
         }
+#pragma unroll
         for (int q = 0; q < 4; ++q) wb[t * 4 + q] = acc[q];
     }
 
@@ -66,10 +68,12 @@ namespace frsdg
         for (int i = 0; i < 4; ++i)
         {
             Dual Uo[4], Un[4], Fs[4], F[4], G[4];
+#pragma unroll
             for (int q = 0; q < 4; ++q) Uo[q] = {in.uo[q], q == i ? 1.0 : 0.0};
             if (in.bnd)
                 bc_exterior(bc, in.dir, Uo, in.nx, in.ny, Un);
             else
+#pragma unroll
                 for (int q = 0; q < 4; ++q) Un[q] = {in.un[q], 0.0};
             if (in.own_right)
                 riemann2(g, kind, Un, Uo, in.nx, in.ny, Fs);
@@ -77,6 +81,7 @@ namespace frsdg
                 riemann2(g, kind, Uo, Un, in.nx, in.ny, Fs);
             flux2(g, Uo, F, G);
             double s = 0.0;
+#pragma unroll
             for (int q = 0; q < 4; ++q) s += Fs[q].d * cs[q] + (F[q].d * in.nx + G[q].d * in.ny) * cn[q];
             ub[i] += s;
         }
@@ -88,6 +93,7 @@ namespace frsdg
         for (int i = 0; i < 4; ++i)
         {
             Dual Uo[4], Un[4], Fs[4];
+#pragma unroll
             for (int q = 0; q < 4; ++q)
             {
                 Uo[q] = {in.uo[q], q == i ? 1.0 : 0.0};
@@ -98,6 +104,7 @@ namespace frsdg
             else
                 riemann2(g, kind, Uo, Un, in.nx, in.ny, Fs);
             double s = 0.0;
+#pragma unroll
             for (int q = 0; q < 4; ++q) s += Fs[q].d * cs[q];
             ub[i] += s;
         }
@@ -131,7 +138,7 @@ namespace frsdg
             ld4d(u + (e * NN + node) * 4, U);
             double mt[4];
             if (P.curved)
-                ld4d(P.vmet + (e * NN + node) * 4, mt);
+                vmet_node<N>(P.vmet, e, k, l, mt);
             else
             {
                 mt[0] = P.xi_x;
@@ -139,6 +146,7 @@ namespace frsdg
                 mt[2] = 0.0;
                 mt[3] = P.eta_y;
             }
+#pragma unroll
             for (int q = 0; q < 4; ++q)
             {
                 sA[el][node][q]  = mt[0] * W[q]; // xi_x  -> d/dxi  of F
@@ -225,6 +233,7 @@ namespace frsdg
             in.dir       = dir;
             const double* wbo = wb + ((eg * 4 + face) * N + idx) * 4;
             double        cs[4], cn[4], ub[4] = {0, 0, 0, 0};
+#pragma unroll
             for (int q = 0; q < 4; ++q)
             {
                 cs[q] = m0[2] * wbo[q];
@@ -238,6 +247,7 @@ namespace frsdg
                 const double* wbn = wb + ((en * 4 + nface) * N + idx) * 4;
                 if (m1[0] == m0[0] && m1[1] == m0[1] && m1[2] == m0[2])
                 {
+#pragma unroll
                     for (int q = 0; q < 4; ++q) cs[q] += m0[2] * wbn[q];
                 }
                 else
@@ -248,17 +258,20 @@ namespace frsdg
                     in2.nx        = m1[0];
                     in2.ny        = m1[1];
                     double c2[4];
+#pragma unroll
                     for (int q = 0; q < 4; ++q) c2[q] = m1[2] * wbn[q];
                     riemann_adjoint(P.gas, P.riemann, in2, c2, ub);
                 }
             }
             face_adjoint(P.gas, P.riemann, P.bc, in, cs, cn, ub);
+#pragma unroll
             for (int q = 0; q < 4; ++q) sFc[fe][face][idx][q] = ub[q];
         }
         __syncthreads();
         if (!act) return;
         // ---- volume: Fbar, Gbar at this node ----
         double Fb[4], Gb[4];
+#pragma unroll
         for (int q = 0; q < 4; ++q)
         {
             double f = 0.0, g = 0.0;
@@ -275,13 +288,16 @@ namespace frsdg
         for (int i = 0; i < 4; ++i)
         {
             Dual Ud[4], F[4], G[4];
+#pragma unroll
             for (int q = 0; q < 4; ++q) Ud[q] = {U[q], q == i ? 1.0 : 0.0};
             flux2(P.gas, Ud, F, G);
             double s = 0.0;
+#pragma unroll
             for (int q = 0; q < 4; ++q) s += F[q].d * Fb[q] + G[q].d * Gb[q];
             ub[i] = s;
         }
         // ---- gather face contributions of this node ----
+#pragma unroll
         for (int q = 0; q < 4; ++q)
         {
             double s = ub[q];
@@ -306,6 +322,7 @@ namespace frsdg
         for (int i = 0; i < 3; ++i)
         {
             Dual A[3], B[3], Fs[3];
+#pragma unroll
             for (int q = 0; q < 3; ++q)
             {
                 A[q] = {a[q], (!wrt_b && q == i) ? 1.0 : 0.0};
@@ -320,6 +337,7 @@ namespace frsdg
             }
             riemann1(g, kind, A, B, Fs);
             double s = 0.0;
+#pragma unroll
             for (int q = 0; q < 3; ++q) s += Fs[q].d * c[q];
             ub[i] += s;
         }
@@ -329,9 +347,11 @@ namespace frsdg
         for (int i = 0; i < 3; ++i)
         {
             Dual A[3], F[3];
+#pragma unroll
             for (int q = 0; q < 3; ++q) A[q] = {a[q], q == i ? 1.0 : 0.0};
             flux1(g, A, F);
             double s = 0.0;
+#pragma unroll
             for (int q = 0; q < 3; ++q) s += F[q].d * c[q];
             ub[i] += s;
         }
@@ -350,12 +370,14 @@ namespace frsdg
         const int ep = (e == 0) ? P.ne - 1 : e - 1, en = (e == P.ne - 1) ? 0 : e + 1;
         const bool lb = (e == 0) && !P.periodic, rb = (e == P.ne - 1) && !P.periodic;
         double ub[3][N];
+#pragma unroll
         for (int q = 0; q < 3; ++q)
             for (int m = 0; m < N; ++m) ub[q][m] = 0.0;
         // volume: R_i += ij * D[i][j] F_j
         for (int jn = 0; jn < N; ++jn)
         {
             double c[3], a[3], t[3] = {0, 0, 0};
+#pragma unroll
             for (int q = 0; q < 3; ++q)
             {
                 double s = 0.0;
@@ -364,10 +386,12 @@ namespace frsdg
                 a[q] = u[b + q * N + jn];
             }
             flux1_adj(P.gas, a, c, t);
+#pragma unroll
             for (int q = 0; q < 3; ++q) ub[q][jn] += t[q];
         }
         // lifted cotangents of own faces and neighbours' copies
         double cL[3], cR[3], cLn[3], cRp[3];
+#pragma unroll
         for (int q = 0; q < 3; ++q)
         {
             double sl = 0, sr = 0, sln = 0, srp = 0;
@@ -384,6 +408,7 @@ namespace frsdg
             cRp[q] = P.inv_jac * srp;
         }
         double first[3], last[3], prev[3], next[3];
+#pragma unroll
         for (int q = 0; q < 3; ++q)
         {
             first[q] = u[b + q * N];
@@ -394,6 +419,7 @@ namespace frsdg
         // left face: F*(prev, first) appears in R_e (cL) and, if interior, in R_{e-1} (cRp)
         {
             double c[3], t[3] = {0, 0, 0}, mc[3];
+#pragma unroll
             for (int q = 0; q < 3; ++q)
             {
                 c[q]  = cL[q] + (lb ? 0.0 : cRp[q]);
@@ -401,11 +427,13 @@ namespace frsdg
             }
             riemann1_adj(P.gas, P.riemann, prev, first, 1, c, t, lb ? &P.bc : nullptr, 0);
             flux1_adj(P.gas, first, mc, t);
+#pragma unroll
             for (int q = 0; q < 3; ++q) ub[q][0] += t[q];
         }
         // right face: F*(last, next) appears in R_e (cR) and, if interior, in R_{e+1} (cLn)
         {
             double c[3], t[3] = {0, 0, 0}, mc[3];
+#pragma unroll
             for (int q = 0; q < 3; ++q)
             {
                 c[q]  = cR[q] + (rb ? 0.0 : cLn[q]);
@@ -413,8 +441,10 @@ namespace frsdg
             }
             riemann1_adj(P.gas, P.riemann, last, next, 0, c, t, rb ? &P.bc : nullptr, 1);
             flux1_adj(P.gas, last, mc, t);
+#pragma unroll
             for (int q = 0; q < 3; ++q) ub[q][N - 1] += t[q];
         }
+#pragma unroll
         for (int q = 0; q < 3; ++q)
             for (int m = 0; m < N; ++m) out[b + q * N + m] = sigma * w[b + q * N + m] + ub[q][m];
     }

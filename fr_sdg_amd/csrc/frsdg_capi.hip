@@ -29,10 +29,12 @@ namespace
         const double cx = hx * (static_cast<double>(i) + 0.5 * (1.0 + csi[k]));
         const double cy = hy * (static_cast<double>(j) + 0.5 * (1.0 + csi[l]));
         const VolM m    = composite_metrics(g, nx, ny, cx, cy);
-        vmet[t * 4 + 0] = m.xi_x;
-        vmet[t * 4 + 1] = m.xi_y;
-        vmet[t * 4 + 2] = m.eta_x;
-        vmet[t * 4 + 3] = m.eta_y;
+        // [E][4][NN]: xi_x, xi_y row-major, eta_x, eta_y column-major (vmet_node)
+        double* b                  = vmet + e * 4 * NN;
+        b[l * n + k]               = m.xi_x;
+        b[NN + l * n + k]          = m.xi_y;
+        b[2 * NN + k * n + l]      = m.eta_x;
+        b[3 * NN + k * n + l]      = m.eta_y;
     }
     // face normal (unit contravariant direction) and ||grad xi|| / ||grad eta||
     // at face points (physics_impl_2D.hpp:697-719, 805-819)

@@ -5,8 +5,11 @@
 //     one thread per solution node; the element blocks are contiguous in HBM
 //     so the 32-byte node loads/stores of a workgroup are fully coalesced;
 //   * each thread evaluates the Euler fluxes F,G of its node once and stages
-//     them in LDS; the derivative-matrix contraction reads them from LDS
-//     (D itself is staged in LDS too);
+//     them in LDS; the derivative-matrix contraction then runs as line tasks
+//     (one element row or column, one equation per thread): the line is read
+//     from LDS once into registers, D comes through the scalar cache, and the
+//     metric-weighted derivatives of all N nodes of the line go back to LDS,
+//     so each flux value is read twice instead of 2N (4N curved) times;
 //   * "owner computes" faces: every element evaluates the Riemann flux of
 //     its own four faces (4n face-node tasks), gathering the neighbour trace
 //     straight from global memory.  A shared face is evaluated by both
@@ -37,7 +40,7 @@ namespace frsdg
         int       curved;
         int       riemann;
         double    xi_x, eta_y, sc_x, sc_y; // affine constants
-        const double* __restrict__ vmet;   // [E][n*n][4]: xi_x, xi_y, eta_x, eta_y
+        const double* __restrict__ vmet;   // [E][4][n*n] (vmet_node)
         const double* __restrict__ fxm;    // [ny][nx+1][n][4]: nx, ny, scale, 0
         const double* __restrict__ fym;    // [ny+1][nx][n][4]
         const double* __restrict__ tab;    // D[n*n], dL[n], dR[n], filt[n*n]
@@ -90,9 +93,11 @@ namespace frsdg
             {
                 double a[4];
                 ld4d(add + i, a);
+#pragma unroll
                 for (int q = 0; q < 4; ++q) o[q] = a[q] + r[q];
             }
             else
+#pragma unroll
                 for (int q = 0; q < 4; ++q) o[q] = scale * r[q];
             st4d(out + i, o);
         }
@@ -115,14 +120,17 @@ namespace frsdg
         }
         __device__ void val4(long long i, const double own[4], const double r[4], double o[4]) const
         {
+#pragma unroll
             for (int q = 0; q < 4; ++q) o[q] = own[q] + cdt * (-r[q]);
             if (a0 != 0.0)
             {
                 double z[4];
                 ld4d(u0 + i, z);
+#pragma unroll
                 for (int q = 0; q < 4; ++q) o[q] = a0 * z[q] + b * o[q];
             }
             else
+#pragma unroll
                 for (int q = 0; q < 4; ++q) o[q] = b * o[q];
         }
         __device__ void put4(long long i, const double o[4]) const { st4d(out + i, o); }
@@ -148,12 +156,14 @@ namespace frsdg
             double a[4], b[4];
             ld4d(u + i, a);
             ld4d(v + i, b);
+#pragma unroll
             for (int q = 0; q < 4; ++q) o[q] = {a[q], b[q]};
         }
         __device__ void st(long long i, Dual own, Dual r) const { out[i] = sigma * own.d + r.d; }
         __device__ void st4(long long i, const Dual own[4], const Dual r[4]) const
         {
             double o[4];
+#pragma unroll
             for (int q = 0; q < 4; ++q) o[q] = sigma * own[q].d + r[q].d;
             st4d(out + i, o);
         }
@@ -173,6 +183,7 @@ namespace frsdg
             double a[4], b[4];
             ld4d(u + i, a);
             ld4d(v + i, b);
+#pragma unroll
             for (int q = 0; q < 4; ++q) o[q] = {a[q], b[q] * h};
         }
         __device__ void st(long long i, Cplx own, Cplx r) const
@@ -182,6 +193,7 @@ namespace frsdg
         __device__ void st4(long long i, const Cplx own[4], const Cplx r[4]) const
         {
             double o[4];
+#pragma unroll
             for (int q = 0; q < 4; ++q) o[q] = sigma * own[q].im * inv_h + r[q].im * inv_h;
             st4d(out + i, o);
         }
@@ -213,23 +225,261 @@ namespace frsdg
     // ------------------------------------------------------------------
     // 2D residual: physics_impl_2D.hpp:267-357 (volume) + :689-899 (faces)
     // ------------------------------------------------------------------
-    // LDS of one residual tile
+    // LDS of one residual tile.  Node (k, l) of tile element el, equation q,
+    // lives at X[el][l][k * 4 + q]; rows are padded so that the row stride is
+    // an odd multiple of 32 bytes (line tasks of one wavefront then spread
+    // over all LDS banks).
     template <int N, class S>
     struct TileSmem
     {
         static constexpr int NN = N * N, EPB = Tile2D<N, S>::EPB;
-        S      U[EPB][NN][4];
-        S      F[EPB][NN][4];
-        S      G[EPB][NN][4];
+        static constexpr int RB = N * 4 * static_cast<int>(sizeof(S));                 // row bytes
+        static constexpr int RP = N * 4 + ((RB % 64) == 0 ? 32 / static_cast<int>(sizeof(S)) : 0);
+        S      U[EPB][N][RP]; // state; after the face phase: xi-line part of the volume term
+        S      F[EPB][N][RP];
+        S      G[EPB][N][RP];
+        S      V[EPB][N][RP]; // eta-line part of the volume term
         S      Del[EPB][4][N][4];
-        double D[NN], dL[N], dR[N], Fm[NN];
-        int    I[EPB], J[EPB]; // element coordinates of the tile
+        double dL[N], dR[N], Fm[NN];
     };
+
+    // Metrics of node (k, l) of element e.  Layout [E][4][NN]: xi_x, xi_y
+    // row-major (l * N + k), eta_x, eta_y column-major (k * N + l), so that a
+    // xi-line reads its row and an eta-line its column contiguously.
+    template <int N>
+    __device__ __forceinline__ void vmet_node(const double* __restrict__ vm, long long e, int k, int l, double mt[4])
+    {
+        constexpr int NN = N * N;
+        const double* b  = vm + e * 4 * NN;
+        mt[0]            = b[l * N + k];
+        mt[1]            = b[NN + l * N + k];
+        mt[2]            = b[2 * NN + k * N + l];
+        mt[3]            = b[3 * NN + k * N + l];
+    }
+
+    // One face-node task of a tile (element fe, face, node idx along the face):
+    // geometry, and for interior faces the neighbour's trace gathered from HBM.
+    template <int N, class S>
+    struct FaceTask
+    {
+        int    fe, face, idx, ok, ol, dir;
+        bool   live, bnd;
+        double fnx, fny, sc;
+        S      Un[4];
+    };
+
+    // element coordinates of the first element of a tile (uniform; 32-bit:
+    // E < 2^31 is checked at setup)
+    struct TileOrigin
+    {
+        int i0, j0;
+    };
+    __device__ __forceinline__ TileOrigin tile_origin(const Res2DP& P, long long blk, int epb)
+    {
+        const int e0 = static_cast<int>(blk * epb);
+        const int j0 = e0 / P.nx;
+        return {e0 - j0 * P.nx, j0};
+    }
+
+    template <int N, class IO>
+    __device__ __forceinline__ void face_task(const Res2DP& P, const IO& io, long long blk, int epb, TileOrigin o,
+                                              int t, FaceTask<N, typename IO::S>& f)
+    {
+        constexpr int NN   = N * N;
+        f.fe               = t / (4 * N);
+        const int       ft = t - f.fe * 4 * N;
+        const long long eg = blk * epb + f.fe;
+        f.live             = (f.fe < epb) && (eg < P.E);
+        if (!f.live) return;
+        f.face = ft / N;
+        f.idx  = ft - f.face * N;
+        // element coordinates: walk from the tile origin (one wrap at most
+        // when nx >= epb)
+        int i = o.i0 + f.fe, j = o.j0;
+        while (i >= P.nx)
+        {
+            i -= P.nx;
+            ++j;
+        }
+        const int idx = f.idx;
+        int       nk, nl, ni = i, nj = j;
+        if (f.face == 0)
+        {
+            f.ok = 0; f.ol = idx; nk = N - 1; nl = idx; ni = i - 1; f.dir = 0;
+            f.bnd = (i == 0) && !P.per_x;
+            if (ni < 0) ni += P.nx;
+        }
+        else if (f.face == 1)
+        {
+            f.ok = N - 1; f.ol = idx; nk = 0; nl = idx; ni = i + 1; f.dir = 1;
+            f.bnd = (i == P.nx - 1) && !P.per_x;
+            if (ni >= P.nx) ni -= P.nx;
+        }
+        else if (f.face == 2)
+        {
+            f.ok = idx; f.ol = 0; nk = idx; nl = N - 1; nj = j - 1; f.dir = 3;
+            f.bnd = (j == 0) && !P.per_y;
+            if (nj < 0) nj += P.ny;
+        }
+        else
+        {
+            f.ok = idx; f.ol = N - 1; nk = idx; nl = 0; nj = j + 1; f.dir = 2;
+            f.bnd = (j == P.ny - 1) && !P.per_y;
+            if (nj >= P.ny) nj -= P.ny;
+        }
+        if (f.face < 2)
+        {
+            if (P.curved)
+            {
+                const double* fm = P.fxm + ((static_cast<long long>(j) * (P.nx + 1) + i + f.face) * N + idx) * 4;
+                f.fnx = fm[0];
+                f.fny = fm[1];
+                f.sc  = fm[2];
+            }
+            else
+            {
+                f.fnx = 1.0;
+                f.fny = 0.0;
+                f.sc  = P.sc_x;
+            }
+        }
+        else
+        {
+            if (P.curved)
+            {
+                const double* fm = P.fym + ((static_cast<long long>(j + (f.face - 2)) * P.nx + i) * N + idx) * 4;
+                f.fnx = fm[0];
+                f.fny = fm[1];
+                f.sc  = fm[2];
+            }
+            else
+            {
+                f.fnx = 0.0;
+                f.fny = 1.0;
+                f.sc  = P.sc_y;
+            }
+        }
+        if (!f.bnd) io.ld4(((static_cast<long long>(nj) * P.nx + ni) * NN + nl * N + nk) * 4, f.Un);
+    }
+
+    // Riemann flux of a face task -> scaled flux jump in sm.Del
+    template <int N, class IO>
+    __device__ __forceinline__ void face_flux(const Res2DP& P, FaceTask<N, typename IO::S>& f,
+                                              TileSmem<N, typename IO::S>& sm)
+    {
+        using S = typename IO::S;
+        S Uo[4], Fs[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Uo[q] = sm.U[f.fe][f.ol][f.ok * 4 + q];
+        if (f.bnd) bc_exterior(P.bc, f.dir, Uo, f.fnx, f.fny, f.Un);
+        if (f.face == 0 || f.face == 2)
+            riemann2(P.gas, P.riemann, f.Un, Uo, f.fnx, f.fny, Fs);
+        else
+            riemann2(P.gas, P.riemann, Uo, f.Un, f.fnx, f.fny, Fs);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+            const S fn = sm.F[f.fe][f.ol][f.ok * 4 + q] * f.fnx + sm.G[f.fe][f.ol][f.ok * 4 + q] * f.fny;
+            sm.Del[f.fe][f.face][f.idx][q] = f.sc * (Fs[q] - fn);
+        }
+    }
+
+    // Volume term along one line of one element for equation q.
+    //   xi-line  (row l):    U[l][k] <- xi_x d/dxi F + xi_y d/dxi G   (k = 0..N-1)
+    //   eta-line (column k): V[l][k] <- eta_x d/deta F + eta_y d/deta G (l = 0..N-1)
+    // The line is held in registers; D comes through the scalar cache.  Affine
+    // meshes keep only d/dxi F and d/deta G (the cross terms are exact zeros).
+    template <int N, class S>
+    __device__ __forceinline__ void volume_line(const Res2DP& P, TileSmem<N, S>& sm, bool eta, int le, int ln,
+                                                int q, long long eg)
+    {
+        constexpr int NN = N * N;
+        const double* D  = P.tab;
+        if (!eta)
+        {
+            S f[N], g[N];
+#pragma unroll
+            for (int m = 0; m < N; ++m)
+            {
+                f[m] = sm.F[le][ln][m * 4 + q];
+                if (P.curved) g[m] = sm.G[le][ln][m * 4 + q];
+            }
+            if (P.curved)
+            {
+                const double* mx = P.vmet + eg * 4 * NN + ln * N;
+                const double* my = mx + NN;
+#pragma unroll
+                for (int k = 0; k < N; ++k)
+                {
+                    S dF = Sc<S>::c(0.0), dG = Sc<S>::c(0.0);
+#pragma unroll
+                    for (int m = 0; m < N; ++m)
+                    {
+                        dF = dF + D[k * N + m] * f[m];
+                        dG = dG + D[k * N + m] * g[m];
+                    }
+                    sm.U[le][ln][k * 4 + q] = mx[k] * dF + my[k] * dG;
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int k = 0; k < N; ++k)
+                {
+                    S dF = Sc<S>::c(0.0);
+#pragma unroll
+                    for (int m = 0; m < N; ++m) dF = dF + D[k * N + m] * f[m];
+                    sm.U[le][ln][k * 4 + q] = P.xi_x * dF;
+                }
+            }
+        }
+        else
+        {
+            S f[N], g[N];
+#pragma unroll
+            for (int m = 0; m < N; ++m)
+            {
+                g[m] = sm.G[le][m][ln * 4 + q];
+                if (P.curved) f[m] = sm.F[le][m][ln * 4 + q];
+            }
+            if (P.curved)
+            {
+                const double* ex = P.vmet + eg * 4 * NN + 2 * NN + ln * N;
+                const double* ey = ex + NN;
+#pragma unroll
+                for (int l = 0; l < N; ++l)
+                {
+                    S dF = Sc<S>::c(0.0), dG = Sc<S>::c(0.0);
+#pragma unroll
+                    for (int m = 0; m < N; ++m)
+                    {
+                        dF = dF + D[l * N + m] * f[m];
+                        dG = dG + D[l * N + m] * g[m];
+                    }
+                    sm.V[le][l][ln * 4 + q] = ex[l] * dF + ey[l] * dG;
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int l = 0; l < N; ++l)
+                {
+                    S dG = Sc<S>::c(0.0);
+#pragma unroll
+                    for (int m = 0; m < N; ++m) dG = dG + D[l * N + m] * g[m];
+                    sm.V[le][l][ln * 4 + q] = P.eta_y * dG;
+                }
+            }
+        }
+    }
 
     // One tile = EPB consecutive elements of one mesh, processed by one
     // workgroup.  `epb` <= EPB elements are active (the batched sweep may run
     // smaller tiles).  With `filt` (stage outputs only) the element-local
     // spectral filter of the post-step hook is applied in LDS before the store.
+    // Phases: (1) node state -> LDS, fluxes F, G -> LDS (neighbour traces of the
+    // face tasks already in flight); (2) face Riemann fluxes -> Del; (3) volume
+    // term by xi- and eta-lines; (4) per node: volume + VCJH lift -> output.
     template <int N, class IO>
     __device__ __forceinline__ void res2d_tile(const Res2DP& P, const IO& io, long long blk, int epb, bool filt,
                                                TileSmem<N, typename IO::S>& sm)
@@ -238,11 +488,8 @@ namespace frsdg
         constexpr int BS  = Tile2D<N, S>::BS;
         constexpr int NN  = N * N;
         const int     tid = threadIdx.x;
-        for (int q = tid; q < NN; q += BS)
-        {
-            sm.D[q] = P.tab[q];
-            if (filt) sm.Fm[q] = P.tab[NN + 2 * N + q];
-        }
+        if (filt)
+            for (int q = tid; q < NN; q += BS) sm.Fm[q] = P.tab[NN + 2 * N + q];
         if (tid < N)
         {
             sm.dL[tid] = P.tab[NN + tid];
@@ -254,13 +501,10 @@ namespace frsdg
         const int       l    = node / N;
         const long long e    = blk * epb + el;
         const bool      act  = (el < epb) && (e < P.E);
-        if (tid < epb)
-        {
-            // element coordinates once per element (32-bit: E < 2^31 is checked at setup)
-            const int eg = static_cast<int>(blk * epb + tid);
-            sm.J[tid]    = eg / P.nx;
-            sm.I[tid]    = eg - sm.J[tid] * P.nx;
-        }
+
+        const TileOrigin org = tile_origin(P, blk, epb);
+        FaceTask<N, S>   f0;
+        face_task<N, IO>(P, io, blk, epb, org, tid, f0);
 
         S own[4];
         if (act)
@@ -268,100 +512,48 @@ namespace frsdg
             io.ld4((e * NN + node) * 4, own);
             S F[4], G[4];
             flux2(P.gas, own, F, G);
+#pragma unroll
             for (int q = 0; q < 4; ++q)
             {
-                sm.U[el][node][q] = own[q];
-                sm.F[el][node][q] = F[q];
-                sm.G[el][node][q] = G[q];
+                sm.U[el][l][k * 4 + q] = own[q];
+                sm.F[el][l][k * 4 + q] = F[q];
+                sm.G[el][l][k * 4 + q] = G[q];
             }
         }
         __syncthreads();
 
         // ---- face tasks: 4 faces x N nodes per element ----
-        for (int t = tid; t < epb * 4 * N; t += BS)
+        if (f0.live) face_flux<N, IO>(P, f0, sm);
+        for (int t = tid + BS; t < epb * 4 * N; t += BS)
         {
-            const int       fe = t / (4 * N);
-            const int       ft = t - fe * 4 * N;
-            const long long eg = blk * epb + fe;
-            if (eg >= P.E) continue;
-            const int face = ft / N, idx = ft - (ft / N) * N;
-            const int i = sm.I[fe], j = sm.J[fe];
-            int       ok, ol, nk, nl, ni = i, nj = j, dir;
-            bool      bnd;
-            if (face == 0)
+            FaceTask<N, S> f;
+            face_task<N, IO>(P, io, blk, epb, org, t, f);
+            if (f.live) face_flux<N, IO>(P, f, sm);
+        }
+        __syncthreads();
+
+        // ---- volume term by lines (q fastest).  Curved: xi-line tasks, then
+        // eta-line tasks (4 contractions per node); affine: one task does
+        // xi-line ln of F and eta-line ln of G (2 contractions per node) ----
+        {
+            const int nlt = epb * N * 4;
+            const int nt  = P.curved ? 2 * nlt : nlt;
+            for (int t = tid; t < nt; t += BS)
             {
-                ok = 0; ol = idx; nk = N - 1; nl = idx; ni = i - 1; dir = 0;
-                bnd = (i == 0) && !P.per_x;
-                if (ni < 0) ni += P.nx;
-            }
-            else if (face == 1)
-            {
-                ok = N - 1; ol = idx; nk = 0; nl = idx; ni = i + 1; dir = 1;
-                bnd = (i == P.nx - 1) && !P.per_x;
-                if (ni >= P.nx) ni -= P.nx;
-            }
-            else if (face == 2)
-            {
-                ok = idx; ol = 0; nk = idx; nl = N - 1; nj = j - 1; dir = 3;
-                bnd = (j == 0) && !P.per_y;
-                if (nj < 0) nj += P.ny;
-            }
-            else
-            {
-                ok = idx; ol = N - 1; nk = idx; nl = 0; nj = j + 1; dir = 2;
-                bnd = (j == P.ny - 1) && !P.per_y;
-                if (nj >= P.ny) nj -= P.ny;
-            }
-            const int on = ol * N + ok;
-            double    fnx, fny, sc;
-            if (face < 2)
-            {
+                const bool      eta = t >= nlt;
+                const int       w   = eta ? t - nlt : t;
+                const int       q   = w & 3;
+                const int       le  = (w >> 2) / N;
+                const int       ln  = (w >> 2) - le * N;
+                const long long eg  = blk * epb + le;
+                if (eg >= P.E) continue;
                 if (P.curved)
-                {
-                    const double* fm =
-                        P.fxm + ((static_cast<long long>(j) * (P.nx + 1) + i + face) * N + idx) * 4;
-                    fnx = fm[0];
-                    fny = fm[1];
-                    sc  = fm[2];
-                }
+                    volume_line<N, S>(P, sm, eta, le, ln, q, eg);
                 else
                 {
-                    fnx = 1.0;
-                    fny = 0.0;
-                    sc  = P.sc_x;
+                    volume_line<N, S>(P, sm, false, le, ln, q, eg);
+                    volume_line<N, S>(P, sm, true, le, ln, q, eg);
                 }
-            }
-            else
-            {
-                if (P.curved)
-                {
-                    const double* fm =
-                        P.fym + ((static_cast<long long>(j + (face - 2)) * P.nx + i) * N + idx) * 4;
-                    fnx = fm[0];
-                    fny = fm[1];
-                    sc  = fm[2];
-                }
-                else
-                {
-                    fnx = 0.0;
-                    fny = 1.0;
-                    sc  = P.sc_y;
-                }
-            }
-            S Uo[4], Un[4], Fs[4];
-            for (int q = 0; q < 4; ++q) Uo[q] = sm.U[fe][on][q];
-            if (bnd)
-                bc_exterior(P.bc, dir, Uo, fnx, fny, Un);
-            else
-                io.ld4(((static_cast<long long>(nj) * P.nx + ni) * NN + nl * N + nk) * 4, Un);
-            if (face == 0 || face == 2)
-                riemann2(P.gas, P.riemann, Un, Uo, fnx, fny, Fs);
-            else
-                riemann2(P.gas, P.riemann, Uo, Un, fnx, fny, Fs);
-            for (int q = 0; q < 4; ++q)
-            {
-                const S fn               = sm.F[fe][on][q] * fnx + sm.G[fe][on][q] * fny;
-                sm.Del[fe][face][idx][q] = sc * (Fs[q] - fn);
             }
         }
         __syncthreads();
@@ -369,46 +561,10 @@ namespace frsdg
         S r[4];
         if (act)
         {
-            if (!P.curved)
-            {
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                {
-                    S dFx = Sc<S>::c(0.0), dGy = Sc<S>::c(0.0);
-#pragma unroll
-                    for (int m = 0; m < N; ++m)
-                    {
-                        dFx = dFx + sm.D[k * N + m] * sm.F[el][l * N + m][q];
-                        dGy = dGy + sm.D[l * N + m] * sm.G[el][m * N + k][q];
-                    }
-                    r[q] = P.xi_x * dFx + P.eta_y * dGy;
-                }
-            }
-            else
-            {
-                double mt[4];
-                ld4d(P.vmet + (e * NN + node) * 4, mt);
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                {
-                    S dFx = Sc<S>::c(0.0), dGx = Sc<S>::c(0.0), dFy = Sc<S>::c(0.0), dGy = Sc<S>::c(0.0);
-#pragma unroll
-                    for (int m = 0; m < N; ++m)
-                    {
-                        const double a = sm.D[k * N + m], b = sm.D[l * N + m];
-                        dGx = dGx + a * sm.G[el][l * N + m][q];
-                        dFx = dFx + a * sm.F[el][l * N + m][q];
-                        dGy = dGy + b * sm.G[el][m * N + k][q];
-                        dFy = dFy + b * sm.F[el][m * N + k][q];
-                    }
-                    const S px = mt[0] * dFx + mt[2] * dFy;
-                    const S py = mt[1] * dGx + mt[3] * dGy;
-                    r[q]       = px + py;
-                }
-            }
 #pragma unroll
             for (int q = 0; q < 4; ++q)
             {
+                r[q] = sm.U[el][l][k * 4 + q] + sm.V[el][l][k * 4 + q];
                 r[q] = r[q] + sm.Del[el][0][l][q] * sm.dL[k];
                 r[q] = r[q] + sm.Del[el][1][l][q] * sm.dR[k];
                 r[q] = r[q] + sm.Del[el][2][k][q] * sm.dL[l];
@@ -419,26 +575,28 @@ namespace frsdg
         {
             if (filt)
             {
-                // stage value -> LDS (U is free: its last readers, the face
-                // tasks, finished at the previous barrier), then
+                // stage value -> LDS (each node overwrites only its own slot of
+                // U, which only this thread reads in this phase), then
                 // out = 1/2 (F_x v + F_y v)
                 if (act)
                 {
                     double o[4];
                     io.val4((e * NN + node) * 4, own, r, o);
-                    for (int q = 0; q < 4; ++q) sm.U[el][node][q] = o[q];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) sm.U[el][l][k * 4 + q] = o[q];
                 }
                 __syncthreads();
                 if (act)
                 {
                     double o[4];
+#pragma unroll
                     for (int q = 0; q < 4; ++q)
                     {
                         double sx = 0.0, sy = 0.0;
                         for (int c = 0; c < N; ++c)
                         {
-                            sx += sm.Fm[k * N + c] * sm.U[el][l * N + c][q];
-                            sy += sm.Fm[l * N + c] * sm.U[el][c * N + k][q];
+                            sx += sm.Fm[k * N + c] * sm.U[el][l][c * 4 + q];
+                            sy += sm.Fm[l * N + c] * sm.U[el][c][k * 4 + q];
                         }
                         o[q] = 0.5 * (sx + sy);
                     }
@@ -492,6 +650,7 @@ namespace frsdg
         const double* dR = P.tab + N * N + N;
         const long long base = static_cast<long long>(e) * 3 * N;
         S u[3][N];
+#pragma unroll
         for (int q = 0; q < 3; ++q)
             for (int m = 0; m < N; ++m) u[q][m] = io.ld(base + q * N + m);
         S prev[3], next[3];
@@ -502,12 +661,15 @@ namespace frsdg
         if (e == 0 && !P.periodic)
             bc_exterior1(P.bc, 0, first, prev);
         else
+#pragma unroll
             for (int q = 0; q < 3; ++q) prev[q] = io.ld(static_cast<long long>(ep) * 3 * N + q * N + (N - 1));
         if (e == P.ne - 1 && !P.periodic)
             bc_exterior1(P.bc, 1, last, next);
         else
+#pragma unroll
             for (int q = 0; q < 3; ++q) next[q] = io.ld(static_cast<long long>(en) * 3 * N + q * N);
         S res[3][N];
+#pragma unroll
         for (int q = 0; q < 3; ++q)
             for (int m = 0; m < N; ++m) res[q][m] = Sc<S>::c(0.0);
         for (int jn = 0; jn < N; ++jn)
@@ -517,6 +679,7 @@ namespace frsdg
             for (int in = 0; in < N; ++in)
             {
                 const double Dij = P.inv_jac * D[in * N + jn];
+#pragma unroll
                 for (int q = 0; q < 3; ++q) res[q][in] = res[q][in] + Dij * F[q];
             }
         }
@@ -525,12 +688,14 @@ namespace frsdg
         flux1(P.gas, last, FdR);
         riemann1(P.gas, P.riemann, prev, first, FsL);
         riemann1(P.gas, P.riemann, last, next, FsR);
+#pragma unroll
         for (int q = 0; q < 3; ++q)
         {
             const S dl = FsL[q] - FdL[q], dr = FsR[q] - FdR[q];
             for (int in = 0; in < N; ++in)
                 res[q][in] = res[q][in] + P.inv_jac * (dl * dL[in] + dr * dR[in]);
         }
+#pragma unroll
         for (int q = 0; q < 3; ++q)
             for (int m = 0; m < N; ++m) io.st(base + q * N + m, u[q][m], res[q][m]);
     }
@@ -560,11 +725,13 @@ namespace frsdg
         {
             double o[4];
             ld4d(u + (e * NN + node) * 4, o);
+#pragma unroll
             for (int q = 0; q < 4; ++q) sU[el][node][q] = o[q];
         }
         __syncthreads();
         if (!act) return;
         double o[4];
+#pragma unroll
         for (int q = 0; q < 4; ++q)
         {
             double sx = 0.0, sy = 0.0;
@@ -586,6 +753,7 @@ namespace frsdg
         const double* Fm = tab + N * N + 2 * N;
         double        x[3 * N];
         for (int q = 0; q < 3 * N; ++q) x[q] = u[static_cast<long long>(e) * 3 * N + q];
+#pragma unroll
         for (int q = 0; q < 3; ++q)
             for (int r = 0; r < N; ++r)
             {

@@ -126,37 +126,37 @@ namespace frsdg
         }
         else
         {
-            const S sL   = smooth_min(unL - cL, unR - cR);
-            const S sR   = smooth_max(unL + cL, unR + cR);
-            const S rLd  = UL[0] * (sL - unL);
-            const S rRd  = UR[0] * (sR - unR);
-            const S sM   = (R.p - L.p + unL * rLd - unR * rRd) / (rLd - rRd);
-            if (Sc<S>::branch(sL) >= 0.0)
-            {
-                for (int q = 0; q < 4; ++q) fs[q] = FL[q];
-            }
-            else if (Sc<S>::branch(sM) >= 0.0)
-            {
-                const S ps  = L.p + UL[0] * (sL - unL) * (sM - unL);
-                const S inv = 1.0 / (sL - sM);
-                fs[0]       = (sM * (sL * UL[0] - FL[0])) * inv;
-                fs[1]       = (sM * (sL * mL - FL[1]) + sL * ps) * inv;
-                fs[2]       = utL * fs[0];
-                fs[3]       = (sM * (sL * UL[3] - FL[3]) + sL * ps * sM) * inv;
-            }
-            else if (Sc<S>::branch(sR) >= 0.0)
-            {
-                const S ps  = R.p + UR[0] * (sR - unR) * (sM - unR);
-                const S inv = 1.0 / (sR - sM);
-                fs[0]       = (sM * (sR * UR[0] - FR[0])) * inv;
-                fs[1]       = (sM * (sR * mR - FR[1]) + sR * ps) * inv;
-                fs[2]       = utR * fs[0];
-                fs[3]       = (sM * (sR * UR[3] - FR[3]) + sR * ps * sM) * inv;
-            }
-            else
-            {
-                for (int q = 0; q < 4; ++q) fs[q] = FR[q];
-            }
+            const S sL  = smooth_min(unL - cL, unR - cR);
+            const S sR  = smooth_max(unL + cL, unR + cR);
+            const S rLd = UL[0] * (sL - unL);
+            const S rRd = UR[0] * (sR - unR);
+            const S sM  = (R.p - L.p + unL * rLd - unR * rRd) / (rLd - rRd);
+            // Branch order of the reference: sL >= 0 -> F_L; sM >= 0 -> left
+            // star state; sR >= 0 -> right star state; else F_R.  The star
+            // state of the selected side is formed once with selects (no
+            // divergent double evaluation across the lanes of a wavefront).
+            const bool supL = Sc<S>::branch(sL) >= 0.0;
+            const bool left = Sc<S>::branch(sM) >= 0.0;
+            const bool supR = !left && !(Sc<S>::branch(sR) >= 0.0);
+            const S    sK   = Sc<S>::sel(left, sL, sR);
+            const S    unK  = Sc<S>::sel(left, unL, unR);
+            const S    utK  = Sc<S>::sel(left, utL, utR);
+            const S    rK   = Sc<S>::sel(left, UL[0], UR[0]);
+            const S    mK   = Sc<S>::sel(left, mL, mR);
+            const S    EK   = Sc<S>::sel(left, UL[3], UR[3]);
+            const S    pK   = Sc<S>::sel(left, L.p, R.p);
+            S          FK[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) FK[q] = Sc<S>::sel(left, FL[q], FR[q]);
+            const S ps  = pK + rK * (sK - unK) * (sM - unK);
+            const S inv = 1.0 / (sK - sM);
+            S       st[4];
+            st[0] = (sM * (sK * rK - FK[0])) * inv;
+            st[1] = (sM * (sK * mK - FK[1]) + sK * ps) * inv;
+            st[2] = utK * st[0];
+            st[3] = (sM * (sK * EK - FK[3]) + sK * ps * sM) * inv;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) fs[q] = Sc<S>::sel(supL, FL[q], Sc<S>::sel(supR, FR[q], st[q]));
         }
         out[0] = fs[0];
         out[1] = fs[1] * nx - fs[2] * ny;
@@ -201,6 +201,7 @@ namespace frsdg
             const S sL   = smooth_abs(uL) + aL;
             const S sR   = smooth_abs(uR) + aR;
             const S smax = smooth_max(sL, sR);
+#pragma unroll
             for (int q = 0; q < 3; ++q) out[q] = 0.5 * (FL[q] + FR[q]) - 0.5 * smax * (UR[q] - UL[q]);
             return;
         }
@@ -213,6 +214,7 @@ namespace frsdg
         const S sM  = (pR - pL + uL * rLd - uR * rRd) / (rLd - rRd);
         if (Sc<S>::branch(sL) >= 0.0)
         {
+#pragma unroll
             for (int q = 0; q < 3; ++q) out[q] = FL[q];
         }
         else if (Sc<S>::branch(sM) >= 0.0)
@@ -233,6 +235,7 @@ namespace frsdg
         }
         else
         {
+#pragma unroll
             for (int q = 0; q < 3; ++q) out[q] = FR[q];
         }
     }
@@ -273,44 +276,55 @@ namespace frsdg
     template <class S>
     FR_HD void bc_exterior(const BCP& bc, int dir, const S in[4], double nx, double ny, S ex[4])
     {
-        const int t = bc_type_of(bc, dir);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) ex[q] = in[q];
+        // each case sets the four scalars; ex is written once at the end (per-case
+        // stores to ex get merged by the optimiser into one store at a
+        // case-dependent index, which puts ex in scratch memory)
+        const int t  = bc_type_of(bc, dir);
+        S         e0 = in[0], e1 = in[1], e2 = in[2], e3 = in[3];
         if (t == BC_WALL)
         {
             // :300-308, axis = dir / 2
-            if (dir < 2)
-                ex[1] = -in[1];
-            else
-                ex[2] = -in[2];
+            const double sx = dir < 2 ? -1.0 : 1.0;
+            e1              = in[1] * sx;
+            e2              = in[2] * (0.0 - sx);
         }
         else if (t == BC_DIRICHLET)
         {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) ex[q] = Sc<S>::c(bc_state_of(bc, dir, q));
+            e0 = Sc<S>::c(bc_state_of(bc, dir, 0));
+            e1 = Sc<S>::c(bc_state_of(bc, dir, 1));
+            e2 = Sc<S>::c(bc_state_of(bc, dir, 2));
+            e3 = Sc<S>::c(bc_state_of(bc, dir, 3));
         }
         else if (t == BC_SLIPWALL)
         {
             // tests/bump.cpp:250-272: reflect the velocity about the face normal
-            S src[4];
-            for (int q = 0; q < 4; ++q) src[q] = bc.freeze ? Sc<S>::frozen(in[q]) : in[q];
-            const S uu = src[1] / src[0];
-            const S vv = src[2] / src[0];
+            const S r  = bc.freeze ? Sc<S>::frozen(in[0]) : in[0];
+            const S mu = bc.freeze ? Sc<S>::frozen(in[1]) : in[1];
+            const S mv = bc.freeze ? Sc<S>::frozen(in[2]) : in[2];
+            const S en = bc.freeze ? Sc<S>::frozen(in[3]) : in[3];
+            const S uu = mu / r;
+            const S vv = mv / r;
             const S vn = uu * nx + vv * ny;
-            ex[0]      = src[0];
-            ex[1]      = src[0] * (uu - 2.0 * vn * nx);
-            ex[2]      = src[0] * (vv - 2.0 * vn * ny);
-            ex[3]      = src[3];
+            e0         = r;
+            e1         = r * (uu - 2.0 * vn * nx);
+            e2         = r * (vv - 2.0 * vn * ny);
+            e3         = en;
         }
+        ex[0] = e0;
+        ex[1] = e1;
+        ex[2] = e2;
+        ex[3] = e3;
     }
     template <class S>
     FR_HD void bc_exterior1(const BCP& bc, int side, const S in[3], S ex[3])
     {
         const int t = bc_type_of(bc, side);
+#pragma unroll
         for (int q = 0; q < 3; ++q) ex[q] = in[q];
         if (t == BC_WALL)
             ex[1] = -in[1];
         else if (t == BC_DIRICHLET)
+#pragma unroll
             for (int q = 0; q < 3; ++q) ex[q] = Sc<S>::c(bc_state_of(bc, side, q));
     }
 } // namespace frsdg

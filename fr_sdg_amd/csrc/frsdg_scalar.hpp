@@ -102,6 +102,7 @@ namespace frsdg
         static FR_HD double val(double x) { return x; }
         static FR_HD double wrap_positive(double x) { return (DBL_MIN < x) ? x : DBL_MIN; }
         static FR_HD double frozen(double x) { return x; }
+        static FR_HD double sel(bool k, double a, double b) { return k ? a : b; }
     };
     template <>
     struct Sc<Dual>
@@ -109,8 +110,13 @@ namespace frsdg
         static FR_HD Dual c(double x) { return {x, 0.0}; }
         static FR_HD double branch(Dual x) { return x.v; }
         static FR_HD double val(Dual x) { return x.v; }
-        static FR_HD Dual wrap_positive(Dual x) { return (DBL_MIN < x.v) ? x : Dual{DBL_MIN, 0.0}; }
+        static FR_HD Dual wrap_positive(Dual x)
+        {
+            const bool k = DBL_MIN < x.v; // component-wise selects keep aggregates out of memory
+            return {k ? x.v : DBL_MIN, k ? x.d : 0.0};
+        }
         static FR_HD Dual frozen(Dual x) { return {x.v, 0.0}; }
+        static FR_HD Dual sel(bool k, Dual a, Dual b) { return {k ? a.v : b.v, k ? a.d : b.d}; }
     };
     template <>
     struct Sc<Cplx>
@@ -118,8 +124,13 @@ namespace frsdg
         static FR_HD Cplx c(double x) { return {x, 0.0}; }
         static FR_HD double branch(Cplx x) { return x.re + x.im; }
         static FR_HD double val(Cplx x) { return x.re; }
-        static FR_HD Cplx wrap_positive(Cplx x) { return (x.re < DBL_MIN) ? Cplx{DBL_MIN, 0.0} : x; }
+        static FR_HD Cplx wrap_positive(Cplx x)
+        {
+            const bool k = x.re < DBL_MIN;
+            return {k ? DBL_MIN : x.re, k ? 0.0 : x.im};
+        }
         static FR_HD Cplx frozen(Cplx x) { return {x.re, 0.0}; }
+        static FR_HD Cplx sel(bool k, Cplx a, Cplx b) { return {k ? a.re : b.re, k ? a.im : b.im}; }
     };
     FR_HD double sqrt_(double x) { return ::sqrt(x); }
     FR_HD Dual sqrt_(Dual x) { return sqrt(x); }
