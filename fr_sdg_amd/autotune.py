@@ -1,10 +1,11 @@
 """Launch-shape autotuner for the batched sweep.
 
 Knobs (frsdg_batch_set_tuning): the number of persistent workgroups of the
-stage kernels, the tile size as a divisor of the largest elements-per-
-workgroup tile that fits 256 threads (smaller tiles = more, shorter work
-items, better balance across uneven instances), and the launch mode (one
-kernel per stage over all orders, or one specialised kernel per order).  Each candidate is timed over
+stage kernels (or one workgroup per element tile), the element tile shape
+(the default tile of each order, or that tile with its rows halved: more,
+shorter work items, better balance across uneven instances), and the launch
+mode (one kernel per stage over all orders, or one specialised kernel per
+order and mesh kind).  Each candidate is timed over
 a few graph-free batched steps on a scratch copy of the state; the winner is
 cached in a JSON file keyed by the batch signature (per-instance mesh size,
 order, gas model) and the device name.
@@ -44,12 +45,13 @@ def _store(db):
 
 
 def candidates():
+    """(grid, epb_div, mode); grid 0 = one workgroup per element tile."""
     ncu = torch.cuda.get_device_properties(0).multi_processor_count
-    return [(g * ncu, d, m) for m in (1, 0) for g in (1, 2, 4, 8) for d in (1, 2)]
+    return [(g * ncu, d, m) for m in (1, 0) for g in (0, 1, 2, 4, 8) for d in (1, 2)]
 
 
 def tune_batch(batch, u, steps=3, use_cache=True):
-    """Pick (grid, epb_div) for `batch` by timing; returns (best, table)."""
+    """Pick (grid, epb_div, mode) for `batch` by timing; returns (best, table)."""
     key = signature(batch)
     db = _load() if use_cache else {}
     if use_cache and key in db:

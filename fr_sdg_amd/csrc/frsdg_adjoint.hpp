@@ -138,7 +138,7 @@ namespace frsdg
             ld4d(u + (e * NN + node) * 4, U);
             double mt[4];
             if (P.curved)
-                vmet_node<N>(P.vmet, e, k, l, mt);
+                ld4d(P.vmet + (e * NN + node) * 4, mt);
             else
             {
                 mt[0] = P.xi_x;

@@ -22,21 +22,30 @@ namespace
         long long tile0;  // first tile index of the instance
         long long ntiles;
         int       n;
-        int       epb;    // elements per tile
+        int       tx, ty; // element tile shape
         int       filt;   // post-step filter active
+        int       curved;
         int       pad;
     };
 
     union BatchSmem
     {
-        TileSmem<2, double> s2;
-        TileSmem<3, double> s3;
-        TileSmem<4, double> s4;
-        TileSmem<5, double> s5;
-        TileSmem<6, double> s6;
-        TileSmem<7, double> s7;
-        TileSmem<8, double> s8;
-        TileSmem<9, double> s9;
+        TileSmem<2, double, false> a2;
+        TileSmem<3, double, false> a3;
+        TileSmem<4, double, false> a4;
+        TileSmem<5, double, false> a5;
+        TileSmem<6, double, false> a6;
+        TileSmem<7, double, false> a7;
+        TileSmem<8, double, false> a8;
+        TileSmem<9, double, false> a9;
+        TileSmem<2, double, true>  c2;
+        TileSmem<3, double, true>  c3;
+        TileSmem<4, double, true>  c4;
+        TileSmem<5, double, true>  c5;
+        TileSmem<6, double, true>  c6;
+        TileSmem<7, double, true>  c7;
+        TileSmem<8, double, true>  c8;
+        TileSmem<9, double, true>  c9;
     };
 
     struct StageArgs
@@ -69,30 +78,31 @@ namespace
             const IOStage    io  = {A.us + I.off, A.u0 + I.off, A.out + I.off, A.a0, A.b, A.cfac * A.dt[lo]};
             const long long  blk = t - I.tile0;
             const bool       flt = A.last && I.filt;
+#define FR_BT(NN)                                                                                            \
+    case NN:                                                                                                 \
+        if (I.curved)                                                                                        \
+            res2d_tile<NN, IOStage, true>(I.P, io, blk, I.tx, I.ty, flt, sm.c##NN);                          \
+        else                                                                                                 \
+            res2d_tile<NN, IOStage, false>(I.P, io, blk, I.tx, I.ty, flt, sm.a##NN);                         \
+        break;
             switch (I.n)
             {
-                case 2: res2d_tile<2, IOStage>(I.P, io, blk, I.epb, flt, sm.s2); break;
-                case 3: res2d_tile<3, IOStage>(I.P, io, blk, I.epb, flt, sm.s3); break;
-                case 4: res2d_tile<4, IOStage>(I.P, io, blk, I.epb, flt, sm.s4); break;
-                case 5: res2d_tile<5, IOStage>(I.P, io, blk, I.epb, flt, sm.s5); break;
-                case 6: res2d_tile<6, IOStage>(I.P, io, blk, I.epb, flt, sm.s6); break;
-                case 7: res2d_tile<7, IOStage>(I.P, io, blk, I.epb, flt, sm.s7); break;
-                case 8: res2d_tile<8, IOStage>(I.P, io, blk, I.epb, flt, sm.s8); break;
-                case 9: res2d_tile<9, IOStage>(I.P, io, blk, I.epb, flt, sm.s9); break;
+                FR_BT(2) FR_BT(3) FR_BT(4) FR_BT(5) FR_BT(6) FR_BT(7) FR_BT(8) FR_BT(9)
             }
+#undef FR_BT
             __syncthreads(); // LDS is reused by the next tile
         }
     }
 
-    // one launch per order: the tiles of all instances with n = N, fully
-    // specialised on N (registers and LDS sized for this order only)
-    template <int N>
+    // one launch per (order, curvature): the tiles of all instances of the
+    // group, fully specialised (registers and LDS sized for this order only)
+    template <int N, bool CURVED>
     __global__ __launch_bounds__(256) void k_batch_stage_n(const BatchInst* __restrict__ tab,
                                                            const int* __restrict__ gidx,
                                                            const long long* __restrict__ gtile0, int ng,
                                                            long long total, StageArgs A)
     {
-        __shared__ TileSmem<N, double> sm;
+        __shared__ TileSmem<N, double, CURVED> sm;
         const long long slot = xcd_tile(blockIdx.x, gridDim.x);
         for (long long t = slot; t < total; t += gridDim.x)
         {
@@ -108,25 +118,9 @@ namespace
             const int        k  = gidx[lo];
             const BatchInst& I  = tab[k];
             const IOStage    io = {A.us + I.off, A.u0 + I.off, A.out + I.off, A.a0, A.b, A.cfac * A.dt[k]};
-            res2d_tile<N, IOStage>(I.P, io, t - gtile0[lo], I.epb, A.last && I.filt, sm);
+            res2d_tile<N, IOStage, CURVED>(I.P, io, t - gtile0[lo], I.tx, I.ty, A.last && I.filt, sm);
             __syncthreads();
         }
-    }
-
-    int max_epb(int n)
-    {
-        switch (n)
-        {
-            case 2: return Tile2D<2, double>::EPB;
-            case 3: return Tile2D<3, double>::EPB;
-            case 4: return Tile2D<4, double>::EPB;
-            case 5: return Tile2D<5, double>::EPB;
-            case 6: return Tile2D<6, double>::EPB;
-            case 7: return Tile2D<7, double>::EPB;
-            case 8: return Tile2D<8, double>::EPB;
-            case 9: return Tile2D<9, double>::EPB;
-        }
-        return 1;
     }
 } // namespace
 
@@ -141,7 +135,7 @@ struct frsdg_batch
     double*                 w2    = nullptr;
     long long               total_tiles = 0;
     long long               nd          = 0;
-    int                     grid        = 0; // persistent workgroups
+    int                     grid        = 0; // persistent workgroups (0: one workgroup per tile)
     int                     epb_div     = 1;
     hipGraph_t              graph = nullptr;
     hipGraphExec_t          gexec = nullptr;
@@ -152,6 +146,7 @@ struct frsdg_batch
     struct Group
     {
         int        n;
+        int        curved;
         int        ng;
         long long  total;
         int*       d_idx = nullptr;
@@ -167,9 +162,9 @@ namespace
         long long t = 0;
         for (auto& I : b->inst)
         {
-            I.epb    = std::max(1, max_epb(I.n) / b->epb_div);
+            tile_shape(I.n, b->epb_div, I.tx, I.ty);
             I.tile0  = t;
-            I.ntiles = (I.P.E + I.epb - 1) / I.epb;
+            I.ntiles = tile_count(I.P.nx, I.P.ny, I.tx, I.ty);
             t += I.ntiles;
         }
         b->total_tiles = t;
@@ -180,13 +175,14 @@ namespace
             if (g.d_t0) (void) hipFree(g.d_t0);
         }
         b->groups.clear();
-        for (int n = 2; n <= 9; ++n)
+        for (int key = 0; key < 16; ++key)
         {
+            const int              n = 2 + (key >> 1), cv = key & 1;
             std::vector<int>       idx;
             std::vector<long long> t0;
             long long              tot = 0;
             for (size_t k = 0; k < b->inst.size(); ++k)
-                if (b->inst[k].n == n)
+                if (b->inst[k].n == n && b->inst[k].curved == cv)
                 {
                     idx.push_back(static_cast<int>(k));
                     t0.push_back(tot);
@@ -194,8 +190,9 @@ namespace
                 }
             if (idx.empty()) continue;
             frsdg_batch::Group g;
-            g.n     = n;
-            g.ng    = static_cast<int>(idx.size());
+            g.n      = n;
+            g.curved = cv;
+            g.ng     = static_cast<int>(idx.size());
             g.total = tot;
             FR_TRY(hipMalloc(&g.d_idx, sizeof(int) * idx.size()));
             FR_TRY(hipMalloc(&g.d_t0, sizeof(long long) * t0.size()));
@@ -218,15 +215,21 @@ namespace
             StageArgs A{us, u, out, b->d_dt, a0, bb, 1.0, last};
             if (b->mode == 0)
             {
-                hipLaunchKernelGGL(k_batch_stage, dim3(b->grid), dim3(256), 0, s, b->d_tab, ninst, b->total_tiles, A);
+                const unsigned gr = static_cast<unsigned>(b->grid > 0 ? b->grid : b->total_tiles);
+                hipLaunchKernelGGL(k_batch_stage, dim3(gr), dim3(256), 0, s, b->d_tab, ninst, b->total_tiles, A);
                 return hip_rc(hipGetLastError());
             }
             for (const auto& g : b->groups)
             {
-                const unsigned gr = static_cast<unsigned>(std::min<long long>(b->grid, g.total));
-#define FR_BN(NN)                                                                                                  \
-    case NN:                                                                                                       \
-        hipLaunchKernelGGL(k_batch_stage_n<NN>, dim3(gr), dim3(256), 0, s, b->d_tab, g.d_idx, g.d_t0, g.ng, g.total, A); \
+                const unsigned gr = static_cast<unsigned>(b->grid > 0 ? std::min<long long>(b->grid, g.total) : g.total);
+#define FR_BN(NN)                                                                                              \
+    case NN:                                                                                                   \
+        if (g.curved)                                                                                          \
+            hipLaunchKernelGGL((k_batch_stage_n<NN, true>), dim3(gr), dim3(256), 0, s, b->d_tab, g.d_idx, g.d_t0, \
+                               g.ng, g.total, A);                                                             \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_batch_stage_n<NN, false>), dim3(gr), dim3(256), 0, s, b->d_tab, g.d_idx,       \
+                               g.d_t0, g.ng, g.total, A);                                                     \
         break;
                 switch (g.n)
                 {
@@ -266,8 +269,9 @@ int frsdg_batch_create(const frsdg_desc* descs, int count, frsdg_batch** out)
         BatchInst I{};
         I.P    = c->p2;
         I.off  = b->off.back();
-        I.n    = c->n;
-        I.filt = descs[k].apply_filter;
+        I.n      = c->n;
+        I.filt   = descs[k].apply_filter;
+        I.curved = c->p2.curved;
         b->inst.push_back(I);
         b->off.push_back(b->off.back() + c->nd);
     }
@@ -322,7 +326,7 @@ int frsdg_batch_instance(frsdg_batch* b, int k, frsdg_ctx** out)
 
 int frsdg_batch_set_tuning(frsdg_batch* b, int grid, int epb_div, int mode)
 {
-    if (!b || grid < 1 || epb_div < 1 || epb_div > 64 || mode < 0 || mode > 1) return FRSDG_ERR_INVALID;
+    if (!b || grid < 0 || epb_div < 1 || epb_div > 64 || mode < 0 || mode > 1) return FRSDG_ERR_INVALID;
     b->grid    = grid;
     b->epb_div = epb_div;
     b->mode    = mode;

@@ -29,12 +29,10 @@ namespace
         const double cx = hx * (static_cast<double>(i) + 0.5 * (1.0 + csi[k]));
         const double cy = hy * (static_cast<double>(j) + 0.5 * (1.0 + csi[l]));
         const VolM m    = composite_metrics(g, nx, ny, cx, cy);
-        // [E][4][NN]: xi_x, xi_y row-major, eta_x, eta_y column-major (vmet_node)
-        double* b                  = vmet + e * 4 * NN;
-        b[l * n + k]               = m.xi_x;
-        b[NN + l * n + k]          = m.xi_y;
-        b[2 * NN + k * n + l]      = m.eta_x;
-        b[3 * NN + k * n + l]      = m.eta_y;
+        vmet[t * 4 + 0] = m.xi_x;
+        vmet[t * 4 + 1] = m.xi_y;
+        vmet[t * 4 + 2] = m.eta_x;
+        vmet[t * 4 + 3] = m.eta_y;
     }
     // face normal (unit contravariant direction) and ||grad xi|| / ||grad eta||
     // at face points (physics_impl_2D.hpp:697-719, 805-819)
@@ -361,10 +359,14 @@ namespace
         {
             static int run(const frsdg_ctx* c, const IO& io, hipStream_t s)
             {
-                using S          = typename IO::S;
-                constexpr int BS = Tile2D<N, S>::BS, EPB = Tile2D<N, S>::EPB;
-                const long long nb = (c->E + EPB - 1) / EPB;
-                hipLaunchKernelGGL((k_res2d<N, IO>), dim3(static_cast<unsigned>(nb)), dim3(BS), 0, s, c->p2, io);
+                using T            = Tile2D<N, typename IO::S>;
+                const long long nb = tile_count(c->p2.nx, c->p2.ny, T::TX, T::TY);
+                if (c->p2.curved)
+                    hipLaunchKernelGGL((k_res2d<N, IO, true>), dim3(static_cast<unsigned>(nb)), dim3(T::BS), 0, s,
+                                       c->p2, io);
+                else
+                    hipLaunchKernelGGL((k_res2d<N, IO, false>), dim3(static_cast<unsigned>(nb)), dim3(T::BS), 0, s,
+                                       c->p2, io);
                 return hip_rc(hipGetLastError());
             }
         };
@@ -441,9 +443,14 @@ namespace
     {
         static int run(const frsdg_ctx* c, const IOStage& io, hipStream_t s)
         {
-            constexpr int EPB = Tile2D<N, double>::EPB;
-            const long long nb = (c->E + EPB - 1) / EPB;
-            hipLaunchKernelGGL((k_res2d_stage_filter<N>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, c->p2, io);
+            using T            = Tile2D<N, double>;
+            const long long nb = tile_count(c->p2.nx, c->p2.ny, T::TX, T::TY);
+            if (c->p2.curved)
+                hipLaunchKernelGGL((k_res2d_stage_filter<N, true>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, s,
+                                   c->p2, io);
+            else
+                hipLaunchKernelGGL((k_res2d_stage_filter<N, false>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, s,
+                                   c->p2, io);
             return hip_rc(hipGetLastError());
         }
     };

@@ -40,7 +40,7 @@ namespace frsdg
         int       curved;
         int       riemann;
         double    xi_x, eta_y, sc_x, sc_y; // affine constants
-        const double* __restrict__ vmet;   // [E][4][n*n] (vmet_node)
+        const double* __restrict__ vmet;   // [E][n*n][4]: xi_x, xi_y, eta_x, eta_y
         const double* __restrict__ fxm;    // [ny][nx+1][n][4]: nx, ny, scale, 0
         const double* __restrict__ fym;    // [ny+1][nx][n][4]
         const double* __restrict__ tab;    // D[n*n], dL[n], dR[n], filt[n*n]
@@ -214,13 +214,44 @@ namespace frsdg
     {
         static constexpr int value = 128;
     };
+    // largest divisor of epb that is <= sqrt(epb): rows of a 2D element tile
+    __host__ __device__ constexpr int tile_rows(int epb)
+    {
+        int best = 1;
+        for (int d = 1; d * d <= epb; ++d)
+            if (epb % d == 0) best = d;
+        return best;
+    }
+    // Elements of one workgroup: EPB = floor(BS / n^2) elements as a TX x TY
+    // block of the element grid (TX >= TY).
     template <int N, class S>
     struct Tile2D
     {
         static constexpr int BS  = BlockSize<S>::value;
         static constexpr int NN  = N * N;
         static constexpr int EPB = (BS / NN) > 0 ? (BS / NN) : 1;
+        static constexpr int TY  = tile_rows(EPB);
+        static constexpr int TX  = EPB / TY;
     };
+    // element tile shape (TX, TY) of order n in the double kernels, reduced by `div`
+    // (rows first); host helper of the launch code
+    inline void tile_shape(int n, int div, int& tx, int& ty)
+    {
+        const int epb = (256 / (n * n)) > 0 ? 256 / (n * n) : 1;
+        ty            = tile_rows(epb);
+        tx            = epb / ty;
+        for (int d = div; d > 1; d /= 2)
+        {
+            if (ty > 1)
+                ty = (ty + 1) / 2;
+            else if (tx > 1)
+                tx = (tx + 1) / 2;
+        }
+    }
+    inline long long tile_count(int nx, int ny, int tx, int ty)
+    {
+        return static_cast<long long>((nx + tx - 1) / tx) * ((ny + ty - 1) / ty);
+    }
 
     // ------------------------------------------------------------------
     // 2D residual: physics_impl_2D.hpp:267-357 (volume) + :689-899 (faces)
@@ -228,110 +259,86 @@ namespace frsdg
     // LDS of one residual tile.  Node (k, l) of tile element el, equation q,
     // lives at X[el][l][k * 4 + q]; rows are padded so that the row stride is
     // an odd multiple of 32 bytes (line tasks of one wavefront then spread
-    // over all LDS banks).
-    template <int N, class S>
+    // over all LDS banks).  Curved meshes also stage the node metrics:
+    // M[el][0|1][l][k] = xi_x|xi_y (rows), M[el][2|3][k][l] = eta_x|eta_y
+    // (columns), rows of odd length.
+    template <int N, class S, bool CURVED>
     struct TileSmem
     {
         static constexpr int NN = N * N, EPB = Tile2D<N, S>::EPB;
-        static constexpr int RB = N * 4 * static_cast<int>(sizeof(S));                 // row bytes
+        static constexpr int RB = N * 4 * static_cast<int>(sizeof(S)); // row bytes
         static constexpr int RP = N * 4 + ((RB % 64) == 0 ? 32 / static_cast<int>(sizeof(S)) : 0);
-        S      U[EPB][N][RP]; // state; after the face phase: xi-line part of the volume term
-        S      F[EPB][N][RP];
-        S      G[EPB][N][RP];
-        S      V[EPB][N][RP]; // eta-line part of the volume term
-        S      Del[EPB][4][N][4];
-        double dL[N], dR[N], Fm[NN];
+        static constexpr int MP = N + ((N % 2 == 0) ? 1 : 0);
+        S         U[EPB][N][RP]; // state; after the face phase: xi-line part of the volume term
+        S         F[EPB][N][RP];
+        S         G[EPB][N][RP];
+        S         V[EPB][N][RP]; // eta-line part of the volume term
+        S         Del[EPB][4][N][4];
+        double    M[CURVED ? EPB : 1][4][N][CURVED ? MP : 1];
+        double    dL[N], dR[N], Fm[NN];
+        long long eid[EPB];      // global element of each tile slot, -1 if outside the mesh
     };
 
-    // Metrics of node (k, l) of element e.  Layout [E][4][NN]: xi_x, xi_y
-    // row-major (l * N + k), eta_x, eta_y column-major (k * N + l), so that a
-    // xi-line reads its row and an eta-line its column contiguously.
-    template <int N>
-    __device__ __forceinline__ void vmet_node(const double* __restrict__ vm, long long e, int k, int l, double mt[4])
+    // Tile origin and shape.  Tiles cover the element grid row-major; the
+    // last tile of a row or column may hang over the mesh (its outside slots
+    // are idle).
+    struct TileGeom
     {
-        constexpr int NN = N * N;
-        const double* b  = vm + e * 4 * NN;
-        mt[0]            = b[l * N + k];
-        mt[1]            = b[NN + l * N + k];
-        mt[2]            = b[2 * NN + k * N + l];
-        mt[3]            = b[3 * NN + k * N + l];
+        int i0, j0, tx, ty;
+    };
+    __device__ __forceinline__ TileGeom tile_geom(const Res2DP& P, long long blk, int tx, int ty)
+    {
+        const int ntx = (P.nx + tx - 1) / tx;
+        const int b   = static_cast<int>(blk);
+        const int tj  = b / ntx;
+        return {(b - tj * ntx) * tx, tj * ty, tx, ty};
     }
 
-    // One face-node task of a tile (element fe, face, node idx along the face):
-    // geometry, and for interior faces the neighbour's trace gathered from HBM.
-    template <int N, class S>
+    // One face-node task of a tile.  Each face of the tile's elements is
+    // evaluated once: faces between two elements of the tile take both
+    // traces from LDS; faces on the tile rim gather the outer trace from HBM
+    // (issued at kernel start) or form it from the boundary condition.
+    template <class S>
     struct FaceTask
     {
-        int    fe, face, idx, ok, ol, dir;
-        bool   live, bnd;
+        bool   live, xdir;
+        int    idx;
+        int    elA, elB;   // tile slots left/below (A) and right/above (B), -1 if outside the tile
+        int    bdir;       // boundary direction if the outer side is a physical boundary, else -1
         double fnx, fny, sc;
-        S      Un[4];
+        S      Uext[4];    // outer trace gathered from HBM
     };
-
-    // element coordinates of the first element of a tile (uniform; 32-bit:
-    // E < 2^31 is checked at setup)
-    struct TileOrigin
-    {
-        int i0, j0;
-    };
-    __device__ __forceinline__ TileOrigin tile_origin(const Res2DP& P, long long blk, int epb)
-    {
-        const int e0 = static_cast<int>(blk * epb);
-        const int j0 = e0 / P.nx;
-        return {e0 - j0 * P.nx, j0};
-    }
 
     template <int N, class IO>
-    __device__ __forceinline__ void face_task(const Res2DP& P, const IO& io, long long blk, int epb, TileOrigin o,
-                                              int t, FaceTask<N, typename IO::S>& f)
+    __device__ __forceinline__ void face_task(const Res2DP& P, const IO& io, const TileGeom& g, int t,
+                                              FaceTask<typename IO::S>& f)
     {
-        constexpr int NN   = N * N;
-        f.fe               = t / (4 * N);
-        const int       ft = t - f.fe * 4 * N;
-        const long long eg = blk * epb + f.fe;
-        f.live             = (f.fe < epb) && (eg < P.E);
-        if (!f.live) return;
-        f.face = ft / N;
-        f.idx  = ft - f.face * N;
-        // element coordinates: walk from the tile origin (one wrap at most
-        // when nx >= epb)
-        int i = o.i0 + f.fe, j = o.j0;
-        while (i >= P.nx)
+        constexpr int NN = N * N;
+        const int     XT = (g.tx + 1) * g.ty * N;
+        const int     YT = g.tx * (g.ty + 1) * N;
+        f.live           = false;
+        if (t >= XT + YT) return;
+        f.xdir = t < XT;
+        f.bdir = -1;
+        if (f.xdir)
         {
-            i -= P.nx;
-            ++j;
-        }
-        const int idx = f.idx;
-        int       nk, nl, ni = i, nj = j;
-        if (f.face == 0)
-        {
-            f.ok = 0; f.ol = idx; nk = N - 1; nl = idx; ni = i - 1; f.dir = 0;
-            f.bnd = (i == 0) && !P.per_x;
-            if (ni < 0) ni += P.nx;
-        }
-        else if (f.face == 1)
-        {
-            f.ok = N - 1; f.ol = idx; nk = 0; nl = idx; ni = i + 1; f.dir = 1;
-            f.bnd = (i == P.nx - 1) && !P.per_x;
-            if (ni >= P.nx) ni -= P.nx;
-        }
-        else if (f.face == 2)
-        {
-            f.ok = idx; f.ol = 0; nk = idx; nl = N - 1; nj = j - 1; f.dir = 3;
-            f.bnd = (j == 0) && !P.per_y;
-            if (nj < 0) nj += P.ny;
-        }
-        else
-        {
-            f.ok = idx; f.ol = N - 1; nk = idx; nl = 0; nj = j + 1; f.dir = 2;
-            f.bnd = (j == P.ny - 1) && !P.per_y;
-            if (nj >= P.ny) nj -= P.ny;
-        }
-        if (f.face < 2)
-        {
+            // face between slot columns a-1 and a of tile row b
+            const int b  = t / ((g.tx + 1) * N);
+            const int r  = t - b * (g.tx + 1) * N;
+            const int a  = r / N;
+            f.idx        = r - a * N;
+            const int j  = g.j0 + b;
+            const int fi = g.i0 + a; // face index in [0, nx]
+            if (j >= P.ny) return;
+            const bool inA = a >= 1 && fi - 1 < P.nx;
+            const bool inB = a < g.tx && fi < P.nx;
+            if (!inA && !inB) return;
+            f.live = true;
+            f.elA  = inA ? b * g.tx + a - 1 : -1;
+            f.elB  = inB ? b * g.tx + a : -1;
             if (P.curved)
             {
-                const double* fm = P.fxm + ((static_cast<long long>(j) * (P.nx + 1) + i + f.face) * N + idx) * 4;
+                const double* fm = P.fxm + ((static_cast<long long>(j) * (P.nx + 1) + fi) * N + f.idx) * 4;
                 f.fnx = fm[0];
                 f.fny = fm[1];
                 f.sc  = fm[2];
@@ -342,12 +349,59 @@ namespace frsdg
                 f.fny = 0.0;
                 f.sc  = P.sc_x;
             }
+            int ni = -1, nk = 0;
+            if (!inA)
+            {
+                ni = fi - 1;
+                nk = N - 1;
+                if (ni < 0)
+                {
+                    if (P.per_x)
+                        ni = P.nx - 1;
+                    else
+                    {
+                        ni     = -1;
+                        f.bdir = 0;
+                    }
+                }
+            }
+            else if (!inB)
+            {
+                ni = fi;
+                nk = 0;
+                if (ni >= P.nx)
+                {
+                    if (P.per_x)
+                        ni = 0;
+                    else
+                    {
+                        ni     = -1;
+                        f.bdir = 1;
+                    }
+                }
+            }
+            if (ni >= 0) io.ld4(((static_cast<long long>(j) * P.nx + ni) * NN + f.idx * N + nk) * 4, f.Uext);
         }
         else
         {
+            // face between slot rows bf-1 and bf of tile column a
+            const int u  = t - XT;
+            const int bf = u / (g.tx * N);
+            const int r  = u - bf * g.tx * N;
+            const int a  = r / N;
+            f.idx        = r - a * N;
+            const int i  = g.i0 + a;
+            const int fj = g.j0 + bf; // face index in [0, ny]
+            if (i >= P.nx) return;
+            const bool inA = bf >= 1 && fj - 1 < P.ny;
+            const bool inB = bf < g.ty && fj < P.ny;
+            if (!inA && !inB) return;
+            f.live = true;
+            f.elA  = inA ? (bf - 1) * g.tx + a : -1;
+            f.elB  = inB ? bf * g.tx + a : -1;
             if (P.curved)
             {
-                const double* fm = P.fym + ((static_cast<long long>(j + (f.face - 2)) * P.nx + i) * N + idx) * 4;
+                const double* fm = P.fym + ((static_cast<long long>(fj) * P.nx + i) * N + f.idx) * 4;
                 f.fnx = fm[0];
                 f.fny = fm[1];
                 f.sc  = fm[2];
@@ -358,29 +412,101 @@ namespace frsdg
                 f.fny = 1.0;
                 f.sc  = P.sc_y;
             }
+            int nj = -1, nl = 0;
+            if (!inA)
+            {
+                nj = fj - 1;
+                nl = N - 1;
+                if (nj < 0)
+                {
+                    if (P.per_y)
+                        nj = P.ny - 1;
+                    else
+                    {
+                        nj     = -1;
+                        f.bdir = 3;
+                    }
+                }
+            }
+            else if (!inB)
+            {
+                nj = fj;
+                nl = 0;
+                if (nj >= P.ny)
+                {
+                    if (P.per_y)
+                        nj = 0;
+                    else
+                    {
+                        nj     = -1;
+                        f.bdir = 2;
+                    }
+                }
+            }
+            if (nj >= 0) io.ld4(((static_cast<long long>(nj) * P.nx + i) * NN + nl * N + f.idx) * 4, f.Uext);
         }
-        if (!f.bnd) io.ld4(((static_cast<long long>(nj) * P.nx + ni) * NN + nl * N + nk) * 4, f.Un);
     }
 
-    // Riemann flux of a face task -> scaled flux jump in sm.Del
-    template <int N, class IO>
-    __device__ __forceinline__ void face_flux(const Res2DP& P, FaceTask<N, typename IO::S>& f,
-                                              TileSmem<N, typename IO::S>& sm)
+    // Riemann flux of a face task -> scaled flux jumps of the tile elements on
+    // either side (Del[el][face][idx]: faces 0 left, 1 right, 2 bottom, 3 top)
+    template <int N, class IO, bool CURVED>
+    __device__ __forceinline__ void face_flux(const Res2DP& P, FaceTask<typename IO::S>& f,
+                                              TileSmem<N, typename IO::S, CURVED>& sm)
     {
         using S = typename IO::S;
-        S Uo[4], Fs[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) Uo[q] = sm.U[f.fe][f.ol][f.ok * 4 + q];
-        if (f.bnd) bc_exterior(P.bc, f.dir, Uo, f.fnx, f.fny, f.Un);
-        if (f.face == 0 || f.face == 2)
-            riemann2(P.gas, P.riemann, f.Un, Uo, f.fnx, f.fny, Fs);
-        else
-            riemann2(P.gas, P.riemann, Uo, f.Un, f.fnx, f.fny, Fs);
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
+        // trace nodes of side A (left / below) and side B (right / above)
+        const int kA = f.xdir ? N - 1 : f.idx, lA = f.xdir ? f.idx : N - 1;
+        const int kB = f.xdir ? 0 : f.idx, lB = f.xdir ? f.idx : 0;
+        S         UA[4], UB[4], Fs[4];
+        if (f.elA >= 0)
         {
-            const S fn = sm.F[f.fe][f.ol][f.ok * 4 + q] * f.fnx + sm.G[f.fe][f.ol][f.ok * 4 + q] * f.fny;
-            sm.Del[f.fe][f.face][f.idx][q] = f.sc * (Fs[q] - fn);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) UA[q] = sm.U[f.elA][lA][kA * 4 + q];
+        }
+        if (f.elB >= 0)
+        {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) UB[q] = sm.U[f.elB][lB][kB * 4 + q];
+        }
+        if (f.elA < 0)
+        {
+            if (f.bdir >= 0)
+                bc_exterior(P.bc, f.bdir, UB, f.fnx, f.fny, UA);
+            else
+            {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) UA[q] = f.Uext[q];
+            }
+        }
+        if (f.elB < 0)
+        {
+            if (f.bdir >= 0)
+                bc_exterior(P.bc, f.bdir, UA, f.fnx, f.fny, UB);
+            else
+            {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) UB[q] = f.Uext[q];
+            }
+        }
+        riemann2(P.gas, P.riemann, UA, UB, f.fnx, f.fny, Fs);
+        const int fA = f.xdir ? 1 : 3, fB = f.xdir ? 0 : 2;
+        if (f.elA >= 0)
+        {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                const S fn = sm.F[f.elA][lA][kA * 4 + q] * f.fnx + sm.G[f.elA][lA][kA * 4 + q] * f.fny;
+                sm.Del[f.elA][fA][f.idx][q] = f.sc * (Fs[q] - fn);
+            }
+        }
+        if (f.elB >= 0)
+        {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                const S fn = sm.F[f.elB][lB][kB * 4 + q] * f.fnx + sm.G[f.elB][lB][kB * 4 + q] * f.fny;
+                sm.Del[f.elB][fB][f.idx][q] = f.sc * (Fs[q] - fn);
+            }
         }
     }
 
@@ -389,12 +515,11 @@ namespace frsdg
     //   eta-line (column k): V[l][k] <- eta_x d/deta F + eta_y d/deta G (l = 0..N-1)
     // The line is held in registers; D comes through the scalar cache.  Affine
     // meshes keep only d/dxi F and d/deta G (the cross terms are exact zeros).
-    template <int N, class S>
-    __device__ __forceinline__ void volume_line(const Res2DP& P, TileSmem<N, S>& sm, bool eta, int le, int ln,
-                                                int q, long long eg)
+    template <int N, class S, bool CURVED>
+    __device__ __forceinline__ void volume_line(const Res2DP& P, TileSmem<N, S, CURVED>& sm, bool eta, int le,
+                                                int ln, int q)
     {
-        constexpr int NN = N * N;
-        const double* D  = P.tab;
+        const double* D = P.tab;
         if (!eta)
         {
             S f[N], g[N];
@@ -402,35 +527,22 @@ namespace frsdg
             for (int m = 0; m < N; ++m)
             {
                 f[m] = sm.F[le][ln][m * 4 + q];
-                if (P.curved) g[m] = sm.G[le][ln][m * 4 + q];
+                if constexpr (CURVED) g[m] = sm.G[le][ln][m * 4 + q];
             }
-            if (P.curved)
+#pragma unroll
+            for (int k = 0; k < N; ++k)
             {
-                const double* mx = P.vmet + eg * 4 * NN + ln * N;
-                const double* my = mx + NN;
+                S dF = Sc<S>::c(0.0), dG = Sc<S>::c(0.0);
 #pragma unroll
-                for (int k = 0; k < N; ++k)
+                for (int m = 0; m < N; ++m)
                 {
-                    S dF = Sc<S>::c(0.0), dG = Sc<S>::c(0.0);
-#pragma unroll
-                    for (int m = 0; m < N; ++m)
-                    {
-                        dF = dF + D[k * N + m] * f[m];
-                        dG = dG + D[k * N + m] * g[m];
-                    }
-                    sm.U[le][ln][k * 4 + q] = mx[k] * dF + my[k] * dG;
+                    dF = dF + D[k * N + m] * f[m];
+                    if constexpr (CURVED) dG = dG + D[k * N + m] * g[m];
                 }
-            }
-            else
-            {
-#pragma unroll
-                for (int k = 0; k < N; ++k)
-                {
-                    S dF = Sc<S>::c(0.0);
-#pragma unroll
-                    for (int m = 0; m < N; ++m) dF = dF + D[k * N + m] * f[m];
+                if constexpr (CURVED)
+                    sm.U[le][ln][k * 4 + q] = sm.M[le][0][ln][k] * dF + sm.M[le][1][ln][k] * dG;
+                else
                     sm.U[le][ln][k * 4 + q] = P.xi_x * dF;
-                }
             }
         }
         else
@@ -440,49 +552,36 @@ namespace frsdg
             for (int m = 0; m < N; ++m)
             {
                 g[m] = sm.G[le][m][ln * 4 + q];
-                if (P.curved) f[m] = sm.F[le][m][ln * 4 + q];
+                if constexpr (CURVED) f[m] = sm.F[le][m][ln * 4 + q];
             }
-            if (P.curved)
+#pragma unroll
+            for (int l = 0; l < N; ++l)
             {
-                const double* ex = P.vmet + eg * 4 * NN + 2 * NN + ln * N;
-                const double* ey = ex + NN;
+                S dF = Sc<S>::c(0.0), dG = Sc<S>::c(0.0);
 #pragma unroll
-                for (int l = 0; l < N; ++l)
+                for (int m = 0; m < N; ++m)
                 {
-                    S dF = Sc<S>::c(0.0), dG = Sc<S>::c(0.0);
-#pragma unroll
-                    for (int m = 0; m < N; ++m)
-                    {
-                        dF = dF + D[l * N + m] * f[m];
-                        dG = dG + D[l * N + m] * g[m];
-                    }
-                    sm.V[le][l][ln * 4 + q] = ex[l] * dF + ey[l] * dG;
+                    dG = dG + D[l * N + m] * g[m];
+                    if constexpr (CURVED) dF = dF + D[l * N + m] * f[m];
                 }
-            }
-            else
-            {
-#pragma unroll
-                for (int l = 0; l < N; ++l)
-                {
-                    S dG = Sc<S>::c(0.0);
-#pragma unroll
-                    for (int m = 0; m < N; ++m) dG = dG + D[l * N + m] * g[m];
+                if constexpr (CURVED)
+                    sm.V[le][l][ln * 4 + q] = sm.M[le][2][ln][l] * dF + sm.M[le][3][ln][l] * dG;
+                else
                     sm.V[le][l][ln * 4 + q] = P.eta_y * dG;
-                }
             }
         }
     }
 
-    // One tile = EPB consecutive elements of one mesh, processed by one
-    // workgroup.  `epb` <= EPB elements are active (the batched sweep may run
-    // smaller tiles).  With `filt` (stage outputs only) the element-local
-    // spectral filter of the post-step hook is applied in LDS before the store.
-    // Phases: (1) node state -> LDS, fluxes F, G -> LDS (neighbour traces of the
-    // face tasks already in flight); (2) face Riemann fluxes -> Del; (3) volume
+    // One tile = a tx x ty block of elements of one mesh, processed by one
+    // workgroup (tx * ty <= EPB; the batched sweep may run smaller tiles).
+    // With `filt` (stage outputs only) the element-local spectral filter of
+    // the post-step hook is applied in LDS before the store.
+    // Phases: (1) node state -> LDS, fluxes F, G -> LDS (outer traces of the
+    // rim faces already in flight); (2) face Riemann fluxes -> Del; (3) volume
     // term by xi- and eta-lines; (4) per node: volume + VCJH lift -> output.
-    template <int N, class IO>
-    __device__ __forceinline__ void res2d_tile(const Res2DP& P, const IO& io, long long blk, int epb, bool filt,
-                                               TileSmem<N, typename IO::S>& sm)
+    template <int N, class IO, bool CURVED>
+    __device__ __forceinline__ void res2d_tile(const Res2DP& P, const IO& io, long long blk, int tx, int ty, bool filt,
+                                               TileSmem<N, typename IO::S, CURVED>& sm)
     {
         using S           = typename IO::S;
         constexpr int BS  = Tile2D<N, S>::BS;
@@ -495,21 +594,33 @@ namespace frsdg
             sm.dL[tid] = P.tab[NN + tid];
             sm.dR[tid] = P.tab[NN + N + tid];
         }
-        const int       el   = tid / NN;
-        const int       node = tid - el * NN;
-        const int       k    = node % N;
-        const int       l    = node / N;
-        const long long e    = blk * epb + el;
-        const bool      act  = (el < epb) && (e < P.E);
+        const TileGeom g    = tile_geom(P, blk, tx, ty);
+        const int      el   = tid / NN;
+        const int      node = tid - el * NN;
+        const int      k    = node % N;
+        const int      l    = node / N;
+        const int      eb   = el / tx;
+        const int      ea   = el - eb * tx;
+        const bool     act  = (el < tx * ty) && (g.i0 + ea < P.nx) && (g.j0 + eb < P.ny);
+        const long long e   = static_cast<long long>(g.j0 + eb) * P.nx + (g.i0 + ea);
+        if (node == 0 && el < tx * ty) sm.eid[el] = act ? e : -1;
 
-        const TileOrigin org = tile_origin(P, blk, epb);
-        FaceTask<N, S>   f0;
-        face_task<N, IO>(P, io, blk, epb, org, tid, f0);
+        FaceTask<S> f0;
+        face_task<N, IO>(P, io, g, tid, f0);
 
         S own[4];
         if (act)
         {
             io.ld4((e * NN + node) * 4, own);
+            if constexpr (CURVED)
+            {
+                double mt[4];
+                ld4d(P.vmet + (e * NN + node) * 4, mt);
+                sm.M[el][0][l][k] = mt[0];
+                sm.M[el][1][l][k] = mt[1];
+                sm.M[el][2][k][l] = mt[2];
+                sm.M[el][3][k][l] = mt[3];
+            }
             S F[4], G[4];
             flux2(P.gas, own, F, G);
 #pragma unroll
@@ -522,13 +633,14 @@ namespace frsdg
         }
         __syncthreads();
 
-        // ---- face tasks: 4 faces x N nodes per element ----
-        if (f0.live) face_flux<N, IO>(P, f0, sm);
-        for (int t = tid + BS; t < epb * 4 * N; t += BS)
+        // ---- face tasks: every face of the tile once ----
+        if (f0.live) face_flux<N, IO, CURVED>(P, f0, sm);
+        const int nft = ((tx + 1) * ty + tx * (ty + 1)) * N;
+        for (int t = tid + BS; t < nft; t += BS)
         {
-            FaceTask<N, S> f;
-            face_task<N, IO>(P, io, blk, epb, org, t, f);
-            if (f.live) face_flux<N, IO>(P, f, sm);
+            FaceTask<S> f;
+            face_task<N, IO>(P, io, g, t, f);
+            if (f.live) face_flux<N, IO, CURVED>(P, f, sm);
         }
         __syncthreads();
 
@@ -536,23 +648,22 @@ namespace frsdg
         // eta-line tasks (4 contractions per node); affine: one task does
         // xi-line ln of F and eta-line ln of G (2 contractions per node) ----
         {
-            const int nlt = epb * N * 4;
-            const int nt  = P.curved ? 2 * nlt : nlt;
+            const int nlt = tx * ty * N * 4;
+            const int nt  = CURVED ? 2 * nlt : nlt;
             for (int t = tid; t < nt; t += BS)
             {
-                const bool      eta = t >= nlt;
-                const int       w   = eta ? t - nlt : t;
-                const int       q   = w & 3;
-                const int       le  = (w >> 2) / N;
-                const int       ln  = (w >> 2) - le * N;
-                const long long eg  = blk * epb + le;
-                if (eg >= P.E) continue;
-                if (P.curved)
-                    volume_line<N, S>(P, sm, eta, le, ln, q, eg);
+                const bool eta = t >= nlt;
+                const int  w   = eta ? t - nlt : t;
+                const int  q   = w & 3;
+                const int  le  = (w >> 2) / N;
+                const int  ln  = (w >> 2) - le * N;
+                if (sm.eid[le] < 0) continue;
+                if constexpr (CURVED)
+                    volume_line<N, S, CURVED>(P, sm, eta, le, ln, q);
                 else
                 {
-                    volume_line<N, S>(P, sm, false, le, ln, q, eg);
-                    volume_line<N, S>(P, sm, true, le, ln, q, eg);
+                    volume_line<N, S, CURVED>(P, sm, false, le, ln, q);
+                    volume_line<N, S, CURVED>(P, sm, true, le, ln, q);
                 }
             }
         }
@@ -611,18 +722,20 @@ namespace frsdg
     // ------------------------------------------------------------------
     // 2D residual: physics_impl_2D.hpp:267-357 (volume) + :689-899 (faces)
     // ------------------------------------------------------------------
-    template <int N, class IO>
+    template <int N, class IO, bool CURVED>
     __global__ __launch_bounds__(BlockSize<typename IO::S>::value) void k_res2d(const Res2DP P, const IO io)
     {
-        __shared__ TileSmem<N, typename IO::S> sm;
-        res2d_tile<N, IO>(P, io, xcd_tile(blockIdx.x, gridDim.x), Tile2D<N, typename IO::S>::EPB, false, sm);
+        using T = Tile2D<N, typename IO::S>;
+        __shared__ TileSmem<N, typename IO::S, CURVED> sm;
+        res2d_tile<N, IO, CURVED>(P, io, xcd_tile(blockIdx.x, gridDim.x), T::TX, T::TY, false, sm);
     }
     // last SSP-RK stage fused with the post-step spectral filter
-    template <int N>
+    template <int N, bool CURVED>
     __global__ __launch_bounds__(256) void k_res2d_stage_filter(const Res2DP P, const IOStage io)
     {
-        __shared__ TileSmem<N, double> sm;
-        res2d_tile<N, IOStage>(P, io, xcd_tile(blockIdx.x, gridDim.x), Tile2D<N, double>::EPB, true, sm);
+        using T = Tile2D<N, double>;
+        __shared__ TileSmem<N, double, CURVED> sm;
+        res2d_tile<N, IOStage, CURVED>(P, io, xcd_tile(blockIdx.x, gridDim.x), T::TX, T::TY, true, sm);
     }
 
     // ------------------------------------------------------------------
