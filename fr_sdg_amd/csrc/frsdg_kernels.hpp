@@ -271,8 +271,8 @@ namespace frsdg
         static constexpr int MP = N + ((N % 2 == 0) ? 1 : 0);
         S         U[EPB][N][RP]; // state; after the face phase: xi-line part of the volume term
         S         F[EPB][N][RP];
-        S         G[EPB][N][RP];
-        S         V[EPB][N][RP]; // eta-line part of the volume term
+        S         G[EPB][N][RP]; // affine meshes: eta-line part of the volume term after the line phase
+        S         V[CURVED ? EPB : 1][N][CURVED ? RP : 1]; // curved meshes: eta-line part
         S         Del[EPB][4][N][4];
         double    M[CURVED ? EPB : 1][4][N][CURVED ? MP : 1];
         double    dL[N], dR[N], Fm[NN];
@@ -513,6 +513,7 @@ namespace frsdg
     // Volume term along one line of one element for equation q.
     //   xi-line  (row l):    U[l][k] <- xi_x d/dxi F + xi_y d/dxi G   (k = 0..N-1)
     //   eta-line (column k): V[l][k] <- eta_x d/deta F + eta_y d/deta G (l = 0..N-1)
+    //                        (affine: G[l][k] <- eta_y d/deta G)
     // The line is held in registers; D comes through the scalar cache.  Affine
     // meshes keep only d/dxi F and d/deta G (the cross terms are exact zeros).
     template <int N, class S, bool CURVED>
@@ -564,10 +565,12 @@ namespace frsdg
                     dG = dG + D[l * N + m] * g[m];
                     if constexpr (CURVED) dF = dF + D[l * N + m] * f[m];
                 }
+                // affine: the result replaces column ln of G (equation q), which
+                // only this task reads (the xi-lines read F only)
                 if constexpr (CURVED)
                     sm.V[le][l][ln * 4 + q] = sm.M[le][2][ln][l] * dF + sm.M[le][3][ln][l] * dG;
                 else
-                    sm.V[le][l][ln * 4 + q] = P.eta_y * dG;
+                    sm.G[le][l][ln * 4 + q] = P.eta_y * dG;
             }
         }
     }
@@ -675,7 +678,10 @@ namespace frsdg
 #pragma unroll
             for (int q = 0; q < 4; ++q)
             {
-                r[q] = sm.U[el][l][k * 4 + q] + sm.V[el][l][k * 4 + q];
+                if constexpr (CURVED)
+                    r[q] = sm.U[el][l][k * 4 + q] + sm.V[el][l][k * 4 + q];
+                else
+                    r[q] = sm.U[el][l][k * 4 + q] + sm.G[el][l][k * 4 + q];
                 r[q] = r[q] + sm.Del[el][0][l][q] * sm.dL[k];
                 r[q] = r[q] + sm.Del[el][1][l][q] * sm.dR[k];
                 r[q] = r[q] + sm.Del[el][2][k][q] * sm.dL[l];
