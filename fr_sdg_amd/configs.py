@@ -106,7 +106,7 @@ EXPLOSION_SMOOTH_PRM = EXPLOSION_PRM + [0.1]
 
 def explicit_cfl(p):
     """CFL (on the element size) used for explicit runs: 0.4 / (2p + 1)
-    (0.05 at p = 4, where 0.1 is unstable)."""
+    (0.044 at p = 4, where 0.1 is unstable)."""
     return 0.4 / (2 * p + 1)
 
 
