@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--adj-snapshots", type=int, default=32)
     ap.add_argument("--adj-n", type=int, default=1024)
     ap.add_argument("--kernel-only", choices=["stage", "residual", "tangent", "cstep", "adjoint", "large",
-                                                "copy"], default=None,
+                                                "copy", "batch", "batch_each"], default=None,
                     help="profiling mode: launch only this kernel `--steps` times, print nothing else")
     return ap.parse_args()
 
@@ -201,6 +201,15 @@ def main():
 def kernel_only(torch, fr, args):
     """Launch one kernel kind back to back (for rocprofv3 counter / trace runs)."""
     from fr_sdg_amd import configs as CF
+    if args.kernel_only in ("batch", "batch_each"):
+        b, u = batch_setup(fr)
+        b.set_tuning(0, 1, 1)  # one workgroup per tile, one launch per order
+        step = b.ssprk_step if args.kernel_only == "batch" else b.ssprk_step_each
+        for _ in range(args.steps):
+            step(u)
+        torch.cuda.synchronize()
+        print(json.dumps({"kernel_only": args.kernel_only, "steps": args.steps, "ndofs": b.ndofs}))
+        return
     if args.kernel_only == "large":
         cfg = dict(dims=2, nx=1024, ny=1024, p=7, lx=2.0, ly=2.0, riemann=1, geometry=fr.GEOM_WARPED,
                    geom_param=[0.05, -1.0, -1.0, 0.0], bc_type=[1, 1, 1, 1], gas_kind=1, vdw_a=0.1,
@@ -306,11 +315,8 @@ def bench_newton(torch, fr):
     return out
 
 
-def bench_batch(torch, fr, steps=20):
-    """64 radial shock tubes (N in {32..192}, p in {2..6}, seeded), SSP-RK3 at
-    CFL 0.4/(2p+1) per instance: 3 persistent batched launches per step (graph
-    replay, autotuned launch shape) vs one set of launches per instance."""
-    from fr_sdg_amd import autotune
+def batch_setup(fr):
+    """The 64-instance radial shock-tube batch with its initial state and per-instance dt."""
     from fr_sdg_amd import configs as CF
     cfgs = CF.batch64()
     b = fr.Batch(cfgs)
@@ -321,6 +327,15 @@ def bench_batch(torch, fr, steps=20):
         h = cfgs[k]["lx"] / cfgs[k]["nx"]
         dts.append(CF.explicit_cfl(cfgs[k]["p"]) * h / CF.sound_speed_max(uk, cfgs[k]))
     b.set_dt(dts)
+    return b, u
+
+
+def bench_batch(torch, fr, steps=20):
+    """64 radial shock tubes (N in {32..192}, p in {2..6}, seeded), SSP-RK3 at
+    CFL 0.4/(2p+1) per instance: 3 persistent batched launches per step (graph
+    replay, autotuned launch shape) vs one set of launches per instance."""
+    from fr_sdg_amd import autotune
+    b, u = batch_setup(fr)
     best, table = autotune.tune_batch(b, u, use_cache=False)
     s = torch.cuda.current_stream()
 

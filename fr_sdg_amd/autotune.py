@@ -4,8 +4,10 @@ Knobs (frsdg_batch_set_tuning): the number of persistent workgroups of the
 stage kernels (or one workgroup per element tile), the element tile shape
 (the default tile of each order, or that tile with its rows halved: more,
 shorter work items, better balance across uneven instances), and the launch
-mode (one kernel per stage over all orders, or one specialised kernel per
-order and mesh kind).  Each candidate is timed over
+mode (one kernel per stage over all orders; one specialised kernel per order
+and mesh kind; or the same cut into cache-sized runs of instances that take
+their three stages back to back, so stages 2 and 3 find their inputs in the
+last-level cache).  Each candidate is timed over
 a few graph-free batched steps on a scratch copy of the state; the winner is
 cached in a JSON file keyed by the batch signature (per-instance mesh size,
 order, gas model) and the device name.
@@ -47,7 +49,7 @@ def _store(db):
 def candidates():
     """(grid, epb_div, mode); grid 0 = one workgroup per element tile."""
     ncu = torch.cuda.get_device_properties(0).multi_processor_count
-    return [(g * ncu, d, m) for m in (1, 0) for g in (0, 1, 2, 4, 8) for d in (1, 2)]
+    return [(g * ncu, d, m) for m in (1, 2, 0) for g in (0, 1, 2, 4, 8) for d in (1, 2)]
 
 
 def tune_batch(batch, u, steps=3, use_cache=True):

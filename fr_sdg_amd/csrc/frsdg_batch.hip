@@ -58,32 +58,84 @@ namespace
         int           last;
     };
 
-    __global__ __launch_bounds__(256) void k_batch_stage(const BatchInst* __restrict__ tab, int ninst,
-                                                         long long total, StageArgs A)
+    // Work list of one launch: tile ranges of its instances.  Up to KMAX
+    // entries travel by value in the kernel arguments (the per-tile instance
+    // search then reads the constant cache instead of issuing dependent HBM
+    // loads); longer lists are searched in device memory.
+    constexpr int KMAX = 64;
+    // mode 2 chunk size: three stage states of the chunk's instances (about
+    // half of the 256 MB last-level cache)
+    constexpr long long CHUNK_BYTES = 128LL << 20;
+    struct WorkList
     {
-        __shared__ BatchSmem sm;
-        const long long slot = xcd_tile(blockIdx.x, gridDim.x);
-        for (long long t = slot; t < total; t += gridDim.x)
+        int                    ng;
+        long long              total;
+        const int*       __restrict__ d_idx; // ng entries (device), used when ng > KMAX
+        const long long* __restrict__ d_t0;
+        long long              t0[KMAX];
+        int                    idx[KMAX];
+    };
+    // instance slot k and local tile of global tile t (uniform across the
+    // workgroup; the results are pinned to scalar registers so that the
+    // instance record is read with scalar loads)
+    __device__ __forceinline__ int work_find_any(const WorkList& W, long long t, long long& local);
+    __device__ __forceinline__ int work_find(const WorkList& W, long long t, long long& local)
+    {
+        long long  l;
+        const int  k  = work_find_any(W, t, l);
+        const auto lo = static_cast<unsigned>(static_cast<unsigned long long>(l));
+        const auto hi = static_cast<unsigned>(static_cast<unsigned long long>(l) >> 32);
+        local         = static_cast<long long>((static_cast<unsigned long long>(__builtin_amdgcn_readfirstlane(hi)) << 32) |
+                                               __builtin_amdgcn_readfirstlane(lo));
+        return __builtin_amdgcn_readfirstlane(k);
+    }
+    __device__ __forceinline__ int work_find_any(const WorkList& W, long long t, long long& local)
+    {
+        int lo = 0, hi = W.ng - 1;
+        if (W.ng <= KMAX)
         {
-            int lo = 0, hi = ninst - 1;
             while (lo < hi)
             {
                 const int mid = (lo + hi + 1) >> 1;
-                if (tab[mid].tile0 <= t)
+                if (W.t0[mid] <= t)
                     lo = mid;
                 else
                     hi = mid - 1;
             }
-            const BatchInst& I   = tab[lo];
-            const IOStage    io  = {A.us + I.off, A.u0 + I.off, A.out + I.off, A.a0, A.b, A.cfac * A.dt[lo]};
-            const long long  blk = t - I.tile0;
+            local = t - W.t0[lo];
+            return W.idx[lo];
+        }
+        while (lo < hi)
+        {
+            const int mid = (lo + hi + 1) >> 1;
+            if (W.d_t0[mid] <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        local = t - W.d_t0[lo];
+        return W.d_idx[lo];
+    }
+
+    template <bool FILT>
+    __global__ __launch_bounds__(256) void k_batch_stage(const BatchInst* __restrict__ tab, const WorkList W,
+                                                         StageArgs A)
+    {
+        __shared__ BatchSmem sm;
+        const long long slot = xcd_tile(blockIdx.x, gridDim.x);
+        for (long long t = slot; t < W.total; t += gridDim.x)
+        {
+            long long       blk;
+            const int       k   = work_find(W, t, blk);
+            const BatchInst I   = tab[k];
+            const IOStage    io  = {A.us + I.off, A.u0 + I.off, A.out + I.off, A.a0, A.b, A.cfac * A.dt[k]};
             const bool       flt = A.last && I.filt;
 #define FR_BT(NN)                                                                                            \
     case NN:                                                                                                 \
         if (I.curved)                                                                                        \
-            res2d_tile<NN, IOStage, true>(I.P, io, blk, I.tx, I.ty, flt, sm.c##NN);                          \
+            res2d_tile<NN, IOStage, true, FILT>(I.P, io, blk, I.tx, I.ty, flt, sm.c##NN);                          \
         else                                                                                                 \
-            res2d_tile<NN, IOStage, false>(I.P, io, blk, I.tx, I.ty, flt, sm.a##NN);                         \
+            res2d_tile<NN, IOStage, false, FILT>(I.P, io, blk, I.tx, I.ty, flt, sm.a##NN);                         \
         break;
             switch (I.n)
             {
@@ -96,29 +148,37 @@ namespace
 
     // one launch per (order, curvature): the tiles of all instances of the
     // group, fully specialised (registers and LDS sized for this order only)
-    template <int N, bool CURVED>
-    __global__ __launch_bounds__(256) void k_batch_stage_n(const BatchInst* __restrict__ tab,
-                                                           const int* __restrict__ gidx,
-                                                           const long long* __restrict__ gtile0, int ng,
-                                                           long long total, StageArgs A)
+    // the same with one workgroup per tile (grid = tiles of the group) and the
+    // tile shape fixed at compile time (DIV = 1 default tile, 2 halved): no
+    // loop and no run-time shape arithmetic, so the register budget is the
+    // single-problem kernel's
+    template <int N, bool CURVED, bool FILT, int DIV>
+    __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_batch_tile_n(const BatchInst* __restrict__ tab, const WorkList W,
+                                                          StageArgs A)
+    {
+        using T       = Tile2D<N, double>;
+        constexpr int TX = tile_halve(T::TX, T::TY, DIV, true), TY = tile_halve(T::TX, T::TY, DIV, false);
+        __shared__ TileSmem<N, double, CURVED> sm;
+        long long       blk;
+        const int       k  = work_find(W, xcd_tile(blockIdx.x, gridDim.x), blk);
+        const BatchInst I  = tab[k];
+        const IOStage   io = {A.us + I.off, A.u0 + I.off, A.out + I.off, A.a0, A.b, A.cfac * A.dt[k]};
+        res2d_tile<N, IOStage, CURVED, FILT>(I.P, io, blk, TX, TY, I.filt, sm);
+    }
+
+    template <int N, bool CURVED, bool FILT>
+    __global__ __launch_bounds__(256) void k_batch_stage_n(const BatchInst* __restrict__ tab, const WorkList W,
+                                                           StageArgs A)
     {
         __shared__ TileSmem<N, double, CURVED> sm;
         const long long slot = xcd_tile(blockIdx.x, gridDim.x);
-        for (long long t = slot; t < total; t += gridDim.x)
+        for (long long t = slot; t < W.total; t += gridDim.x)
         {
-            int lo = 0, hi = ng - 1;
-            while (lo < hi)
-            {
-                const int mid = (lo + hi + 1) >> 1;
-                if (gtile0[mid] <= t)
-                    lo = mid;
-                else
-                    hi = mid - 1;
-            }
-            const int        k  = gidx[lo];
-            const BatchInst& I  = tab[k];
+            long long       blk;
+            const int       k  = work_find(W, t, blk);
+            const BatchInst I  = tab[k];
             const IOStage    io = {A.us + I.off, A.u0 + I.off, A.out + I.off, A.a0, A.b, A.cfac * A.dt[k]};
-            res2d_tile<N, IOStage, CURVED>(I.P, io, t - gtile0[lo], I.tx, I.ty, A.last && I.filt, sm);
+            res2d_tile<N, IOStage, CURVED, FILT>(I.P, io, blk, I.tx, I.ty, I.filt, sm);
             __syncthreads();
         }
     }
@@ -142,21 +202,38 @@ struct frsdg_batch
     double*                 g_u   = nullptr;
     int                     g_ok  = 0;
     int                     mode  = 1; // 0 one launch for all orders, 1 one launch per order
-    // per-order groups
+    // launch groups: all instances (mode 0), or one per (order, mesh kind) (mode 1)
     struct Group
     {
-        int        n;
-        int        curved;
-        int        ng;
-        long long  total;
+        int        n      = 0;
+        int        curved = 0;
+        WorkList   W{};
         int*       d_idx = nullptr;
         long long* d_t0  = nullptr;
     };
+    Group              all;
     std::vector<Group> groups;
+    std::vector<Group> chunks; // mode 2: groups cut into cache-sized runs of instances
 };
 
 namespace
 {
+    void free_groups(frsdg_batch* b)
+    {
+        auto rel = [](frsdg_batch::Group& g)
+        {
+            if (g.d_idx) (void) hipFree(g.d_idx);
+            if (g.d_t0) (void) hipFree(g.d_t0);
+            g.d_idx = nullptr;
+            g.d_t0  = nullptr;
+        };
+        rel(b->all);
+        for (auto& g : b->groups) rel(g);
+        for (auto& g : b->chunks) rel(g);
+        b->groups.clear();
+        b->chunks.clear();
+    }
+
     int rebuild_tiles(frsdg_batch* b)
     {
         long long t = 0;
@@ -169,36 +246,84 @@ namespace
         }
         b->total_tiles = t;
         FR_TRY(hipMemcpy(b->d_tab, b->inst.data(), sizeof(BatchInst) * b->inst.size(), hipMemcpyHostToDevice));
-        for (auto& g : b->groups)
+        free_groups(b);
+        // fills a group's work list from the instances selected by `pick`
+        auto make = [&](frsdg_batch::Group& g, auto pick) -> int
         {
-            if (g.d_idx) (void) hipFree(g.d_idx);
-            if (g.d_t0) (void) hipFree(g.d_t0);
-        }
-        b->groups.clear();
-        for (int key = 0; key < 16; ++key)
-        {
-            const int              n = 2 + (key >> 1), cv = key & 1;
             std::vector<int>       idx;
             std::vector<long long> t0;
             long long              tot = 0;
             for (size_t k = 0; k < b->inst.size(); ++k)
-                if (b->inst[k].n == n && b->inst[k].curved == cv)
+                if (pick(b->inst[k]))
                 {
                     idx.push_back(static_cast<int>(k));
                     t0.push_back(tot);
                     tot += b->inst[k].ntiles;
                 }
-            if (idx.empty()) continue;
-            frsdg_batch::Group g;
-            g.n      = n;
-            g.curved = cv;
-            g.ng     = static_cast<int>(idx.size());
-            g.total = tot;
+            g.W.ng    = static_cast<int>(idx.size());
+            g.W.total = tot;
+            for (size_t q = 0; q < idx.size() && q < static_cast<size_t>(KMAX); ++q)
+            {
+                g.W.t0[q]  = t0[q];
+                g.W.idx[q] = idx[q];
+            }
+            if (idx.empty()) return FRSDG_OK;
             FR_TRY(hipMalloc(&g.d_idx, sizeof(int) * idx.size()));
             FR_TRY(hipMalloc(&g.d_t0, sizeof(long long) * t0.size()));
             FR_TRY(hipMemcpy(g.d_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
             FR_TRY(hipMemcpy(g.d_t0, t0.data(), sizeof(long long) * t0.size(), hipMemcpyHostToDevice));
-            b->groups.push_back(g);
+            g.W.d_idx = g.d_idx;
+            g.W.d_t0  = g.d_t0;
+            return FRSDG_OK;
+        };
+        {
+            const int rc = make(b->all, [](const BatchInst&) { return true; });
+            if (rc) return rc;
+        }
+        for (int key = 0; key < 16; ++key)
+        {
+            const int          n = 2 + (key >> 1), cv = key & 1;
+            frsdg_batch::Group g;
+            g.n      = n;
+            g.curved = cv;
+            const int rc = make(g, [&](const BatchInst& I) { return I.n == n && I.curved == cv; });
+            if (g.W.ng > 0) b->groups.push_back(g);
+            if (rc) return rc;
+            // cache-sized chunks of this group: consecutive instances whose three
+            // stage states (u, w1, w2) together stay within CHUNK_BYTES
+            size_t k0 = 0;
+            while (true)
+            {
+                while (k0 < b->inst.size() && !(b->inst[k0].n == n && b->inst[k0].curved == cv)) ++k0;
+                if (k0 >= b->inst.size()) break;
+                size_t    k1    = k0;
+                long long bytes = 0;
+                while (k1 < b->inst.size())
+                {
+                    const BatchInst& I = b->inst[k1];
+                    if (!(I.n == n && I.curved == cv))
+                    {
+                        ++k1;
+                        continue;
+                    }
+                    const long long ib = 24LL * (b->off[k1 + 1] - b->off[k1]);
+                    if (bytes > 0 && bytes + ib > CHUNK_BYTES) break;
+                    bytes += ib;
+                    ++k1;
+                }
+                frsdg_batch::Group c;
+                c.n      = n;
+                c.curved = cv;
+                const size_t lo = k0, hi = k1;
+                const int    rc2 = make(c, [&](const BatchInst& I)
+                                        {
+                                            const size_t k = static_cast<size_t>(&I - b->inst.data());
+                                            return k >= lo && k < hi && I.n == n && I.curved == cv;
+                                        });
+                if (c.W.ng > 0) b->chunks.push_back(c);
+                if (rc2) return rc2;
+                k0 = k1;
+            }
         }
         if (b->gexec) (void) hipGraphExecDestroy(b->gexec);
         if (b->graph) (void) hipGraphDestroy(b->graph);
@@ -209,40 +334,80 @@ namespace
 
     int batch_step(frsdg_batch* b, double* u, hipStream_t s)
     {
-        const int ninst = static_cast<int>(b->inst.size());
-        auto launch     = [&](const double* us, double a0, double bb, double* out, int last)
+        // one stage over the instances of group g
+        auto launch_group = [&](const frsdg_batch::Group& g, const StageArgs& A, int last)
         {
-            StageArgs A{us, u, out, b->d_dt, a0, bb, 1.0, last};
+            const WorkList& W  = g.W;
+            const unsigned  gr = static_cast<unsigned>(b->grid > 0 ? std::min<long long>(b->grid, W.total) : W.total);
+#define FR_BK(K, ...) hipLaunchKernelGGL((K<__VA_ARGS__>), dim3(gr), dim3(256), 0, s, b->d_tab, W, A)
+#define FR_BT(NN, C)                                                                                          \
+    {                                                                                                         \
+        if (b->grid > 0)                                                                                      \
+        {                                                                                                     \
+            if (last) FR_BK(k_batch_stage_n, NN, C, true); else FR_BK(k_batch_stage_n, NN, C, false);         \
+        }                                                                                                     \
+        else if (b->epb_div == 1)                                                                             \
+        {                                                                                                     \
+            if (last) FR_BK(k_batch_tile_n, NN, C, true, 1); else FR_BK(k_batch_tile_n, NN, C, false, 1);     \
+        }                                                                                                     \
+        else                                                                                                  \
+        {                                                                                                     \
+            if (last) FR_BK(k_batch_tile_n, NN, C, true, 2); else FR_BK(k_batch_tile_n, NN, C, false, 2);     \
+        }                                                                                                     \
+    }
+#define FR_BN(NN)                                                                                             \
+    case NN:                                                                                                  \
+        if (g.curved)                                                                                         \
+            FR_BT(NN, true)                                                                                   \
+        else                                                                                                  \
+            FR_BT(NN, false)                                                                                  \
+        break;
+            switch (g.n)
+            {
+                FR_BN(2) FR_BN(3) FR_BN(4) FR_BN(5) FR_BN(6) FR_BN(7) FR_BN(8) FR_BN(9)
+            }
+#undef FR_BN
+#undef FR_BT
+#undef FR_BK
+            return hip_rc(hipGetLastError());
+        };
+        const StageArgs S1{u, u, b->w1, b->d_dt, 0.0, 1.0, 1.0, 0};
+        const StageArgs S2{b->w1, u, b->w2, b->d_dt, 0.75, 0.25, 1.0, 0};
+        const StageArgs S3{b->w2, u, u, b->d_dt, 1.0 / 3.0, 2.0 / 3.0, 1.0, 1};
+        int             rc;
+        if (b->mode == 2)
+        {
+            // each cache-sized chunk takes its three stages back to back
+            for (const auto& c : b->chunks)
+            {
+                if ((rc = launch_group(c, S1, 0))) return rc;
+                if ((rc = launch_group(c, S2, 0))) return rc;
+                if ((rc = launch_group(c, S3, 1))) return rc;
+            }
+            return FRSDG_OK;
+        }
+        auto launch = [&](const StageArgs& A, int last)
+        {
             if (b->mode == 0)
             {
-                const unsigned gr = static_cast<unsigned>(b->grid > 0 ? b->grid : b->total_tiles);
-                hipLaunchKernelGGL(k_batch_stage, dim3(gr), dim3(256), 0, s, b->d_tab, ninst, b->total_tiles, A);
+                const WorkList& W  = b->all.W;
+                const unsigned  gr = static_cast<unsigned>(b->grid > 0 ? std::min<long long>(b->grid, W.total) : W.total);
+                if (last)
+                    hipLaunchKernelGGL(k_batch_stage<true>, dim3(gr), dim3(256), 0, s, b->d_tab, W, A);
+                else
+                    hipLaunchKernelGGL(k_batch_stage<false>, dim3(gr), dim3(256), 0, s, b->d_tab, W, A);
                 return hip_rc(hipGetLastError());
             }
             for (const auto& g : b->groups)
             {
-                const unsigned gr = static_cast<unsigned>(b->grid > 0 ? std::min<long long>(b->grid, g.total) : g.total);
-#define FR_BN(NN)                                                                                              \
-    case NN:                                                                                                   \
-        if (g.curved)                                                                                          \
-            hipLaunchKernelGGL((k_batch_stage_n<NN, true>), dim3(gr), dim3(256), 0, s, b->d_tab, g.d_idx, g.d_t0, \
-                               g.ng, g.total, A);                                                             \
-        else                                                                                                   \
-            hipLaunchKernelGGL((k_batch_stage_n<NN, false>), dim3(gr), dim3(256), 0, s, b->d_tab, g.d_idx,       \
-                               g.d_t0, g.ng, g.total, A);                                                     \
-        break;
-                switch (g.n)
-                {
-                    FR_BN(2) FR_BN(3) FR_BN(4) FR_BN(5) FR_BN(6) FR_BN(7) FR_BN(8) FR_BN(9)
-                }
-#undef FR_BN
+                const int r = launch_group(g, A, last);
+                if (r) return r;
             }
-            return hip_rc(hipGetLastError());
+            return FRSDG_OK;
         };
-        int rc;
-        if ((rc = launch(u, 0.0, 1.0, b->w1, 0))) return rc;
-        if ((rc = launch(b->w1, 0.75, 0.25, b->w2, 0))) return rc;
-        return launch(b->w2, 1.0 / 3.0, 2.0 / 3.0, u, 1);
+        if ((rc = launch(S1, 0))) return rc;
+        if ((rc = launch(S2, 0))) return rc;
+        return launch(S3, 1);
     }
 } // namespace
 
@@ -295,11 +460,7 @@ int frsdg_batch_destroy(frsdg_batch* b)
     if (!b) return FRSDG_OK;
     if (b->gexec) (void) hipGraphExecDestroy(b->gexec);
     if (b->graph) (void) hipGraphDestroy(b->graph);
-    for (auto& g : b->groups)
-    {
-        if (g.d_idx) (void) hipFree(g.d_idx);
-        if (g.d_t0) (void) hipFree(g.d_t0);
-    }
+    free_groups(b);
     for (auto* c : b->ctx) frsdg_destroy(c);
     for (void* p : {static_cast<void*>(b->d_tab), static_cast<void*>(b->d_dt), static_cast<void*>(b->w1),
                     static_cast<void*>(b->w2)})
@@ -326,7 +487,7 @@ int frsdg_batch_instance(frsdg_batch* b, int k, frsdg_ctx** out)
 
 int frsdg_batch_set_tuning(frsdg_batch* b, int grid, int epb_div, int mode)
 {
-    if (!b || grid < 0 || epb_div < 1 || epb_div > 64 || mode < 0 || mode > 1) return FRSDG_ERR_INVALID;
+    if (!b || grid < 0 || (epb_div != 1 && epb_div != 2) || mode < 0 || mode > 2) return FRSDG_ERR_INVALID;
     b->grid    = grid;
     b->epb_div = epb_div;
     b->mode    = mode;

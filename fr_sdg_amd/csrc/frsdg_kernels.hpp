@@ -233,13 +233,10 @@ namespace frsdg
         static constexpr int TY  = tile_rows(EPB);
         static constexpr int TX  = EPB / TY;
     };
-    // element tile shape (TX, TY) of order n in the double kernels, reduced by `div`
-    // (rows first); host helper of the launch code
-    inline void tile_shape(int n, int div, int& tx, int& ty)
+    // element tile shape of order n in the double kernels: the default TX x TY
+    // tile with its rows (then columns) halved log2(div) times
+    __host__ __device__ constexpr int tile_halve(int tx, int ty, int div, bool want_tx)
     {
-        const int epb = (256 / (n * n)) > 0 ? 256 / (n * n) : 1;
-        ty            = tile_rows(epb);
-        tx            = epb / ty;
         for (int d = div; d > 1; d /= 2)
         {
             if (ty > 1)
@@ -247,6 +244,14 @@ namespace frsdg
             else if (tx > 1)
                 tx = (tx + 1) / 2;
         }
+        return want_tx ? tx : ty;
+    }
+    inline void tile_shape(int n, int div, int& tx, int& ty)
+    {
+        const int epb = (256 / (n * n)) > 0 ? 256 / (n * n) : 1;
+        const int ty0 = tile_rows(epb), tx0 = epb / ty0;
+        tx            = tile_halve(tx0, ty0, div, true);
+        ty            = tile_halve(tx0, ty0, div, false);
     }
     inline long long tile_count(int nx, int ny, int tx, int ty)
     {
@@ -275,7 +280,7 @@ namespace frsdg
         S         V[CURVED ? EPB : 1][N][CURVED ? RP : 1]; // curved meshes: eta-line part
         S         Del[EPB][4][N][4];
         double    M[CURVED ? EPB : 1][4][N][CURVED ? MP : 1];
-        double    dL[N], dR[N], Fm[NN];
+        double    dL[N], dR[N];
         long long eid[EPB];      // global element of each tile slot, -1 if outside the mesh
     };
 
@@ -575,14 +580,39 @@ namespace frsdg
         }
     }
 
+    // Spectral filter of the stage value along one line (equation q), the
+    // operator from the scalar cache: xi-line (row l) -> F[l][k] = sum_c
+    // Fm[k][c] U[l][c]; eta-line (column k) -> G[l][k] = sum_c Fm[l][c] U[c][k].
+    template <int N, bool CURVED>
+    __device__ __forceinline__ void filter_line(const Res2DP& P, TileSmem<N, double, CURVED>& sm, bool eta, int le,
+                                                int ln, int q)
+    {
+        const double* Fm = P.tab + N * N + 2 * N;
+        double        x[N];
+#pragma unroll
+        for (int c = 0; c < N; ++c) x[c] = eta ? sm.U[le][c][ln * 4 + q] : sm.U[le][ln][c * 4 + q];
+#pragma unroll
+        for (int r = 0; r < N; ++r)
+        {
+            double a = 0.0;
+#pragma unroll
+            for (int c = 0; c < N; ++c) a += Fm[r * N + c] * x[c];
+            if (eta)
+                sm.G[le][r][ln * 4 + q] = a;
+            else
+                sm.F[le][ln][r * 4 + q] = a;
+        }
+    }
+
     // One tile = a tx x ty block of elements of one mesh, processed by one
     // workgroup (tx * ty <= EPB; the batched sweep may run smaller tiles).
-    // With `filt` (stage outputs only) the element-local spectral filter of
-    // the post-step hook is applied in LDS before the store.
+    // With FILT and `filt` (stage outputs only) the element-local spectral
+    // filter of the post-step hook is applied in LDS before the store.
     // Phases: (1) node state -> LDS, fluxes F, G -> LDS (outer traces of the
     // rim faces already in flight); (2) face Riemann fluxes -> Del; (3) volume
-    // term by xi- and eta-lines; (4) per node: volume + VCJH lift -> output.
-    template <int N, class IO, bool CURVED>
+    // term by xi- and eta-lines; (4) per node: volume + VCJH lift -> output
+    // [(5) stage value -> LDS, filter by xi- and eta-lines, store].
+    template <int N, class IO, bool CURVED, bool FILT>
     __device__ __forceinline__ void res2d_tile(const Res2DP& P, const IO& io, long long blk, int tx, int ty, bool filt,
                                                TileSmem<N, typename IO::S, CURVED>& sm)
     {
@@ -590,8 +620,6 @@ namespace frsdg
         constexpr int BS  = Tile2D<N, S>::BS;
         constexpr int NN  = N * N;
         const int     tid = threadIdx.x;
-        if (filt)
-            for (int q = tid; q < NN; q += BS) sm.Fm[q] = P.tab[NN + 2 * N + q];
         if (tid < N)
         {
             sm.dL[tid] = P.tab[NN + tid];
@@ -688,13 +716,13 @@ namespace frsdg
                 r[q] = r[q] + sm.Del[el][3][k][q] * sm.dR[l];
             }
         }
-        if constexpr (IO::can_filter)
+        if constexpr (FILT && IO::can_filter)
         {
             if (filt)
             {
                 // stage value -> LDS (each node overwrites only its own slot of
-                // U, which only this thread reads in this phase), then
-                // out = 1/2 (F_x v + F_y v)
+                // U, which only this thread reads in this phase; F and G are
+                // free after the previous barrier), then out = 1/2 (F_x v + F_y v)
                 if (act)
                 {
                     double o[4];
@@ -703,20 +731,21 @@ namespace frsdg
                     for (int q = 0; q < 4; ++q) sm.U[el][l][k * 4 + q] = o[q];
                 }
                 __syncthreads();
+                const int nlt = tx * ty * N * 4;
+                for (int t = tid; t < 2 * nlt; t += BS)
+                {
+                    const bool eta = t >= nlt;
+                    const int  w   = eta ? t - nlt : t;
+                    const int  le  = (w >> 2) / N;
+                    if (sm.eid[le] < 0) continue;
+                    filter_line<N, CURVED>(P, sm, eta, le, (w >> 2) - le * N, w & 3);
+                }
+                __syncthreads();
                 if (act)
                 {
                     double o[4];
 #pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                    {
-                        double sx = 0.0, sy = 0.0;
-                        for (int c = 0; c < N; ++c)
-                        {
-                            sx += sm.Fm[k * N + c] * sm.U[el][l][c * 4 + q];
-                            sy += sm.Fm[l * N + c] * sm.U[el][c][k * 4 + q];
-                        }
-                        o[q] = 0.5 * (sx + sy);
-                    }
+                    for (int q = 0; q < 4; ++q) o[q] = 0.5 * (sm.F[el][l][k * 4 + q] + sm.G[el][l][k * 4 + q]);
                     io.put4((e * NN + node) * 4, o);
                 }
                 return;
@@ -733,7 +762,7 @@ namespace frsdg
     {
         using T = Tile2D<N, typename IO::S>;
         __shared__ TileSmem<N, typename IO::S, CURVED> sm;
-        res2d_tile<N, IO, CURVED>(P, io, xcd_tile(blockIdx.x, gridDim.x), T::TX, T::TY, false, sm);
+        res2d_tile<N, IO, CURVED, false>(P, io, xcd_tile(blockIdx.x, gridDim.x), T::TX, T::TY, false, sm);
     }
     // last SSP-RK stage fused with the post-step spectral filter
     template <int N, bool CURVED>
@@ -741,7 +770,7 @@ namespace frsdg
     {
         using T = Tile2D<N, double>;
         __shared__ TileSmem<N, double, CURVED> sm;
-        res2d_tile<N, IOStage, CURVED>(P, io, xcd_tile(blockIdx.x, gridDim.x), T::TX, T::TY, true, sm);
+        res2d_tile<N, IOStage, CURVED, true>(P, io, xcd_tile(blockIdx.x, gridDim.x), T::TX, T::TY, true, sm);
     }
 
     // ------------------------------------------------------------------

@@ -202,9 +202,11 @@ int       frsdg_batch_instance(frsdg_batch* b, int k, frsdg_ctx** out);
 /* per-instance time steps (host array of count doubles), copied to the device; synchronous */
 int       frsdg_batch_set_dt(frsdg_batch* b, const double* dt);
 /* launch shape: persistent workgroups (0: one workgroup per element tile), element
- * tile = default tile of the order with its rows (then columns) halved log2(epb_div)
- * times, and mode 0 (one kernel per stage over all orders) or 1 (one kernel per stage,
- * order and mesh kind) */
+ * tile = default tile of the order (epb_div 1) or that tile with its rows (columns if it
+ * has one row) halved (epb_div 2), and mode 0 (one kernel per stage over all orders),
+ * 1 (one kernel per stage, order and mesh kind) or 2 (as 1, cut into runs of instances
+ * whose stage states fit in about half of the last-level cache, each run taking its
+ * three stages back to back) */
 int       frsdg_batch_set_tuning(frsdg_batch* b, int grid, int epb_div, int mode);
 int       frsdg_batch_ssprk_step(frsdg_batch* b, double* u, void* stream);
 /* nsteps steps replayed from a captured hipGraph */

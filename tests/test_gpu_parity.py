@@ -371,7 +371,8 @@ def test_batched_sweep_matches_per_instance_and_oracle():
     u0 = _host(u).copy()
     ua = u.clone()
     ub = u.clone()
-    for grid, div, mode in [(1024, 1, 1), (64, 2, 1), (1024, 1, 0), (7, 2, 0)]:
+    for grid, div, mode in [(0, 1, 1), (0, 2, 1), (1024, 1, 1), (64, 2, 1), (0, 1, 2), (64, 1, 2), (0, 1, 0),
+                            (7, 2, 0)]:
         b.set_tuning(grid, div, mode)
         ua.copy_(u)
         b.ssprk_step(ua)
@@ -383,14 +384,16 @@ def test_batched_sweep_matches_per_instance_and_oracle():
         o = Oracle(cfg)
         ref = o.ssprk_step(u0[b.offsets[k]:b.offsets[k + 1]], dts[k])
         assert C.rel_l2(_host(b.view(ua, k)), ref) <= TOL, k
-    # graph replay == stepping, on smooth pulses (10 steps)
-    b.set_tuning(1024, 1)
+    # graph replay == stepping, on smooth pulses (10 steps), persistent and
+    # one-workgroup-per-tile launches
     for k, iv in enumerate(b.instances):
         iv.impose_ic(fr.IC_PULSE, [0.2, 0.5, 1.0, 1.0, 1.0, 1.0], out=b.view(u, k))
-    uc = u.clone()
-    b.ssprk_run(uc, 10)
-    ud = u.clone()
-    for _ in range(10):
-        b.ssprk_step(ud)
-    assert np.isfinite(_host(uc)).all()
-    assert np.array_equal(_host(uc), _host(ud))
+    for grid, mode in ((1024, 1), (0, 1), (0, 2)):
+        b.set_tuning(grid, 1, mode)
+        uc = u.clone()
+        b.ssprk_run(uc, 10)
+        ud = u.clone()
+        for _ in range(10):
+            b.ssprk_step(ud)
+        assert np.isfinite(_host(uc)).all()
+        assert np.array_equal(_host(uc), _host(ud)), (grid, mode)
