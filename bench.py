@@ -192,6 +192,11 @@ def main():
             line["batched"] = bench_batch(torch, fr)
         except Exception as exc:
             line["batched"] = {"error": repr(exc)}
+        try:
+            # the same sweep on meshes 1/4 the size (N in 8..48): launch-bound instances
+            line["batched_small"] = bench_batch(torch, fr, steps=100, scale=0.25)
+        except Exception as exc:
+            line["batched_small"] = {"error": repr(exc)}
 
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
@@ -315,10 +320,11 @@ def bench_newton(torch, fr):
     return out
 
 
-def batch_setup(fr):
-    """The 64-instance radial shock-tube batch with its initial state and per-instance dt."""
+def batch_setup(fr, scale=1.0):
+    """The 64-instance radial shock-tube batch (meshes scaled by `scale`) with
+    its initial state and per-instance dt."""
     from fr_sdg_amd import configs as CF
-    cfgs = CF.batch64()
+    cfgs = CF.batch64(scale=scale)
     b = fr.Batch(cfgs)
     u = b.empty()
     dts = []
@@ -330,12 +336,12 @@ def batch_setup(fr):
     return b, u
 
 
-def bench_batch(torch, fr, steps=20):
+def bench_batch(torch, fr, steps=20, scale=1.0):
     """64 radial shock tubes (N in {32..192}, p in {2..6}, seeded), SSP-RK3 at
     CFL 0.4/(2p+1) per instance: 3 persistent batched launches per step (graph
     replay, autotuned launch shape) vs one set of launches per instance."""
     from fr_sdg_amd import autotune
-    b, u = batch_setup(fr)
+    b, u = batch_setup(fr, scale)
     best, table = autotune.tune_batch(b, u, use_cache=False)
     s = torch.cuda.current_stream()
 
@@ -358,7 +364,8 @@ def bench_batch(torch, fr, steps=20):
     t_e, we = timed(each)
     same = bool(torch.equal(wb, we))
     ok = bool(torch.isfinite(wb).all())
-    return {"workload": "64 radial shock tubes (tanh-smoothed jump, width 0.1), seeded N/p/gas, Rusanov, "
+    mesh = "seeded N" if scale == 1.0 else f"seeded N scaled by {scale}"
+    return {"workload": f"64 radial shock tubes (tanh-smoothed jump, width 0.1), {mesh}/p/gas, Rusanov, "
                         "filter s=16 a=36, SSP-RK3 at CFL 0.4/(2p+1)", "finite": ok,
             "ndofs": b.ndofs, "steps": steps, "batched_s_per_step": t_b, "per_instance_s_per_step": t_e,
             "speedup": t_e / t_b, "dof_rhs_per_s_batched": 3 * b.ndofs / t_b, "tuning": list(best),
