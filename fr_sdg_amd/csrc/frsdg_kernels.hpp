@@ -262,27 +262,44 @@ namespace frsdg
     // 2D residual: physics_impl_2D.hpp:267-357 (volume) + :689-899 (faces)
     // ------------------------------------------------------------------
     // LDS of one residual tile.  Node (k, l) of tile element el, equation q,
-    // lives at X[el][l][k * 4 + q]; rows are padded so that the row stride is
-    // an odd multiple of 32 bytes (line tasks of one wavefront then spread
-    // over all LDS banks).  Curved meshes also stage the node metrics:
-    // M[el][0|1][l][k] = xi_x|xi_y (rows), M[el][2|3][k][l] = eta_x|eta_y
-    // (columns), rows of odd length.
+    // lives at X[el][l][k * 4 + q]; the arrays that line tasks read or write
+    // by rows are padded so that the row stride is an odd multiple of 32
+    // bytes (a wavefront of line tasks then spreads over all LDS banks).  V is
+    // only written by columns and read per node: no padding (this keeps the
+    // curved n = 8 tile at 39 KB, four workgroups per CU).
     template <int N, class S, bool CURVED>
     struct TileSmem
     {
         static constexpr int NN = N * N, EPB = Tile2D<N, S>::EPB;
         static constexpr int RB = N * 4 * static_cast<int>(sizeof(S)); // row bytes
         static constexpr int RP = N * 4 + ((RB % 64) == 0 ? 32 / static_cast<int>(sizeof(S)) : 0);
-        static constexpr int MP = N + ((N % 2 == 0) ? 1 : 0);
         S         U[EPB][N][RP]; // state; after the face phase: xi-line part of the volume term
         S         F[EPB][N][RP];
         S         G[EPB][N][RP]; // affine meshes: eta-line part of the volume term after the line phase
-        S         V[CURVED ? EPB : 1][N][CURVED ? RP : 1]; // curved meshes: eta-line part
+        S         V[CURVED ? EPB : 1][N][CURVED ? N * 4 : 1]; // curved meshes: eta-line part
         S         Del[EPB][4][N][4];
-        double    M[CURVED ? EPB : 1][4][N][CURVED ? MP : 1];
         double    dL[N], dR[N];
         long long eid[EPB];      // global element of each tile slot, -1 if outside the mesh
     };
+
+    // Metric pair of a curved line task, from the per-node metrics in HBM:
+    // xi-line (row ln): (xi_x, xi_y) of nodes (k, ln); eta-line (column ln):
+    // (eta_x, eta_y) of nodes (ln, l).
+    template <int N>
+    __device__ __forceinline__ void line_metrics(const double* __restrict__ vmet, long long e, bool eta, int ln,
+                                                 double ma[N], double mb[N])
+    {
+        constexpr int NN = N * N;
+        const double* b  = vmet + (e * NN) * 4 + (eta ? 2 : 0);
+#pragma unroll
+        for (int m = 0; m < N; ++m)
+        {
+            const int    node = eta ? m * N + ln : ln * N + m;
+            const double2 v   = *reinterpret_cast<const double2*>(b + node * 4);
+            ma[m]             = v.x;
+            mb[m]             = v.y;
+        }
+    }
 
     // Tile origin and shape.  Tiles cover the element grid row-major; the
     // last tile of a row or column may hang over the mesh (its outside slots
@@ -519,11 +536,12 @@ namespace frsdg
     //   xi-line  (row l):    U[l][k] <- xi_x d/dxi F + xi_y d/dxi G   (k = 0..N-1)
     //   eta-line (column k): V[l][k] <- eta_x d/deta F + eta_y d/deta G (l = 0..N-1)
     //                        (affine: G[l][k] <- eta_y d/deta G)
-    // The line is held in registers; D comes through the scalar cache.  Affine
-    // meshes keep only d/dxi F and d/deta G (the cross terms are exact zeros).
+    // The line is held in registers; D comes through the scalar cache, the
+    // metric pair (ma, mb) of the line's nodes from registers.  Affine meshes
+    // keep only d/dxi F and d/deta G (the cross terms are exact zeros).
     template <int N, class S, bool CURVED>
     __device__ __forceinline__ void volume_line(const Res2DP& P, TileSmem<N, S, CURVED>& sm, bool eta, int le,
-                                                int ln, int q)
+                                                int ln, int q, const double* ma = nullptr, const double* mb = nullptr)
     {
         const double* D = P.tab;
         if (!eta)
@@ -546,7 +564,7 @@ namespace frsdg
                     if constexpr (CURVED) dG = dG + D[k * N + m] * g[m];
                 }
                 if constexpr (CURVED)
-                    sm.U[le][ln][k * 4 + q] = sm.M[le][0][ln][k] * dF + sm.M[le][1][ln][k] * dG;
+                    sm.U[le][ln][k * 4 + q] = ma[k] * dF + mb[k] * dG;
                 else
                     sm.U[le][ln][k * 4 + q] = P.xi_x * dF;
             }
@@ -573,7 +591,7 @@ namespace frsdg
                 // affine: the result replaces column ln of G (equation q), which
                 // only this task reads (the xi-lines read F only)
                 if constexpr (CURVED)
-                    sm.V[le][l][ln * 4 + q] = sm.M[le][2][ln][l] * dF + sm.M[le][3][ln][l] * dG;
+                    sm.V[le][l][ln * 4 + q] = ma[l] * dF + mb[l] * dG;
                 else
                     sm.G[le][l][ln * 4 + q] = P.eta_y * dG;
             }
@@ -639,19 +657,25 @@ namespace frsdg
         FaceTask<S> f0;
         face_task<N, IO>(P, io, g, tid, f0);
 
+        // curved: metrics of this thread's first line task, in flight with the state
+        const int nlt = tx * ty * N * 4;
+        double    ma[CURVED ? N : 1], mb[CURVED ? N : 1];
+        if constexpr (CURVED)
+        {
+            const bool eta = tid >= nlt;
+            const int  w   = eta ? tid - nlt : tid;
+            const int  le  = (w >> 2) / N;
+            const int  lb  = le / tx;
+            const int  la  = le - lb * tx;
+            if (tid < 2 * nlt && g.i0 + la < P.nx && g.j0 + lb < P.ny)
+                line_metrics<N>(P.vmet, static_cast<long long>(g.j0 + lb) * P.nx + (g.i0 + la), eta,
+                                (w >> 2) - le * N, ma, mb);
+        }
+
         S own[4];
         if (act)
         {
             io.ld4((e * NN + node) * 4, own);
-            if constexpr (CURVED)
-            {
-                double mt[4];
-                ld4d(P.vmet + (e * NN + node) * 4, mt);
-                sm.M[el][0][l][k] = mt[0];
-                sm.M[el][1][l][k] = mt[1];
-                sm.M[el][2][k][l] = mt[2];
-                sm.M[el][3][k][l] = mt[3];
-            }
             S F[4], G[4];
             flux2(P.gas, own, F, G);
 #pragma unroll
@@ -679,8 +703,7 @@ namespace frsdg
         // eta-line tasks (4 contractions per node); affine: one task does
         // xi-line ln of F and eta-line ln of G (2 contractions per node) ----
         {
-            const int nlt = tx * ty * N * 4;
-            const int nt  = CURVED ? 2 * nlt : nlt;
+            const int nt = CURVED ? 2 * nlt : nlt;
             for (int t = tid; t < nt; t += BS)
             {
                 const bool eta = t >= nlt;
@@ -690,7 +713,10 @@ namespace frsdg
                 const int  ln  = (w >> 2) - le * N;
                 if (sm.eid[le] < 0) continue;
                 if constexpr (CURVED)
-                    volume_line<N, S, CURVED>(P, sm, eta, le, ln, q);
+                {
+                    if (t >= BS) line_metrics<N>(P.vmet, sm.eid[le], eta, ln, ma, mb); // later passes
+                    volume_line<N, S, CURVED>(P, sm, eta, le, ln, q, ma, mb);
+                }
                 else
                 {
                     volume_line<N, S, CURVED>(P, sm, false, le, ln, q);
@@ -731,7 +757,6 @@ namespace frsdg
                     for (int q = 0; q < 4; ++q) sm.U[el][l][k * 4 + q] = o[q];
                 }
                 __syncthreads();
-                const int nlt = tx * ty * N * 4;
                 for (int t = tid; t < 2 * nlt; t += BS)
                 {
                     const bool eta = t >= nlt;
