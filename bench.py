@@ -401,6 +401,13 @@ def adjoint_large(torch, fr, args):
     J0, g = A.rho_squared(ev, uT, m)
     print(f"[adjoint-large] forward {t_fwd:.1f} s J={J0:.15e}", file=sys.stderr, flush=True)
     adj.forward_steps = 0
+    t_start = time.perf_counter()
+
+    def progress(a):
+        if a.adjoint_steps % 200 == 0:
+            print(f"[adjoint-large] {a.adjoint_steps}/{T} adjoint steps, {a.forward_steps} forward steps, "
+                  f"{time.perf_counter() - t_start:.1f} s", file=sys.stderr, flush=True)
+    adj.on_adjoint_step = progress
     lam, t_adj = timed(lambda: adj.gradient(u0, g))
     print(f"[adjoint-large] adjoint {t_adj:.1f} s, recomputed steps {adj.forward_steps}", file=sys.stderr, flush=True)
     x = torch.arange(u0.numel(), device="cuda", dtype=torch.float64)

@@ -37,6 +37,7 @@ class UnsteadyAdjoint:
         self.adjoint_steps = 0
         self.peak_snapshots = 0
         self._live = 0
+        self.on_adjoint_step = None  # optional callback(self) after each adjoint step
 
     # ---- forward ----
     def advance(self, u, k):
@@ -60,7 +61,10 @@ class UnsteadyAdjoint:
         l2 = (2.0 / 3.0) * (l3 - dt * ev.vjp(u2, l3))
         l1 = 0.25 * (l2 - dt * ev.vjp(u1, l2))
         self.adjoint_steps += 1
-        return (1.0 / 3.0) * l3 + 0.75 * l2 + l1 - dt * ev.vjp(u, l1)
+        out = (1.0 / 3.0) * l3 + 0.75 * l2 + l1 - dt * ev.vjp(u, l1)
+        if self.on_adjoint_step is not None:
+            self.on_adjoint_step(self)
+        return out
 
     def _reverse(self, a, b, ua, lam, free):
         T = b - a
