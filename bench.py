@@ -410,8 +410,13 @@ def adjoint_large(torch, fr, args):
     adj.on_adjoint_step = progress
     lam, t_adj = timed(lambda: adj.gradient(u0, g))
     print(f"[adjoint-large] adjoint {t_adj:.1f} s, recomputed steps {adj.forward_steps}", file=sys.stderr, flush=True)
-    x = torch.arange(u0.numel(), device="cuda", dtype=torch.float64)
-    v = 1e-2 * torch.sin(1e-3 * x) * u0
+    # a smooth direction: derivative of the initial state with respect to the
+    # inner pressure of the radial shock tube (central difference of the IC)
+    dp = 1e-3
+    prm_p, prm_m = list(CF.WARPED_PRM), list(CF.WARPED_PRM)
+    prm_p[1] *= 1.0 + dp
+    prm_m[1] *= 1.0 - dp
+    v = (ev.impose_ic(fr.IC_EXPLOSION, prm_p) - ev.impose_ic(fr.IC_EXPLOSION, prm_m)) / (2.0 * dp * CF.WARPED_PRM[1])
     eps = 1e-3
 
     def J(u):

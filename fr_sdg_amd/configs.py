@@ -9,7 +9,7 @@ are the frsdg_desc fields), and their initial-condition parameters.
 * batch64      — 64 radial shock tubes, N in {32,48,...,192}, p in {2..6}, half
                  ideal / half van der Waals, Rusanov, filter s=16 alpha=36 n_c=0
                  (factory.hpp:400-465 CircularExplosion on [0,2]^2)
-* warped       — [-1,1]^2 sine-warped, 1024^2, p=7, van der Waals, HLLC + filter
+* warped       — [-1,1]^2 sine-warped, 1024^2, p=7, van der Waals, HLLC + filter (order 8)
 """
 import math
 
@@ -120,9 +120,12 @@ def batch64(count=64, scale=1.0):
 
 
 def warped(n=1024, p=7, filt=1):
+    """Filter order 8 (alpha 36): with order 16 the p=7 run on this map loses
+    positivity within 200 steps at CFL 0.4/(2p+1), also from a smoother
+    initial jump; order 8 keeps the 2000-step run positive."""
     return dict(dims=2, nx=n, ny=n, p=p, lx=2.0, ly=2.0, riemann=1, geometry=4,
                 geom_param=[0.05, -1.0, -1.0, 0.0], bc_type=[1, 1, 1, 1], gas_kind=1, vdw_a=0.1,
-                vdw_b=0.02, apply_filter=filt, filt_cutoff=0, filt_strength=16, filt_alpha=36.0)
+                vdw_b=0.02, apply_filter=filt, filt_cutoff=0, filt_strength=8, filt_alpha=36.0)
 
 
 WARPED_PRM = [1.0, 1.0, 0.125, 0.1, 0.4, 0.0, 0.0, 0.1]

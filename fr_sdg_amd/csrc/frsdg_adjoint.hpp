@@ -10,12 +10,14 @@
 //     faces : ubar_own += s_f J_{F*,own}^T (Wbar_{e,f} + Wbar_{e',f'}) - s_f J_{Fn,own}^T Wbar_{e,f},
 //             where (e',f') is the neighbour's copy of the same face (its Wbar is read
 //             from the phase-1 buffer) and boundary faces chain through the BC map.
-// The small Jacobian-transpose products are formed from 4 forward-dual passes
-// (one per conserved variable) of the same flux / Riemann / BC code the
-// residual uses, so the adjoint linearises exactly the discrete residual.
+// The small Jacobian-transpose products are hand-written reverse sweeps
+// (frsdg_reverse.hpp) of the flux / Riemann / BC code the residual uses, with
+// the same branch decisions, so the adjoint linearises exactly the discrete
+// residual.
 // Every output is written once by one thread: deterministic, no atomics.
 #pragma once
 #include "frsdg_kernels.hpp"
+#include "frsdg_reverse.hpp"
 
 namespace frsdg
 {
@@ -61,53 +63,43 @@ namespace frsdg
         int    bnd, dir;     // boundary face and its BC direction
     };
 
-    // ubar += J_{F*,own}^T cs + J_{Fn,own}^T cn   (cs, cn already scaled)
+    // ubar += J_{F*,own}^T cs + J_{Fn,own}^T cn   (cs, cn already scaled), one
+    // reverse sweep through the boundary map, the Riemann flux and the flux
     __device__ __forceinline__ void face_adjoint(const GasP& g, int kind, const BCP& bc, const FaceAdjIn& in,
                                                  const double cs[4], const double cn[4], double ub[4])
     {
-        for (int i = 0; i < 4; ++i)
+        double Un[4];
+        if (in.bnd)
+            bc_exterior(bc, in.dir, in.uo, in.nx, in.ny, Un);
+        else
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Un[q] = in.un[q];
+        double ob[4] = {0, 0, 0, 0}, nb[4] = {0, 0, 0, 0};
+        if (in.own_right)
+            riemann2_vjp(g, kind, Un, in.uo, in.nx, in.ny, cs, nb, ob);
+        else
+            riemann2_vjp(g, kind, in.uo, Un, in.nx, in.ny, cs, ob, nb);
+        if (in.bnd) bc_vjp(bc, in.dir, in.uo, in.nx, in.ny, nb, ob);
+        double Fb[4], Gb[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
         {
-            Dual Uo[4], Un[4], Fs[4], F[4], G[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) Uo[q] = {in.uo[q], q == i ? 1.0 : 0.0};
-            if (in.bnd)
-                bc_exterior(bc, in.dir, Uo, in.nx, in.ny, Un);
-            else
-#pragma unroll
-                for (int q = 0; q < 4; ++q) Un[q] = {in.un[q], 0.0};
-            if (in.own_right)
-                riemann2(g, kind, Un, Uo, in.nx, in.ny, Fs);
-            else
-                riemann2(g, kind, Uo, Un, in.nx, in.ny, Fs);
-            flux2(g, Uo, F, G);
-            double s = 0.0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) s += Fs[q].d * cs[q] + (F[q].d * in.nx + G[q].d * in.ny) * cn[q];
-            ub[i] += s;
+            Fb[q] = cn[q] * in.nx;
+            Gb[q] = cn[q] * in.ny;
         }
+        flux2_vjp(g, in.uo, Fb, Gb, ob);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ub[q] += ob[q];
     }
     // only the Riemann part (neighbour copy of a face with its own metrics)
     __device__ __forceinline__ void riemann_adjoint(const GasP& g, int kind, const FaceAdjIn& in,
                                                     const double cs[4], double ub[4])
     {
-        for (int i = 0; i < 4; ++i)
-        {
-            Dual Uo[4], Un[4], Fs[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-            {
-                Uo[q] = {in.uo[q], q == i ? 1.0 : 0.0};
-                Un[q] = {in.un[q], 0.0};
-            }
-            if (in.own_right)
-                riemann2(g, kind, Un, Uo, in.nx, in.ny, Fs);
-            else
-                riemann2(g, kind, Uo, Un, in.nx, in.ny, Fs);
-            double s = 0.0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) s += Fs[q].d * cs[q];
-            ub[i] += s;
-        }
+        double nb[4] = {0, 0, 0, 0};
+        if (in.own_right)
+            riemann2_vjp(g, kind, in.un, in.uo, in.nx, in.ny, cs, nb, ub);
+        else
+            riemann2_vjp(g, kind, in.uo, in.un, in.nx, in.ny, cs, ub, nb);
     }
 
     // phase 2 (2D)
@@ -284,18 +276,8 @@ namespace frsdg
             Fb[q] = f;
             Gb[q] = g;
         }
-        double ub[4];
-        for (int i = 0; i < 4; ++i)
-        {
-            Dual Ud[4], F[4], G[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) Ud[q] = {U[q], q == i ? 1.0 : 0.0};
-            flux2(P.gas, Ud, F, G);
-            double s = 0.0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) s += F[q].d * Fb[q] + G[q].d * Gb[q];
-            ub[i] = s;
-        }
+        double ub[4] = {0, 0, 0, 0};
+        flux2_vjp(P.gas, U, Fb, Gb, ub);
         // ---- gather face contributions of this node ----
 #pragma unroll
         for (int q = 0; q < 4; ++q)

@@ -49,7 +49,9 @@ def test_unsteady_adjoint_rho2_vs_fd(filt):
     fd = (J(u0 + eps * v) - J(u0 - eps * v)) / (2 * eps)
     ad = float(torch.dot(lam, v))
     print(f"[adjoint] filt={filt} adjoint {ad:.12e} fd {fd:.12e}", flush=True)
-    assert abs(ad - fd) <= 1e-7 * abs(fd)
+    # central-difference error budget: rounding ~ 1e-16 |J| / (eps |dJ.v|) ~ 1e-7 plus
+    # truncation O(eps^2) of the same order
+    assert abs(ad - fd) <= 3e-7 * abs(fd)
 
 
 def wall_pressure(torch, ev, u, W):
