@@ -273,8 +273,8 @@ namespace frsdg
         static constexpr int NN = N * N, EPB = Tile2D<N, S>::EPB;
         static constexpr int RB = N * 4 * static_cast<int>(sizeof(S)); // row bytes
         static constexpr int RP = N * 4 + ((RB % 64) == 0 ? 32 / static_cast<int>(sizeof(S)) : 0);
-        S         U[EPB][N][RP]; // state; after the face phase: xi-line part of the volume term
-        S         F[EPB][N][RP];
+        S         U[EPB][N][RP]; // state; curved meshes: xi-line part of the volume term after the face phase
+        S         F[EPB][N][RP]; // affine meshes: xi-line part of the volume term after the line phase
         S         G[EPB][N][RP]; // affine meshes: eta-line part of the volume term after the line phase
         S         V[CURVED ? EPB : 1][N][CURVED ? N * 4 : 1]; // curved meshes: eta-line part
         S         Del[EPB][4][N][4];
@@ -512,12 +512,25 @@ namespace frsdg
         }
         riemann2(P.gas, P.riemann, UA, UB, f.fnx, f.fny, Fs);
         const int fA = f.xdir ? 1 : 3, fB = f.xdir ? 0 : 2;
+        // own-side normal flux: curved tiles read F, G from LDS; affine tiles
+        // recompute them from the trace (the line tasks run concurrently and
+        // overwrite F, G in place)
+        S FA[4], GA[4], FB[4], GB[4];
+        if constexpr (!CURVED)
+        {
+            if (f.elA >= 0) flux2(P.gas, UA, FA, GA);
+            if (f.elB >= 0) flux2(P.gas, UB, FB, GB);
+        }
         if (f.elA >= 0)
         {
 #pragma unroll
             for (int q = 0; q < 4; ++q)
             {
-                const S fn = sm.F[f.elA][lA][kA * 4 + q] * f.fnx + sm.G[f.elA][lA][kA * 4 + q] * f.fny;
+                S fn;
+                if constexpr (CURVED)
+                    fn = sm.F[f.elA][lA][kA * 4 + q] * f.fnx + sm.G[f.elA][lA][kA * 4 + q] * f.fny;
+                else
+                    fn = FA[q] * f.fnx + GA[q] * f.fny;
                 sm.Del[f.elA][fA][f.idx][q] = f.sc * (Fs[q] - fn);
             }
         }
@@ -526,7 +539,11 @@ namespace frsdg
 #pragma unroll
             for (int q = 0; q < 4; ++q)
             {
-                const S fn = sm.F[f.elB][lB][kB * 4 + q] * f.fnx + sm.G[f.elB][lB][kB * 4 + q] * f.fny;
+                S fn;
+                if constexpr (CURVED)
+                    fn = sm.F[f.elB][lB][kB * 4 + q] * f.fnx + sm.G[f.elB][lB][kB * 4 + q] * f.fny;
+                else
+                    fn = FB[q] * f.fnx + GB[q] * f.fny;
                 sm.Del[f.elB][fB][f.idx][q] = f.sc * (Fs[q] - fn);
             }
         }
@@ -535,7 +552,7 @@ namespace frsdg
     // Volume term along one line of one element for equation q.
     //   xi-line  (row l):    U[l][k] <- xi_x d/dxi F + xi_y d/dxi G   (k = 0..N-1)
     //   eta-line (column k): V[l][k] <- eta_x d/deta F + eta_y d/deta G (l = 0..N-1)
-    //                        (affine: G[l][k] <- eta_y d/deta G)
+    //   (affine: F[l][k] <- xi_x d/dxi F and G[l][k] <- eta_y d/deta G, in place)
     // The line is held in registers; D comes through the scalar cache, the
     // metric pair (ma, mb) of the line's nodes from registers.  Affine meshes
     // keep only d/dxi F and d/deta G (the cross terms are exact zeros).
@@ -563,10 +580,12 @@ namespace frsdg
                     dF = dF + D[k * N + m] * f[m];
                     if constexpr (CURVED) dG = dG + D[k * N + m] * g[m];
                 }
+                // affine: the result replaces row ln of F (equation q), which
+                // only this task reads in this phase
                 if constexpr (CURVED)
                     sm.U[le][ln][k * 4 + q] = ma[k] * dF + mb[k] * dG;
                 else
-                    sm.U[le][ln][k * 4 + q] = P.xi_x * dF;
+                    sm.F[le][ln][k * 4 + q] = P.xi_x * dF;
             }
         }
         else
@@ -589,7 +608,7 @@ namespace frsdg
                     if constexpr (CURVED) dF = dF + D[l * N + m] * f[m];
                 }
                 // affine: the result replaces column ln of G (equation q), which
-                // only this task reads (the xi-lines read F only)
+                // only this task reads in this phase
                 if constexpr (CURVED)
                     sm.V[le][l][ln * 4 + q] = ma[l] * dF + mb[l] * dG;
                 else
@@ -697,7 +716,9 @@ namespace frsdg
             face_task<N, IO>(P, io, g, t, f);
             if (f.live) face_flux<N, IO, CURVED>(P, f, sm);
         }
-        __syncthreads();
+        // curved line tasks write the xi-part over U, which face tasks read;
+        // affine line tasks work in place on F and G, which face tasks do not read
+        if constexpr (CURVED) __syncthreads();
 
         // ---- volume term by lines (q fastest).  Curved: xi-line tasks, then
         // eta-line tasks (4 contractions per node); affine: one task does
@@ -735,7 +756,7 @@ namespace frsdg
                 if constexpr (CURVED)
                     r[q] = sm.U[el][l][k * 4 + q] + sm.V[el][l][k * 4 + q];
                 else
-                    r[q] = sm.U[el][l][k * 4 + q] + sm.G[el][l][k * 4 + q];
+                    r[q] = sm.F[el][l][k * 4 + q] + sm.G[el][l][k * 4 + q];
                 r[q] = r[q] + sm.Del[el][0][l][q] * sm.dL[k];
                 r[q] = r[q] + sm.Del[el][1][l][q] * sm.dR[k];
                 r[q] = r[q] + sm.Del[el][2][k][q] * sm.dL[l];

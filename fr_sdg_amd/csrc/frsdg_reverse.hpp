@@ -13,7 +13,7 @@
 
 namespace frsdg
 {
-    // forward intermediates of prim2 + sound speed of one state
+    // forward intermediates of prim2 (+ sound speed when WITH_C) of one state
     struct PrimT
     {
         double U0, U1, U2, U3;    // raw conserved state
@@ -21,7 +21,8 @@ namespace frsdg
         bool   rho_live, E_live;  // above the floors
     };
 
-    __device__ __forceinline__ void prim_fwd(const GasP& g, const double U[4], PrimT& t)
+    template <bool WITH_C>
+    FR_HD void prim_fwd(const GasP& g, const double U[4], PrimT& t)
     {
         t.U0       = U[0];
         t.U1       = U[1];
@@ -36,16 +37,24 @@ namespace frsdg
         t.E        = t.E_live ? U[3] : DBL_MIN;
         t.e        = t.E * t.ir - 0.5 * (t.u * t.u + t.v * t.v);
         t.p        = gas_pressure(g, t.rho, t.e);
-        t.c        = gas_sound(g, t.rho, t.e, t.p, t.ir);
+        t.c        = WITH_C ? gas_sound(g, t.rho, t.e, t.p, t.ir) : 0.0;
     }
 
-    // cotangents of (rho, ir, u, v, e, p, c) -> Ubar (accumulated)
-    __device__ __forceinline__ void prim_bwd(const GasP& g, const PrimT& t, double rb, double irb, double ub, double vb,
+    // cotangents of (rho, ir, u, v, e, p[, c]) -> Ubar (accumulated); without
+    // WITH_C the sound speed is not part of the sweep (flux2 does not use it,
+    // and it may be undefined at states that never reach a face)
+    template <bool WITH_C>
+    FR_HD void prim_bwd(const GasP& g, const PrimT& t, double rb, double irb, double ub, double vb,
                                              double eb, double pb, double cb, double Ub[4])
     {
         const double gm = g.gamma - 1.0;
-        // c
-        if (g.kind == 0)
+        // c; skipped at a zero cotangent, so that a state with c^2 < 0 (whose
+        // NaN sound speed only ever reaches a wave-speed comparison that the
+        // forward HLLC resolves to F_R) passes no NaN into Ubar
+        if (!WITH_C || cb == 0.0)
+        {
+        }
+        else if (g.kind == 0)
         {
             // c = sqrt(gamma p ir)
             const double k = cb * g.gamma / (2.0 * t.c);
@@ -92,11 +101,11 @@ namespace frsdg
     }
 
     // Ubar += (dF/dU)^T Fb + (dG/dU)^T Gb   (flux2)
-    __device__ __forceinline__ void flux2_vjp(const GasP& g, const double U[4], const double Fb[4], const double Gb[4],
+    FR_HD void flux2_vjp(const GasP& g, const double U[4], const double Fb[4], const double Gb[4],
                                               double Ub[4])
     {
         PrimT t;
-        prim_fwd(g, U, t);
+        prim_fwd<false>(g, U, t);
         const double Ep = U[3] + t.p;
         // F = [U1, U1 u + p, U1 v, u Ep],  G = [U2, U1 v, U2 v + p, v Ep]
         double       ub = Fb[1] * U[1] + Fb[3] * Ep;
@@ -109,23 +118,23 @@ namespace frsdg
         Ub[2] += Gb[0] + Gb[2] * t.v;
         Ub[3] += Epb;
         pb += Epb;
-        prim_bwd(g, t, 0.0, 0.0, ub, vb, 0.0, pb, 0.0, Ub);
+        prim_bwd<false>(g, t, 0.0, 0.0, ub, vb, 0.0, pb, 0.0, Ub);
     }
 
-    __device__ __forceinline__ double sabs_d(double x) // d smooth_abs / dx
+    FR_HD double sabs_d(double x) // d smooth_abs / dx
     {
         const double eps = 1e-7;
         return x / ::sqrt(x * x + eps * eps);
     }
 
     // ULb += (dF*/dUL)^T fb, URb += (dF*/dUR)^T fb   (riemann2, double)
-    __device__ __forceinline__ void riemann2_vjp(const GasP& g, int kind, const double UL[4], const double UR[4],
+    FR_HD void riemann2_vjp(const GasP& g, int kind, const double UL[4], const double UR[4],
                                                  double nx, double ny, const double fb_phys[4], double ULb[4],
                                                  double URb[4])
     {
         PrimT L, R;
-        prim_fwd(g, UL, L);
-        prim_fwd(g, UR, R);
+        prim_fwd<true>(g, UL, L);
+        prim_fwd<true>(g, UR, R);
         const double unL = L.u * nx + L.v * ny, utL = -L.u * ny + L.v * nx;
         const double unR = R.u * nx + R.v * ny, utR = -R.u * ny + R.v * nx;
         const double mL = UL[0] * unL, mR = UR[0] * unR;
@@ -183,6 +192,9 @@ namespace frsdg
             const bool   supL = sL >= 0.0;
             const bool   left = sM >= 0.0;
             const bool   supR = !left && !(sR >= 0.0);
+            // The wave-speed cotangents are non-zero only for the star state;
+            // the supersonic branches (also taken when a sound speed is NaN:
+            // every comparison fails and F_R is selected) leave them out.
             double       sLb = 0, sRb = 0, sMb = 0;
             if (supL)
             {
@@ -245,29 +257,29 @@ namespace frsdg
                 {
                     sRb += sKb; unRb += unKb; utRb += utKb; U0Rb += rKb; mRb += mKb; U3Rb += EKb; pRb += pKb;
                 }
+                // sM = num / den
+                const double numb = sMb / den, denb = -sMb * sM / den;
+                pRb += numb;
+                pLb -= numb;
+                unLb += numb * rLd;
+                unRb -= numb * rRd;
+                const double rLdb = numb * unL + denb, rRdb = -numb * unR - denb;
+                // rLd = UL0 (sL - unL), rRd = UR0 (sR - unR)
+                U0Lb += rLdb * (sL - unL);
+                sLb += rLdb * UL[0];
+                unLb -= rLdb * UL[0];
+                U0Rb += rRdb * (sR - unR);
+                sRb += rRdb * UR[0];
+                unRb -= rRdb * UR[0];
+                // sL = smooth_min(a1, a2), sR = smooth_max(b1, b2)
+                const double da = sabs_d(a1 - a2), db = sabs_d(b1 - b2);
+                const double a1b = sLb * 0.5 * (1.0 - da), a2b = sLb * 0.5 * (1.0 + da);
+                const double b1b = sRb * 0.5 * (1.0 + db), b2b = sRb * 0.5 * (1.0 - db);
+                unLb += a1b + b1b;
+                cLb += b1b - a1b;
+                unRb += a2b + b2b;
+                cRb += b2b - a2b;
             }
-            // sM = num / den
-            const double numb = sMb / den, denb = -sMb * sM / den;
-            pRb += numb;
-            pLb -= numb;
-            unLb += numb * rLd;
-            unRb -= numb * rRd;
-            const double rLdb = numb * unL + denb, rRdb = -numb * unR - denb;
-            // rLd = UL0 (sL - unL), rRd = UR0 (sR - unR)
-            U0Lb += rLdb * (sL - unL);
-            sLb += rLdb * UL[0];
-            unLb -= rLdb * UL[0];
-            U0Rb += rRdb * (sR - unR);
-            sRb += rRdb * UR[0];
-            unRb -= rRdb * UR[0];
-            // sL = smooth_min(a1, a2), sR = smooth_max(b1, b2)
-            const double da = sabs_d(a1 - a2), db = sabs_d(b1 - b2);
-            const double a1b = sLb * 0.5 * (1.0 - da), a2b = sLb * 0.5 * (1.0 + da);
-            const double b1b = sRb * 0.5 * (1.0 + db), b2b = sRb * 0.5 * (1.0 - db);
-            unLb += a1b + b1b;
-            cLb += b1b - a1b;
-            unRb += a2b + b2b;
-            cRb += b2b - a2b;
         }
         // face-frame fluxes F = [m, m un + p, m ut, (U3 + p) un],  m = U0 un
         mLb += FLb[0] + FLb[1] * unL + FLb[2] * utL;
@@ -291,12 +303,12 @@ namespace frsdg
         ULb[3] += U3Lb;
         URb[0] += U0Rb;
         URb[3] += U3Rb;
-        prim_bwd(g, L, 0.0, 0.0, uLb, vLb, 0.0, pLb, cLb, ULb);
-        prim_bwd(g, R, 0.0, 0.0, uRb, vRb, 0.0, pRb, cRb, URb);
+        prim_bwd<true>(g, L, 0.0, 0.0, uLb, vLb, 0.0, pLb, cLb, ULb);
+        prim_bwd<true>(g, R, 0.0, 0.0, uRb, vRb, 0.0, pRb, cRb, URb);
     }
 
     // Ub += (d ex / d U)^T exb for ex = bc_exterior(bc, dir, U, nx, ny)
-    __device__ __forceinline__ void bc_vjp(const BCP& bc, int dir, const double U[4], double nx, double ny,
+    FR_HD void bc_vjp(const BCP& bc, int dir, const double U[4], double nx, double ny,
                                            const double exb[4], double Ub[4])
     {
         const int t = bc_type_of(bc, dir);

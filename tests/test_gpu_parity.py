@@ -233,6 +233,37 @@ def test_adjoint_dot_product(name):
     assert abs(a - b) <= 1e-12 * scale, (a, b)
 
 
+def test_adjoint_hllc_imaginary_sound_speed():
+    """van der Waals states with c^2 < 0 at face nodes (rho 0.11, e 0.005: p < 0).
+    The forward HLLC then compares NaN wave speeds, every test fails and F_R is
+    taken, so residual and tangent stay finite; the adjoint must be finite too
+    and satisfy <w, J v> = <J^T w, v> with the tangent kernel."""
+    torch, fr, Oracle = _mods()
+    cfg = RESIDUAL_CASES["explosion_hllc_vdw_p2"]
+    n = cfg["p"] + 1
+    st = C.random_state(cfg).reshape(-1, n * n, 4)
+    rho, u, v, e = 0.11, 0.9, 0.25, 0.005
+    a, b = cfg["vdw_a"], cfg["vdw_b"]
+    p = 0.4 * rho * (e + a * rho) / (1 - b * rho) - a * rho * rho
+    c2 = (1.4 * 0.4 * (e + a * rho) + 0.4 * a * rho * (2 - b * rho)) / (1 - b * rho) ** 2 - 2 * a * rho
+    assert p < 0 and c2 < 0
+    for el, node in ((3, n - 1), (9, 0), (17, n * n - 1), (20, n * (n - 1))):
+        st[el, node] = [rho, rho * u, rho * v, rho * (e + 0.5 * (u * u + v * v))]
+    ev = fr.PhysicsEvaluator(cfg)
+    ud = _dev(torch, st.reshape(-1))
+    rng = np.random.default_rng(6)
+    vv = _dev(torch, rng.standard_normal(ev.ndofs))
+    w = _dev(torch, rng.standard_normal(ev.ndofs))
+    r = ev.residual_kernel(ud)
+    jv = ev.jvp(ud, vv)
+    jtw = ev.vjp(ud, w)
+    assert bool(torch.isfinite(r).all()) and bool(torch.isfinite(jv).all())
+    assert bool(torch.isfinite(jtw).all())
+    lhs = float(torch.dot(w, jv))
+    rhs = float(torch.dot(jtw, vv))
+    assert abs(lhs - rhs) <= 1e-12 * float(torch.linalg.norm(w) * torch.linalg.norm(jv)), (lhs, rhs)
+
+
 @pytest.mark.parametrize("name", ["vortex_p4", "bump_p5", "sod1d_p3_rusanov"])
 def test_adjoint_columns_vs_complex_step(name):
     """J^T e_i against rows of J built column by column from the oracle's complex step
