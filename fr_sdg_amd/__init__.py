@@ -50,6 +50,8 @@ class FrsdgDesc(ctypes.Structure):
         ("apply_filter", ctypes.c_int), ("filt_cutoff", ctypes.c_int),
         ("filt_strength", ctypes.c_int), ("filt_alpha", ctypes.c_double),
         ("jac_type", ctypes.c_int), ("tangent_bc_mode", ctypes.c_int),
+        ("apply_weno", ctypes.c_int), ("weno_gamma", ctypes.c_double),
+        ("shock_theta", ctypes.c_double), ("shock_threshold", ctypes.c_double),
     ]
 
 
@@ -174,6 +176,10 @@ def make_desc(cfg):
     d.filt_alpha = cfg.get("filt_alpha", 36.0)
     d.jac_type = cfg.get("jac_type", 0)
     d.tangent_bc_mode = cfg.get("tangent_bc_mode", 0)
+    d.apply_weno = cfg.get("apply_weno", 0)
+    d.weno_gamma = cfg.get("weno_gamma", 0.998)  # parser.hpp WenoConstants defaults
+    d.shock_theta = cfg.get("shock_theta", 1.0)
+    d.shock_threshold = cfg.get("shock_threshold", 0.8)
     return d
 
 

@@ -455,21 +455,60 @@ namespace
         }
     };
 
+    // WENO limiter with shock sensor (limit_solution with -apply_weno and no
+    // spectral filter); o must not alias u
+    template <int N>
+    struct Weno
+    {
+        static int run(const frsdg_ctx* c, const double* u, double* o, hipStream_t s)
+        {
+            if (c->d.dims == 2)
+            {
+                const long long nl = c->E * 8 * N;
+                hipLaunchKernelGGL((k_weno_beta2d<N>), dim3(static_cast<unsigned>((nl + 255) / 256)), dim3(256), 0, s,
+                                   c->lim, u, c->wbeta);
+                constexpr int   EPB = LimTile<N>::EPB;
+                const long long nb  = (c->E + EPB - 1) / EPB;
+                hipLaunchKernelGGL((k_weno_apply2d<N>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, c->lim, u,
+                                   static_cast<const double*>(c->wbeta), o);
+            }
+            else
+            {
+                const long long nt = c->E * 3;
+                hipLaunchKernelGGL((k_weno1d<N>), dim3(static_cast<unsigned>((nt + 63) / 64)), dim3(64), 0, s, c->lim,
+                                   u, o);
+            }
+            return hip_rc(hipGetLastError());
+        }
+    };
+
+    bool weno_only(const frsdg_ctx* c) { return c->d.apply_weno && !c->d.apply_filter; }
+
     int ensure_work(frsdg_ctx* c)
     {
         if (!c->w1) FR_TRY(hipMalloc(&c->w1, sizeof(double) * c->nd));
         if (!c->w2) FR_TRY(hipMalloc(&c->w2, sizeof(double) * c->nd));
+        if (weno_only(c) && c->d.dims == 2 && !c->wbeta)
+            FR_TRY(hipMalloc(&c->wbeta, sizeof(double) * c->E * 8 * c->n));
         return FRSDG_OK;
     }
 
+    // limit_solution: the spectral filter when active (it replaces the WENO
+    // result, filter.hpp:740-745), else the WENO limiter, else a copy
     int do_filter(frsdg_ctx* c, const double* u, double* out, hipStream_t s)
     {
-        if (!c->d.apply_filter)
+        if (c->d.apply_filter) return by_n<Filt>(c->n, c, u, out, s);
+        if (c->d.apply_weno)
         {
-            if (out != u) FR_TRY(hipMemcpyAsync(out, u, sizeof(double) * c->nd, hipMemcpyDeviceToDevice, s));
+            int rc = ensure_work(c);
+            if (rc) return rc;
+            if (out != u) return by_n<Weno>(c->n, c, u, out, s);
+            if ((rc = by_n<Weno>(c->n, c, u, c->w1, s))) return rc;
+            FR_TRY(hipMemcpyAsync(out, c->w1, sizeof(double) * c->nd, hipMemcpyDeviceToDevice, s));
             return FRSDG_OK;
         }
-        return by_n<Filt>(c->n, c, u, out, s);
+        if (out != u) FR_TRY(hipMemcpyAsync(out, u, sizeof(double) * c->nd, hipMemcpyDeviceToDevice, s));
+        return FRSDG_OK;
     }
 
     // one explicit step, stages fused with the residual kernel
@@ -491,6 +530,12 @@ namespace
                 IOStage io{c->w2, u, u, 1.0 / 3.0, 2.0 / 3.0, dt};
                 return by_n<StageFilt>(c->n, c, io, s);
             }
+            if (weno_only(c))
+            {
+                // last stage into w1 (free after stage two), then limit into u
+                if ((rc = stage(c->w2, 1.0 / 3.0, 2.0 / 3.0, dt, c->w1))) return rc;
+                return by_n<Weno>(c->n, c, c->w1, u, s);
+            }
             if ((rc = stage(c->w2, 1.0 / 3.0, 2.0 / 3.0, dt, u))) return rc;
         }
         else if (stages == 4)
@@ -499,9 +544,11 @@ namespace
             if ((rc = stage(u, 0.0, 1.0, h, c->w1))) return rc;
             if ((rc = stage(c->w1, 0.0, 1.0, h, c->w2))) return rc;
             if ((rc = stage(c->w2, 2.0 / 3.0, 1.0 / 3.0, h, c->w1))) return rc;
-            // last stage reads w1 (neighbour traces) and writes u
-            IOStage io{c->w1, nullptr, u, 0.0, 1.0, h};
+            // last stage reads w1 (neighbour traces) and writes u (w2 when the
+            // WENO limiter follows)
+            IOStage io{c->w1, nullptr, weno_only(c) ? c->w2 : u, 0.0, 1.0, h};
             if ((rc = launch_res(c, io, s))) return rc;
+            if (weno_only(c)) return by_n<Weno>(c->n, c, c->w2, u, s);
         }
         else
             return FRSDG_ERR_INVALID;
@@ -549,6 +596,11 @@ int frsdg_create(const frsdg_desc* d, frsdg_ctx** out)
         if (!(t == 0 || t == 1 || t == 2 || t == 5 || (t == 6 && d->dims == 2))) return FRSDG_ERR_INVALID;
     }
     if (d->apply_filter && (d->filt_cutoff < 0 || d->filt_cutoff >= d->p)) return FRSDG_ERR_INVALID;
+    // WENOLimiter ctor (filter.hpp:501-512) rejects gamma outside [0, 1]; 0 and 1
+    // make a linear weight vanish and the sensor divide by it, so both are
+    // rejected here as well
+    if (d->apply_weno && !(d->weno_gamma > 0.0 && d->weno_gamma < 1.0 && d->shock_theta > 0.0))
+        return FRSDG_ERR_INVALID;
     auto* c = new (std::nothrow) frsdg_ctx{};
     if (!c) return FRSDG_ERR_INVALID;
     c->d = *d;
@@ -586,6 +638,25 @@ int frsdg_create(const frsdg_desc* d, frsdg_ctx** out)
         cw.insert(cw.end(), c->tb.w.begin(), c->tb.w.end());
         if (hipMemcpy(c->d_csi, cw.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice) != hipSuccess)
             return fail(FRSDG_ERR_HIP);
+    }
+    {
+        // WENO limiter constants (WENOLimiter / ShockSensor ctors, filter.hpp:501-512, :624-630)
+        LimP& L = c->lim;
+        for (int i = 0; i < n * n; ++i) L.D[i] = c->tb.D[i];
+        for (int i = 0; i < n; ++i) L.w[i] = c->tb.w[i];
+        const double gm    = d->weno_gamma;
+        L.ideal[0]         = (1 - gm) * 0.5;
+        L.ideal[1]         = gm;
+        L.ideal[2]         = (1 - gm) * 0.5;
+        const double d_min = std::min(std::min(L.ideal[0], L.ideal[1]), L.ideal[2]);
+        L.theta            = d->shock_theta;
+        L.threshold        = d->shock_threshold;
+        L.inv_den          = 1.0 / (std::pow(std::abs(1.0 / d_min - 1.0), L.theta) + 3 - 1);
+        L.nx               = d->nx;
+        L.ny               = d->dims == 2 ? d->ny : 1;
+        L.per_x            = d->bc_type[0] == 5 && d->bc_type[1] == 5;
+        L.per_y            = d->dims == 2 && d->bc_type[2] == 5 && d->bc_type[3] == 5;
+        L.E                = static_cast<long long>(L.nx) * L.ny;
     }
     if (d->dims == 1)
     {
@@ -650,7 +721,7 @@ int frsdg_destroy(frsdg_ctx* c)
     free_solver(c);
     if (c->gexec) (void) hipGraphExecDestroy(c->gexec);
     if (c->graph) (void) hipGraphDestroy(c->graph);
-    for (double* p : {c->d_tab, c->d_vmet, c->d_fxm, c->d_fym, c->d_csi, c->w1, c->w2, c->red, c->wb})
+    for (double* p : {c->d_tab, c->d_vmet, c->d_fxm, c->d_fym, c->d_csi, c->w1, c->w2, c->red, c->wb, c->wbeta})
         if (p) (void) hipFree(p);
     delete c;
     return FRSDG_OK;
@@ -761,6 +832,7 @@ int frsdg_filter_transpose(frsdg_ctx* c, const double* w, double* out, void* str
     if (!c || !w || !out) return FRSDG_ERR_INVALID;
     if (!aligned(w) || !aligned(out)) return FRSDG_ERR_ALIGN;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (weno_only(c)) return FRSDG_ERR_UNSUPPORTED; // the limiter's branches have no transpose here
     if (!c->d.apply_filter)
     {
         if (out != w) FR_TRY(hipMemcpyAsync(out, w, sizeof(double) * c->nd, hipMemcpyDeviceToDevice, s));
@@ -797,7 +869,7 @@ int frsdg_boundary_weights(frsdg_ctx* c, int dir, double* out, void* stream)
 int frsdg_apply_stabilization(frsdg_ctx* c, double* u, void* stream)
 {
     if (!c || !u) return FRSDG_ERR_INVALID;
-    if (!c->d.apply_filter) return FRSDG_OK;
+    if (!c->d.apply_filter && !c->d.apply_weno) return FRSDG_OK; // filters_active() (filter.hpp:710)
     return frsdg_filter(c, u, u, stream);
 }
 

@@ -3,6 +3,7 @@
 #pragma once
 #include "../../include/frsdg.h"
 #include "frsdg_kernels.hpp"
+#include "frsdg_limiter.hpp"
 #include "frsdg_tables.hpp"
 
 #include <hip/hip_runtime.h>
@@ -32,6 +33,9 @@ struct frsdg_ctx
     double* w2  = nullptr;
     double* red = nullptr; // reduction partials
     double* wb  = nullptr; // adjoint lifted face cotangents [E][4][n][4]
+    double* wbeta = nullptr; // WENO smoothness of every element line [E][2][n][4]
+    // WENO limiter constants (apply_weno without the spectral filter)
+    frsdg::LimP lim{};
     // pre_jv_eval snapshot
     const double* jv_u     = nullptr;
     double        jv_sigma = 0.0;

@@ -69,6 +69,13 @@ typedef struct frsdg_desc
     int    tangent_bc_mode; /* 0 exact linearisation of lambda BCs; 1 reference-compatible:
                                lambda-bound BCs frozen as complex step sees them
                                (boundary_conditions.hpp:270-294) */
+    /* WENO limiter with shock sensor in limit_solution (filter.hpp:486-755;
+     * parser.hpp:53-62, 277-297): -apply_weno, -weno_gamma (0 < gamma < 1),
+     * -shock_sens_theta (> 0), -shock_sens_activation_value.  With the
+     * spectral filter also on, the filter's output replaces the limiter's, as
+     * in the reference. */
+    int    apply_weno;
+    double weno_gamma, shock_theta, shock_threshold;
 } frsdg_desc;
 
 /* PhysicsEvaluator ctor (physics_evaluator.hpp:188-201) + the once-only

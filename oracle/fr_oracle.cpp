@@ -567,40 +567,59 @@ namespace fro
             }
         }
 
-        // physics_impl_2D.hpp:76-124 limit_solution, spectral filter only
-        // (StabilizationTechniques::apply_element_stabilization filter.hpp:714-755)
+        // physics_impl_2D.hpp:76-124 limit_solution: x-lines (rows) of the
+        // element and its left/right neighbours -> scratch, y-lines (columns)
+        // with the bottom/top neighbours -> out, out = (scratch + out) / 2.
+        // Beyond a non-periodic boundary the neighbour is the ghost layer of
+        // the DM_BOUNDARY_GHOSTED local vector (petsc_solver_impl.hpp:502-512,
+        // vec_dof_view.hpp:52-60), which DMGlobalToLocal never writes: zeros.
         void filter(const double* sol, double* fil) const
         {
-            const long nd = ndofs_elem;
+            const long       nd = ndofs_elem;
+            const Stabilizer st(tb, d.apply_filter != 0, d.apply_weno != 0, d.weno_gamma, d.shock_theta,
+                                d.shock_threshold);
+            std::vector<double> zero(nd, 0.0);
+            auto nb = [&](long i, long j) -> const double*
+            {
+                if ((i < 0 || i >= d.nx) && !per_x) return zero.data();
+                if ((j < 0 || j >= d.ny) && !per_y) return zero.data();
+                return elem(sol, i, j);
+            };
+            double a[9], b[9], c[9], o[9];
             for (long j = 0; j < d.ny; ++j)
                 for (long i = 0; i < d.nx; ++i)
                 {
-                    const double*       uC  = elem(sol, i, j);
-                    double*             out = elem(fil, i, j);
+                    const double* uL = nb(i - 1, j);
+                    const double* uC = elem(sol, i, j);
+                    const double* uR = nb(i + 1, j);
+                    const double* uB = nb(i, j - 1);
+                    const double* uT = nb(i, j + 1);
+                    double*       out = elem(fil, i, j);
                     std::vector<double> scratch(nd);
-                    const auto&         Fm = tb.filt;
                     for (int eq = 0; eq < 4; ++eq)
                         for (int l = 0; l < n; ++l)
-                            for (int r = 0; r < n; ++r)
+                        {
+                            for (int k = 0; k < n; ++k)
                             {
-                                double s = 0.0;
-                                if (d.apply_filter)
-                                    for (int c = 0; c < n; ++c) s += Fm[r * n + c] * uC[idx(eq, c, l)];
-                                else
-                                    s = uC[idx(eq, r, l)];
-                                scratch[idx(eq, r, l)] = s;
+                                a[k] = uL[idx(eq, k, l)];
+                                b[k] = uC[idx(eq, k, l)];
+                                c[k] = uR[idx(eq, k, l)];
                             }
+                            st.apply(a, b, c, o);
+                            for (int k = 0; k < n; ++k) scratch[idx(eq, k, l)] = o[k];
+                        }
                     for (int eq = 0; eq < 4; ++eq)
                         for (int k = 0; k < n; ++k)
-                            for (int r = 0; r < n; ++r)
+                        {
+                            for (int l = 0; l < n; ++l)
                             {
-                                double s = 0.0;
-                                if (d.apply_filter)
-                                    for (int c = 0; c < n; ++c) s += Fm[r * n + c] * uC[idx(eq, k, c)];
-                                else
-                                    s = uC[idx(eq, k, r)];
-                                out[idx(eq, k, r)] = s;
+                                a[l] = uB[idx(eq, k, l)];
+                                b[l] = uC[idx(eq, k, l)];
+                                c[l] = uT[idx(eq, k, l)];
                             }
+                            st.apply(a, b, c, o);
+                            for (int l = 0; l < n; ++l) out[idx(eq, k, l)] = o[l];
+                        }
                     for (long q = 0; q < nd; ++q) out[q] = 0.5 * (scratch[q] + out[q]);
                 }
         }
@@ -789,21 +808,24 @@ namespace fro
             }
             for (long e = first; e < last; ++e) local(sp(e), node(sp(e - 1), n - 1), node(sp(e + 1), 0), f + e * nd);
         }
-        // physics_impl_1D.hpp:96-121 limit_solution (spectral filter only)
+        // physics_impl_1D.hpp:96-121 limit_solution (neighbours beyond a
+        // non-periodic end: the zero ghost layer, as in 2D)
         void filter(const double* sol, double* fil) const
         {
-            const long nd = 3L * n;
+            const long       nd = 3L * n;
+            const Stabilizer st(tb, d.apply_filter != 0, d.apply_weno != 0, d.weno_gamma, d.shock_theta,
+                                d.shock_threshold);
+            const bool          per = periodic;
+            std::vector<double> zero(nd, 0.0);
+            auto nb = [&](long e) -> const double*
+            {
+                if (e < 0) return per ? sol + (d.nx - 1) * nd : zero.data();
+                if (e >= d.nx) return per ? sol : zero.data();
+                return sol + e * nd;
+            };
             for (long e = 0; e < d.nx; ++e)
                 for (int eq = 0; eq < 3; ++eq)
-                    for (int r = 0; r < n; ++r)
-                    {
-                        double s = 0.0;
-                        if (d.apply_filter)
-                            for (int c = 0; c < n; ++c) s += tb.filt[r * n + c] * sol[e * nd + eq * n + c];
-                        else
-                            s = sol[e * nd + eq * n + r];
-                        fil[e * nd + eq * n + r] = s;
-                    }
+                    st.apply(nb(e - 1) + eq * n, sol + e * nd + eq * n, nb(e + 1) + eq * n, fil + e * nd + eq * n);
         }
     };
 
@@ -952,7 +974,7 @@ int fro_ssprk_step(void* hh, double t, double dt, double* u, int stages)
     }
     else
         return 1;
-    if (h->d.apply_filter)
+    if (h->d.apply_filter || h->d.apply_weno) // filters_active() (filter.hpp:710)
     {
         std::vector<double> tmp(u, u + N);
         h->filter(tmp.data(), u);

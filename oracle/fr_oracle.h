@@ -31,6 +31,10 @@ typedef struct fro_desc
     int    apply_filter;
     int    filt_cutoff, filt_strength;
     double filt_alpha;
+    int    apply_weno;       /* -apply_weno */
+    double weno_gamma;       /* -weno_gamma (linear weight of the own element) */
+    double shock_theta;      /* -shock_sens_theta */
+    double shock_threshold;  /* -shock_sens_activation_value */
 } fro_desc;
 
 void* fro_create(const fro_desc* d);
@@ -46,7 +50,7 @@ int fro_residual_cplx(void* h, double t, const double* ure, const double* uim, d
 /* ComplexStepJacobianStrategy::matmult: out = sigma*v + Im R(u + i h v)/h */
 int fro_jvp_cstep(void* h, double t, const double* u, const double* v, double sigma,
                   double hstep, double* out);
-/* limit_solution (spectral filter only) */
+/* limit_solution: spectral filter and/or WENO limiter with shock sensor */
 int fro_filter(void* h, const double* u, double* out);
 /* one explicit step (stages 3: Shu-Osher SSPRK(3,3); 4: SSPRK(4,3)) + post-step filter */
 int fro_ssprk_step(void* h, double t, double dt, double* u, int stages);

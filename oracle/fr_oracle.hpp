@@ -593,4 +593,117 @@ namespace fro
             }
         return t;
     }
+
+    // ------------------------------------------------------------------
+    // Element stabilisation of one line (filter.hpp:486-755): WENOLimiter,
+    // ShockSensor and StabilizationTechniques::apply_element_stabilization.
+    // Every sum runs left to right from 0.0, as the reference's
+    // transform_reduce / reduce over these (input-category or < 4 element)
+    // ranges do, and nothing is fused (the reference builds without -march,
+    // so x86 code has no FMA): the smoothness indicator raises D to the
+    // (n-1)-th power and amplifies rounding, so the order is part of the
+    // algorithm.
+    // ------------------------------------------------------------------
+    struct Stabilizer
+    {
+        int                 n{0};
+        bool                spec{false}, weno{false};
+        std::vector<double> D, w, F;
+        double              ideal[3]{0, 0, 0};
+        double              theta{1.0}, threshold{0.8}, inv_den{0.0};
+        static constexpr double eps = 1.0e-6; // filter.hpp:495, :616
+
+        Stabilizer() = default;
+        Stabilizer(const GLLTables& tb, bool spec_, bool weno_, double gamma, double th, double thr)
+            : n(tb.n), spec(spec_), weno(weno_), D(tb.D), w(tb.w), F(tb.filt), theta(th), threshold(thr)
+        {
+            // WENOLimiter ctor :501-512, ShockSensor ctor :624-630
+            ideal[0]           = (1 - gamma) * 0.5;
+            ideal[1]           = gamma;
+            ideal[2]           = (1 - gamma) * 0.5;
+            const double d_min = std::min(std::min(ideal[0], ideal[1]), ideal[2]);
+            inv_den            = 1.0 / (std::pow(std::abs(1.0 / d_min - 1.0), theta) + 3 - 1);
+        }
+        // compute_cell_average :567-579
+        double cell_average(const double* p) const
+        {
+            double s = 0.0;
+            for (int i = 0; i < n; ++i) s = s + w[i] * p[i];
+            return 0.5 * s;
+        }
+        // compute_integral_sq :611-625
+        double integral_sq(const double* v) const
+        {
+            double s = 0.0;
+            for (int i = 0; i < n; ++i) s = s + w[i] * v[i] * v[i];
+            return 0.5 * s;
+        }
+        // compute_smoothness :581-609: sum_s 4^s * integral_sq(D^s p), s = 1..n-1
+        double smoothness(const double* p) const
+        {
+            double cur[9], nxt[9];
+            for (int i = 0; i < n; ++i) cur[i] = p[i];
+            double beta = 0.0, factor = 4.0;
+            for (int s = 1; s <= n - 1; ++s)
+            {
+                for (int r = 0; r < n; ++r)
+                {
+                    double a = 0.0;
+                    for (int c = 0; c < n; ++c) a = a + D[r * n + c] * cur[c];
+                    nxt[r] = a;
+                }
+                for (int i = 0; i < n; ++i) cur[i] = nxt[i];
+                beta = beta + factor * integral_sq(cur);
+                factor *= 4.0;
+            }
+            return beta;
+        }
+        // ShockSensor::detect_shock :643-676 -> shock flag, normalised weights
+        bool detect_shock(const double beta[3], double om[3]) const
+        {
+            for (int i = 0; i < 3; ++i) om[i] = ideal[i] / std::pow(eps + beta[i], 2);
+            const double w_sum = ((0.0 + om[0]) + om[1]) + om[2];
+            for (int i = 0; i < 3; ++i) om[i] /= w_sum;
+            double num = 0.0;
+            for (int i = 0; i < 3; ++i) num = num + std::pow(std::abs(om[i] / ideal[i] - 1.0), theta);
+            const double r = num * inv_den;
+            return std::abs(r) >= threshold;
+        }
+        // apply_element_stabilization :714-755 on contiguous lines
+        void apply(const double* prev, const double* curr, const double* next, double* out) const
+        {
+            if (weno)
+            {
+                const double beta[3] = {smoothness(prev), smoothness(curr), smoothness(next)};
+                double       om[3];
+                if (detect_shock(beta, om))
+                {
+                    // apply_weno_scalar :518-545 (overwritten by the filter when both are on)
+                    const double ap = cell_average(prev), ac = cell_average(curr), an = cell_average(next);
+                    for (int i = 0; i < n; ++i)
+                    {
+                        const double tp = prev[i] - ap + ac;
+                        const double tn = next[i] - an + ac;
+                        out[i]          = om[0] * tp + om[1] * curr[i] + om[2] * tn;
+                    }
+                    if (spec) filter(curr, out);
+                    return;
+                }
+            }
+            if (spec)
+                filter(curr, out);
+            else
+                for (int i = 0; i < n; ++i) out[i] = curr[i];
+        }
+        // SpectralFilter::apply_filter :425-445
+        void filter(const double* in, double* out) const
+        {
+            for (int r = 0; r < n; ++r)
+            {
+                double a = 0.0;
+                for (int c = 0; c < n; ++c) a = a + F[r * n + c] * in[c];
+                out[r] = a;
+            }
+        }
+    };
 } // namespace fro

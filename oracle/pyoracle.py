@@ -26,6 +26,8 @@ class FroDesc(ctypes.Structure):
         ("c_vcjh", ctypes.c_double),
         ("apply_filter", ctypes.c_int), ("filt_cutoff", ctypes.c_int),
         ("filt_strength", ctypes.c_int), ("filt_alpha", ctypes.c_double),
+        ("apply_weno", ctypes.c_int), ("weno_gamma", ctypes.c_double),
+        ("shock_theta", ctypes.c_double), ("shock_threshold", ctypes.c_double),
     ]
 
 
@@ -101,6 +103,10 @@ def make_desc(cfg):
     d.filt_cutoff = cfg.get("filt_cutoff", 2)
     d.filt_strength = cfg.get("filt_strength", 32)
     d.filt_alpha = cfg.get("filt_alpha", 36.0)
+    d.apply_weno = cfg.get("apply_weno", 0)
+    d.weno_gamma = cfg.get("weno_gamma", 0.998)
+    d.shock_theta = cfg.get("shock_theta", 1.0)
+    d.shock_threshold = cfg.get("shock_threshold", 0.8)
     return d
 
 
