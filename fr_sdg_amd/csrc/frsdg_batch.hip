@@ -426,8 +426,9 @@ int frsdg_batch_create(const frsdg_desc* descs, int count, frsdg_batch** out)
     b->off.push_back(0);
     for (int k = 0; k < count; ++k)
     {
-        // 2D only; the WENO limiter's neighbour sweep is not part of the batched stage
-        if (descs[k].dims != 2 || (descs[k].apply_weno && !descs[k].apply_filter)) return fail(FRSDG_ERR_UNSUPPORTED);
+        // 2D explicit stages only: no WENO limiter sweep, no implicit change of variables
+        if (descs[k].dims != 2 || (descs[k].apply_weno && !descs[k].apply_filter) || descs[k].cond_sys)
+            return fail(FRSDG_ERR_UNSUPPORTED);
         frsdg_ctx* c = nullptr;
         int rc = frsdg_create(&descs[k], &c);
         if (rc) return fail(rc);

@@ -484,6 +484,8 @@ namespace
 
     bool weno_only(const frsdg_ctx* c) { return c->d.apply_weno && !c->d.apply_filter; }
 
+    unsigned node_grid(const frsdg_ctx* c) { return static_cast<unsigned>((c->nm.nodes + 255) / 256); }
+
     int ensure_work(frsdg_ctx* c)
     {
         if (!c->w1) FR_TRY(hipMalloc(&c->w1, sizeof(double) * c->nd));
@@ -658,6 +660,9 @@ int frsdg_create(const frsdg_desc* d, frsdg_ctx** out)
         L.per_y            = d->dims == 2 && d->bc_type[2] == 5 && d->bc_type[3] == 5;
         L.E                = static_cast<long long>(L.nx) * L.ny;
     }
+    c->nm.dims  = d->dims;
+    c->nm.n     = n;
+    c->nm.nodes = d->dims == 2 ? static_cast<long long>(d->nx) * d->ny * n * n : static_cast<long long>(d->nx) * n;
     if (d->dims == 1)
     {
         if (d->nx < 2 && !(d->bc_type[0] == 5 && d->bc_type[1] == 5)) return fail(FRSDG_ERR_INVALID);
@@ -763,8 +768,18 @@ int frsdg_ifunction(frsdg_ctx* c, double, const double* u, const double* udot, d
 {
     if (!c || !u || !udot || !F) return FRSDG_ERR_INVALID;
     if (!aligned(u) || !aligned(udot) || !aligned(F)) return FRSDG_ERR_ALIGN;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (c->d.cond_sys)
+    {
+        // fun_implicit with the change of variables: F = U_t + R(C(W))
+        int rc = ensure_work(c);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_p2c, dim3(node_grid(c)), dim3(256), 0, s, c->nm, u, c->w1);
+        FR_TRY(hipGetLastError());
+        u = c->w1;
+    }
     IORes io{u, F, udot, 1.0};
-    return launch_res(c, io, static_cast<hipStream_t>(stream));
+    return launch_res(c, io, s);
 }
 
 int frsdg_jvp(frsdg_ctx* c, double, const double* u, const double* v, double sigma, double* out,
@@ -772,8 +787,19 @@ int frsdg_jvp(frsdg_ctx* c, double, const double* u, const double* v, double sig
 {
     if (!c || !u || !v || !out) return FRSDG_ERR_INVALID;
     if (!aligned(u) || !aligned(v) || !aligned(out)) return FRSDG_ERR_ALIGN;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (c->d.cond_sys)
+    {
+        // (sigma I + J_R(C(W))) dC/dW v: the map's tangent, then the residual's
+        int rc = ensure_work(c);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_p2c_tan, dim3(node_grid(c)), dim3(256), 0, s, c->nm, u, v, c->w1, c->w2);
+        FR_TRY(hipGetLastError());
+        u = c->w1;
+        v = c->w2;
+    }
     IOTan io{u, v, out, sigma};
-    return launch_res(c, io, static_cast<hipStream_t>(stream));
+    return launch_res(c, io, s);
 }
 
 int frsdg_jvp_cstep(frsdg_ctx* c, double, const double* u, const double* v, double sigma, double h,
@@ -782,6 +808,18 @@ int frsdg_jvp_cstep(frsdg_ctx* c, double, const double* u, const double* v, doub
     if (!c || !u || !v || !out || !(h > 0.0)) return FRSDG_ERR_INVALID;
     if (!aligned(u) || !aligned(v) || !aligned(out)) return FRSDG_ERR_ALIGN;
     // the complex residual sees lambda-bound BCs frozen (boundary_conditions.hpp:270-294)
+    if (c->d.cond_sys)
+    {
+        // Q = C(W + i h V) in complex arithmetic (petsc_solver_impl.hpp:297-326),
+        // handed over as (Re Q, Im Q / h)
+        int rc = ensure_work(c);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_p2c_cplx, dim3(node_grid(c)), dim3(256), 0, static_cast<hipStream_t>(stream), c->nm, u,
+                           v, h, 1.0 / h, c->w1, c->w2);
+        FR_TRY(hipGetLastError());
+        u = c->w1;
+        v = c->w2;
+    }
     const int keep      = c->p2.bc.freeze;
     c->p2.bc.freeze     = 1;
     c->p1.bc.freeze     = 1;
@@ -799,8 +837,21 @@ int frsdg_vjp(frsdg_ctx* c, double, const double* u, const double* w, double sig
     if (!aligned(u) || !aligned(w) || !aligned(out)) return FRSDG_ERR_ALIGN;
     if (out == u || out == w) return FRSDG_ERR_INVALID; // neighbours' entries are still being read
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (c->d.dims == 2 && !c->wb) FR_TRY(hipMalloc(&c->wb, sizeof(double) * 16 * c->n * c->E));
+    if (c->d.cond_sys)
+    {
+        // (dC/dW)^T (sigma I + J_R(C(W))^T) w
+        int rc = ensure_work(c);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_p2c, dim3(node_grid(c)), dim3(256), 0, s, c->nm, u, c->w1);
+        FR_TRY(hipGetLastError());
+        rc = c->d.dims == 1 ? by_n<Adj1>(c->n, c, c->w1, w, sigma, c->w2, s)
+                            : by_n<Adj2>(c->n, c, c->w1, w, sigma, c->w2, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_p2c_vjp, dim3(node_grid(c)), dim3(256), 0, s, c->nm, u, c->w2, out);
+        return hip_rc(hipGetLastError());
+    }
     if (c->d.dims == 1) return by_n<Adj1>(c->n, c, u, w, sigma, out, s);
-    if (!c->wb) FR_TRY(hipMalloc(&c->wb, sizeof(double) * 16 * c->n * c->E));
     return by_n<Adj2>(c->n, c, u, w, sigma, out, s);
 }
 
@@ -945,6 +996,7 @@ int frsdg_impose_ic(frsdg_ctx* c, int kind, const double* prm, double* u, void* 
         const long long tot = static_cast<long long>(c->d.nx) * n;
         hipLaunchKernelGGL(k_ic1d, dim3(static_cast<unsigned>((tot + 255) / 256)), dim3(256), 0, s, c->p1.gas, ic,
                            c->d.nx, n, c->d.lx, c->d_csi, u);
+        if (c->d.cond_sys) hipLaunchKernelGGL(k_c2p, dim3(node_grid(c)), dim3(256), 0, s, c->nm, u, u);
         return hip_rc(hipGetLastError());
     }
     if (kind == 3) return FRSDG_ERR_INVALID;
@@ -952,6 +1004,8 @@ int frsdg_impose_ic(frsdg_ctx* c, int kind, const double* prm, double* u, void* 
     hipLaunchKernelGGL(k_ic2d, dim3(static_cast<unsigned>((tot + 255) / 256)), dim3(256), 0, s, geom_of(c->d),
                        c->p2.gas, ic, c->d.geometry == 0 ? 1 : 0, c->d.nx, c->d.ny, n, c->d.lx, c->d.ly,
                        c->d_csi, c->E, u);
+    // impose_ic stores cons_to_prim of the state with cond_sys (physics_impl_2D.hpp:180-190)
+    if (c->d.cond_sys) hipLaunchKernelGGL(k_c2p, dim3(node_grid(c)), dim3(256), 0, s, c->nm, u, u);
     return hip_rc(hipGetLastError());
 }
 
@@ -968,6 +1022,13 @@ int frsdg_l2_error(frsdg_ctx* c, int kind, const double* prm, double t, const do
     const long long tot = c->E * n * n;
     const long long nb  = (tot + 255) / 256;
     if (!c->red) FR_TRY(hipMalloc(&c->red, sizeof(double) * 4 * (nb + 1)));
+    if (c->d.cond_sys) // compute_error converts the stored primitives (physics_impl_2D.hpp:229-240)
+    {
+        int rc = ensure_work(c);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_p2c, dim3(node_grid(c)), dim3(256), 0, s, c->nm, u, c->w1);
+        u = c->w1;
+    }
     hipLaunchKernelGGL(k_l2_part, dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, geom_of(c->d), c->p2.gas, ic,
                        c->d.nx, c->d.ny, n, c->d_csi, c->d_csi + n, c->E, u, c->red + 4);
     hipLaunchKernelGGL(k_sum_blocks, dim3(1), dim3(256), 0, s, nb, c->red + 4, c->red);

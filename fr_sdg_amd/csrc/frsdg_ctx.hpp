@@ -4,6 +4,7 @@
 #include "../../include/frsdg.h"
 #include "frsdg_kernels.hpp"
 #include "frsdg_limiter.hpp"
+#include "frsdg_transform.hpp"
 #include "frsdg_tables.hpp"
 
 #include <hip/hip_runtime.h>
@@ -36,6 +37,8 @@ struct frsdg_ctx
     double* wbeta = nullptr; // WENO smoothness of every element line [E][2][n][4]
     // WENO limiter constants (apply_weno without the spectral filter)
     frsdg::LimP lim{};
+    // node addressing of the log-variable change of variables (cond_sys)
+    frsdg::NodeMap nm{};
     // pre_jv_eval snapshot
     const double* jv_u     = nullptr;
     double        jv_sigma = 0.0;

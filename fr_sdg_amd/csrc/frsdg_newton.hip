@@ -513,12 +513,28 @@ extern "C" int frsdg_beuler_step(frsdg_ctx* c, double t, double dt, double* u, c
     const double sigma = 1.0 / dt;
     Ctx K{c, s, st, o, t + dt, sigma, u, S};
     auto g = K.grid(N);
-    FR_TRY(hipMemcpyAsync(s->un, u, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
-    // F(x) = (x - u^n)/dt + R(x)
+    // F(x) = (x - u^n)/dt + R(x); with cond_sys the transient variable is
+    // C(x) (TSSetTransientVariable): F(x) = (C(x) - C(u^n))/dt + R(C(x))
+    const bool cs = c->d.cond_sys != 0;
+    const unsigned gn = static_cast<unsigned>((c->nm.nodes + 255) / 256);
+    if (cs)
+    {
+        if (!c->w1) FR_TRY(hipMalloc(&c->w1, sizeof(double) * c->nd));
+        if (!c->w2) FR_TRY(hipMalloc(&c->w2, sizeof(double) * c->nd));
+        hipLaunchKernelGGL(k_p2c, dim3(gn), dim3(256), 0, st, c->nm, u, s->un);
+    }
+    else
+        FR_TRY(hipMemcpyAsync(s->un, u, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
     auto eval_F = [&](const double* x, double* F) -> int
     {
         S->res_evals++;
-        hipLaunchKernelGGL(k_waxpby, dim3(g), dim3(256), 0, st, N, sigma, x, -sigma, s->un, s->ud);
+        if (cs)
+        {
+            hipLaunchKernelGGL(k_p2c, dim3(gn), dim3(256), 0, st, c->nm, x, c->w2);
+            hipLaunchKernelGGL(k_waxpby, dim3(g), dim3(256), 0, st, N, sigma, c->w2, -sigma, s->un, s->ud);
+        }
+        else
+            hipLaunchKernelGGL(k_waxpby, dim3(g), dim3(256), 0, st, N, sigma, x, -sigma, s->un, s->ud);
         return frsdg_ifunction(c, t + dt, x, s->ud, F, st);
     };
     if ((rc = eval_F(u, s->F))) return rc;

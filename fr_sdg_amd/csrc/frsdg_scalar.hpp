@@ -135,4 +135,16 @@ namespace frsdg
     FR_HD double sqrt_(double x) { return ::sqrt(x); }
     FR_HD Dual sqrt_(Dual x) { return sqrt(x); }
     FR_HD Cplx sqrt_(Cplx x) { return sqrt(x); }
+    // exponential (change of variables of the log-variable implicit path)
+    FR_HD double exp_(double x) { return ::exp(x); }
+    FR_HD Dual exp_(Dual x)
+    {
+        const double e = ::exp(x.v);
+        return {e, x.d * e};
+    }
+    FR_HD Cplx exp_(Cplx z)
+    {
+        const double m = ::exp(z.re);
+        return {m * ::cos(z.im), m * ::sin(z.im)};
+    }
 } // namespace frsdg

@@ -76,6 +76,14 @@ typedef struct frsdg_desc
      * in the reference. */
     int    apply_weno;
     double weno_gamma, shock_theta, shock_threshold;
+    /* -cond_sys: the implicit path's state holds (log rho, u, [v,] log e) per
+     * node (TSSetTransientVariable, petsc_solver_impl.hpp:572-575, 626-655):
+     * frsdg_ifunction, the J*v products, frsdg_vjp, frsdg_beuler_step,
+     * frsdg_impose_ic and frsdg_l2_error then work through the change of
+     * variables C(W) (problem_def.hpp:36-70).  Explicit stepping and
+     * frsdg_residual/frsdg_rhs see the stored vector as conservative, as the
+     * reference's rhs_func does. */
+    int    cond_sys;
 } frsdg_desc;
 
 /* PhysicsEvaluator ctor (physics_evaluator.hpp:188-201) + the once-only

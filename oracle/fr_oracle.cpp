@@ -919,6 +919,102 @@ int fro_residual_cplx(void* hh, double t, const double* ure, const double* uim, 
 }
 
 // petsc_solver_impl.hpp:270-359 matmult_2d_impl / :361-432 matmult_1d_impl
+} // extern "C"
+
+// prim_to_cons_elem (problem_def.hpp:36-50) over every node; W = (log rho, u, [v,]
+// log e) per node in the state layout
+template <class U>
+static void prim_to_cons_all(const Handle& h, const U* w, U* c)
+{
+    const int n = h.tb().n;
+    if (h.s2)
+    {
+        const long nodes = h.ndofs() / 4;
+        for (long i = 0; i < nodes; ++i)
+        {
+            const U* a = w + 4 * i;
+            U*       b = c + 4 * i;
+            const U  rho = std::exp(a[0]);
+            const U  kin = U(0.5) * ((U(0.0) + a[1] * a[1]) + a[2] * a[2]);
+            const U  E   = rho * (std::exp(a[3]) + kin);
+            b[0] = rho;
+            b[1] = rho * a[1];
+            b[2] = rho * a[2];
+            b[3] = E;
+        }
+        return;
+    }
+    const long ne = h.ndofs() / (3L * n);
+    for (long e = 0; e < ne; ++e)
+        for (int m = 0; m < n; ++m)
+        {
+            const long i0 = (e * 3 + 0) * n + m, i1 = (e * 3 + 1) * n + m, i2 = (e * 3 + 2) * n + m;
+            const U    rho = std::exp(w[i0]);
+            const U    kin = U(0.5) * (U(0.0) + w[i1] * w[i1]);
+            const U    E   = rho * (std::exp(w[i2]) + kin);
+            c[i0] = rho;
+            c[i1] = rho * w[i1];
+            c[i2] = E;
+        }
+}
+
+// cons_to_prim_elem (problem_def.hpp:55-70)
+static void cons_to_prim_all(const Handle& h, const double* c, double* w)
+{
+    const int n = h.tb().n;
+    if (h.s2)
+    {
+        const long nodes = h.ndofs() / 4;
+        for (long i = 0; i < nodes; ++i)
+        {
+            const double* a = c + 4 * i;
+            const double  inv = 1.0 / a[0];
+            const double  u = a[1] * inv, v = a[2] * inv;
+            const double  kin = 0.5 * ((0.0 + u * u) + v * v);
+            const double  r0 = std::log(a[0]), r3 = std::log(a[3] * inv - kin);
+            w[4 * i] = r0; w[4 * i + 1] = u; w[4 * i + 2] = v; w[4 * i + 3] = r3;
+        }
+        return;
+    }
+    const long ne = h.ndofs() / (3L * n);
+    for (long e = 0; e < ne; ++e)
+        for (int m = 0; m < n; ++m)
+        {
+            const long   i0 = (e * 3 + 0) * n + m, i1 = (e * 3 + 1) * n + m, i2 = (e * 3 + 2) * n + m;
+            const double inv = 1.0 / c[i0];
+            const double u   = c[i1] * inv;
+            const double kin = 0.5 * (0.0 + u * u);
+            const double r0 = std::log(c[i0]), r2 = std::log(c[i2] * inv - kin);
+            w[i0] = r0; w[i1] = u; w[i2] = r2;
+        }
+}
+
+extern "C" {
+
+int fro_prim_to_cons(void* hh, const double* w, double* c)
+{
+    prim_to_cons_all(*static_cast<Handle*>(hh), w, c);
+    return 0;
+}
+
+int fro_cons_to_prim(void* hh, const double* c, double* w)
+{
+    cons_to_prim_all(*static_cast<Handle*>(hh), c, w);
+    return 0;
+}
+
+// fun_implicit (petsc_solver_impl.hpp:626-655)
+int fro_ifunction(void* hh, double t, const double* u, const double* udot, double* F)
+{
+    auto*               h = static_cast<Handle*>(hh);
+    const long          N = h->ndofs();
+    std::vector<double> cu(u, u + N);
+    if (h->d.cond_sys) prim_to_cons_all(*h, u, cu.data());
+    h->residual(cu.data(), F, t);
+    for (long q = 0; q < N; ++q) F[q] = udot[q] + F[q];
+    return 0;
+}
+
 int fro_jvp_cstep(void* hh, double t, const double* u, const double* v, double sigma, double hs,
                   double* out)
 {
@@ -927,6 +1023,11 @@ int fro_jvp_cstep(void* hh, double t, const double* u, const double* v, double s
     const double      one_over_h = 1.0 / hs;
     std::vector<cplx> uiv(N), f(N);
     for (long q = 0; q < N; ++q) uiv[q] = cplx(u[q], v[q] * hs);
+    if (h->d.cond_sys) // Q = M(u + i v h): prim-to-cons in complex arithmetic (:297-326, :386-401)
+    {
+        std::vector<cplx> w(uiv);
+        prim_to_cons_all(*h, w.data(), uiv.data());
+    }
     for (long q = 0; q < N; ++q) out[q] = sigma * std::imag(uiv[q]) * one_over_h;
     h->residual(uiv.data(), f.data(), t);
     for (long q = 0; q < N; ++q) out[q] += std::imag(f[q]) * one_over_h;
@@ -982,7 +1083,7 @@ int fro_ssprk_step(void* hh, double t, double dt, double* u, int stages)
     return 0;
 }
 
-int fro_impose_ic(void* hh, int kind, const double* prm, double* u)
+static int impose_ic_cons(void* hh, int kind, const double* prm, double* u)
 {
     auto* h = static_cast<Handle*>(hh);
     if (h->s2)
@@ -1035,13 +1136,30 @@ int fro_impose_ic(void* hh, int kind, const double* prm, double* u)
     return 0;
 }
 
+// impose_ic (physics_impl_2D.hpp:156-201, physics_impl_1D.hpp:207-244): with
+// cond_sys the state is stored as (log rho, u, [v,] log e)
+int fro_impose_ic(void* hh, int kind, const double* prm, double* u)
+{
+    const int rc = impose_ic_cons(hh, kind, prm, u);
+    auto*     h  = static_cast<Handle*>(hh);
+    if (rc == 0 && h->d.cond_sys) cons_to_prim_all(*h, u, u);
+    return rc;
+}
+
 // physics_impl_2D.hpp:203-265 compute_error (exact_sol = same kinds as the ICs)
 int fro_l2_error(void* hh, int kind, const double* prm, double t, const double* u, double* out)
 {
     auto* h = static_cast<Handle*>(hh);
     if (!h->s2) return 1;
-    const auto& s = *h->s2;
-    double      errs[4]{};
+    const auto&         s = *h->s2;
+    std::vector<double> cu;
+    if (h->d.cond_sys) // compute_error converts the stored primitives (physics_impl_2D.hpp:229-240)
+    {
+        cu.resize(h->ndofs());
+        prim_to_cons_all(*h, u, cu.data());
+        u = cu.data();
+    }
+    double errs[4]{};
     for (long j = 0; j < s.d.ny; ++j)
         for (long i = 0; i < s.d.nx; ++i)
         {

@@ -35,6 +35,7 @@ typedef struct fro_desc
     double weno_gamma;       /* -weno_gamma (linear weight of the own element) */
     double shock_theta;      /* -shock_sens_theta */
     double shock_threshold;  /* -shock_sens_activation_value */
+    int    cond_sys;         /* -cond_sys: implicit path on (log rho, u, [v,] log e) */
 } fro_desc;
 
 void* fro_create(const fro_desc* d);
@@ -47,9 +48,16 @@ int fro_residual(void* h, double t, const double* u, double* R);
 /* residual_kernel<complex<double>> on (ure + i uim) */
 int fro_residual_cplx(void* h, double t, const double* ure, const double* uim, double* Rre,
                       double* Rim);
-/* ComplexStepJacobianStrategy::matmult: out = sigma*v + Im R(u + i h v)/h */
+/* ComplexStepJacobianStrategy::matmult: out = sigma*Im(q)/h + Im R(q)/h with
+ * q = u + i h v, or q = C(u + i h v) when cond_sys */
 int fro_jvp_cstep(void* h, double t, const double* u, const double* v, double sigma,
                   double hstep, double* out);
+/* fun_implicit (petsc_solver_impl.hpp:626-655): F = U_t + R(C(U)) with C the
+ * prim->cons map when cond_sys, identity otherwise */
+int fro_ifunction(void* h, double t, const double* u, const double* udot, double* F);
+/* prim_to_cons_kernel / cons_to_prim_elem node by node (problem_def.hpp:36-70) */
+int fro_prim_to_cons(void* h, const double* w, double* c);
+int fro_cons_to_prim(void* h, const double* c, double* w);
 /* limit_solution: spectral filter and/or WENO limiter with shock sensor */
 int fro_filter(void* h, const double* u, double* out);
 /* one explicit step (stages 3: Shu-Osher SSPRK(3,3); 4: SSPRK(4,3)) + post-step filter */

@@ -28,6 +28,7 @@ class FroDesc(ctypes.Structure):
         ("filt_strength", ctypes.c_int), ("filt_alpha", ctypes.c_double),
         ("apply_weno", ctypes.c_int), ("weno_gamma", ctypes.c_double),
         ("shock_theta", ctypes.c_double), ("shock_threshold", ctypes.c_double),
+        ("cond_sys", ctypes.c_int),
     ]
 
 
@@ -56,6 +57,9 @@ def lib():
         L.fro_residual_cplx.argtypes = [vp, ctypes.c_double, dp, dp, dp, dp]
         L.fro_jvp_cstep.argtypes = [vp, ctypes.c_double, dp, dp, ctypes.c_double, ctypes.c_double, dp]
         L.fro_filter.argtypes = [vp, dp, dp]
+        L.fro_ifunction.argtypes = [vp, ctypes.c_double, dp, dp, dp]
+        L.fro_prim_to_cons.argtypes = [vp, dp, dp]
+        L.fro_cons_to_prim.argtypes = [vp, dp, dp]
         L.fro_ssprk_step.argtypes = [vp, ctypes.c_double, ctypes.c_double, dp, ctypes.c_int]
         L.fro_impose_ic.argtypes = [vp, ctypes.c_int, dp, dp]
         L.fro_l2_error.argtypes = [vp, ctypes.c_int, dp, ctypes.c_double, dp, dp]
@@ -107,6 +111,7 @@ def make_desc(cfg):
     d.weno_gamma = cfg.get("weno_gamma", 0.998)
     d.shock_theta = cfg.get("shock_theta", 1.0)
     d.shock_threshold = cfg.get("shock_threshold", 0.8)
+    d.cond_sys = cfg.get("cond_sys", 0)
     return d
 
 
@@ -152,6 +157,24 @@ class Oracle:
         out = np.zeros(self.ndofs)
         lib().fro_jvp_cstep(self.h, t, _p(u), _p(v), sigma, h, _p(out))
         return out
+
+    def ifunction(self, u, udot, t=0.0):
+        u, udot = _c(u), _c(udot)
+        F = np.zeros(self.ndofs)
+        lib().fro_ifunction(self.h, t, _p(u), _p(udot), _p(F))
+        return F
+
+    def prim_to_cons(self, w):
+        w = _c(w)
+        c = np.zeros(self.ndofs)
+        lib().fro_prim_to_cons(self.h, _p(w), _p(c))
+        return c
+
+    def cons_to_prim(self, c):
+        c = _c(c)
+        w = np.zeros(self.ndofs)
+        lib().fro_cons_to_prim(self.h, _p(c), _p(w))
+        return w
 
     def filter(self, u):
         u = _c(u)
