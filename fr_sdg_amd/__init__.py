@@ -52,7 +52,7 @@ class FrsdgDesc(ctypes.Structure):
         ("jac_type", ctypes.c_int), ("tangent_bc_mode", ctypes.c_int),
         ("apply_weno", ctypes.c_int), ("weno_gamma", ctypes.c_double),
         ("shock_theta", ctypes.c_double), ("shock_threshold", ctypes.c_double),
-        ("cond_sys", ctypes.c_int),
+        ("cond_sys", ctypes.c_int), ("coord_sys", ctypes.c_int),
     ]
 
 
@@ -182,6 +182,7 @@ def make_desc(cfg):
     d.shock_theta = cfg.get("shock_theta", 1.0)
     d.shock_threshold = cfg.get("shock_threshold", 0.8)
     d.cond_sys = cfg.get("cond_sys", 0)
+    d.coord_sys = cfg.get("coord_sys", 0)
     return d
 
 

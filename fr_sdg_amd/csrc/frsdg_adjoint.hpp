@@ -278,6 +278,22 @@ namespace frsdg
         }
         double ub[4] = {0, 0, 0, 0};
         flux2_vjp(P.gas, U, Fb, Gb, ub);
+        if (P.rinv)
+        {
+            // radial source (pointwise): one forward-dual pass per state component
+            const double ir = P.rinv[e * NN + node];
+            for (int i = 0; i < 4; ++i)
+            {
+                Dual A[4], sr[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) A[q] = {U[q], q == i ? 1.0 : 0.0};
+                radial_source2(P.gas, A, ir, sr);
+                double acc = 0.0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc += sr[q].d * W[q];
+                ub[i] += acc;
+            }
+        }
         // ---- gather face contributions of this node ----
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -425,6 +441,24 @@ namespace frsdg
             flux1_adj(P.gas, last, mc, t);
 #pragma unroll
             for (int q = 0; q < 3; ++q) ub[q][N - 1] += t[q];
+        }
+        if (P.rinv && (P.periodic || (e > 0 && e < P.ne - 1))) // interior elements only, as the residual
+        {
+            for (int m = 0; m < N; ++m)
+            {
+                const double ir = P.rinv[static_cast<long long>(e) * N + m];
+                for (int i = 0; i < 3; ++i)
+                {
+                    Dual A[3], sr[3];
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) A[q] = {u[b + q * N + m], q == i ? 1.0 : 0.0};
+                    radial_source1(P.gas, A, ir, sr);
+                    double acc = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) acc += sr[q].d * w[b + q * N + m];
+                    ub[i][m] += acc;
+                }
+            }
         }
 #pragma unroll
         for (int q = 0; q < 3; ++q)

@@ -427,7 +427,8 @@ int frsdg_batch_create(const frsdg_desc* descs, int count, frsdg_batch** out)
     for (int k = 0; k < count; ++k)
     {
         // 2D explicit stages only: no WENO limiter sweep, no implicit change of variables
-        if (descs[k].dims != 2 || (descs[k].apply_weno && !descs[k].apply_filter) || descs[k].cond_sys)
+        if (descs[k].dims != 2 || (descs[k].apply_weno && !descs[k].apply_filter) || descs[k].cond_sys ||
+            descs[k].coord_sys)
             return fail(FRSDG_ERR_UNSUPPORTED);
         frsdg_ctx* c = nullptr;
         int rc = frsdg_create(&descs[k], &c);

@@ -234,6 +234,30 @@ namespace
         const double cy = (1.0 / static_cast<double>(ny)) * (static_cast<double>(j) + 0.5 * (1.0 + csi[l]));
         geom_phys(g, cx, cy, x, y);
     }
+    // 1/radius per node for the radial source terms: 2D r = y/2 at the mapped
+    // node (get_x_y_pos, physics_impl_2D.hpp:308-310), 1D r = (e + (1+csi)/2) h
+    // (get_r, physics_evaluator.hpp:148-155); 0 within 1e-8 of the axis
+    __global__ void k_rinv2d(GeomP g, int nx, int ny, int n, const double* __restrict__ csi, long long E,
+                             double* __restrict__ out)
+    {
+        const long long t  = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+        const long long NN = static_cast<long long>(n) * n;
+        if (t >= E * NN) return;
+        const long long e = t / NN;
+        const int node = static_cast<int>(t - e * NN), k = node % n, l = node / n;
+        double x, y;
+        node_xy(g, 0, nx, ny, g.Lx, g.Ly, csi, static_cast<int>(e % nx), static_cast<int>(e / nx), k, l, x, y);
+        const double r = 0.5 * y;
+        out[t]         = (r > 1e-8) ? 1.0 / r : 0.0;
+    }
+    __global__ void k_rinv1d(int ne, int n, double h, const double* __restrict__ csi, double* __restrict__ out)
+    {
+        const long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+        if (t >= static_cast<long long>(ne) * n) return;
+        const int    e = static_cast<int>(t / n), m = static_cast<int>(t % n);
+        const double r = (e + 0.5 * (csi[m] + 1.0)) * h;
+        out[t]         = (r > 1e-8) ? 1.0 / r : 0.0;
+    }
     __global__ void k_ic2d(GeomP g, GasP gas, ICP c, int cart_pos, int nx, int ny, int n, double lx,
                            double ly, const double* __restrict__ csi, long long E, double* __restrict__ u)
     {
@@ -592,6 +616,7 @@ int frsdg_create(const frsdg_desc* d, frsdg_ctx** out)
     if (d->p < 1 || d->p > 8 || d->nx < 1 || (d->dims == 2 && d->ny < 1)) return FRSDG_ERR_INVALID;
     if (d->gas_kind < 0 || d->gas_kind > 1 || d->riemann < 0 || d->riemann > 1) return FRSDG_ERR_INVALID;
     if (d->geometry < 0 || d->geometry > 4 || (d->dims == 1 && d->geometry != 0)) return FRSDG_ERR_INVALID;
+    if (d->coord_sys < 0 || d->coord_sys > 1) return FRSDG_ERR_INVALID;
     for (int k = 0; k < (d->dims == 2 ? 4 : 2); ++k)
     {
         const int t = d->bc_type[k];
@@ -669,7 +694,16 @@ int frsdg_create(const frsdg_desc* d, frsdg_ctx** out)
         c->E  = d->nx;
         c->nd = 3LL * n * d->nx;
         c->p1 = Res1DP{d->nx, d->bc_type[0] == 5 && d->bc_type[1] == 5, d->riemann,
-                       2.0 / (d->lx / d->nx), c->d_tab, gas, bc};
+                       2.0 / (d->lx / d->nx), c->d_tab, gas, bc, nullptr};
+        if (d->coord_sys == 1)
+        {
+            const long long tot = static_cast<long long>(d->nx) * n;
+            if (hipMalloc(&c->d_rinv, sizeof(double) * tot) != hipSuccess) return fail(FRSDG_ERR_HIP);
+            hipLaunchKernelGGL(k_rinv1d, dim3(static_cast<unsigned>((tot + 255) / 256)), dim3(256), 0, nullptr, d->nx,
+                               n, d->lx / d->nx, c->d_csi, c->d_rinv);
+            if (hipDeviceSynchronize() != hipSuccess) return fail(FRSDG_ERR_HIP);
+            c->p1.rinv = c->d_rinv;
+        }
         *out  = c;
         return FRSDG_OK;
     }
@@ -715,6 +749,15 @@ int frsdg_create(const frsdg_desc* d, frsdg_ctx** out)
         P.fxm  = c->d_fxm;
         P.fym  = c->d_fym;
     }
+    if (d->coord_sys == 1)
+    {
+        const long long tot = c->E * n * n;
+        if (hipMalloc(&c->d_rinv, sizeof(double) * tot) != hipSuccess) return fail(FRSDG_ERR_HIP);
+        hipLaunchKernelGGL(k_rinv2d, dim3(static_cast<unsigned>((tot + 255) / 256)), dim3(256), 0, nullptr, g, d->nx,
+                           d->ny, n, c->d_csi, c->E, c->d_rinv);
+        if (hipDeviceSynchronize() != hipSuccess) return fail(FRSDG_ERR_HIP);
+        P.rinv = c->d_rinv;
+    }
     (void) rc;
     *out = c;
     return FRSDG_OK;
@@ -726,7 +769,8 @@ int frsdg_destroy(frsdg_ctx* c)
     free_solver(c);
     if (c->gexec) (void) hipGraphExecDestroy(c->gexec);
     if (c->graph) (void) hipGraphDestroy(c->graph);
-    for (double* p : {c->d_tab, c->d_vmet, c->d_fxm, c->d_fym, c->d_csi, c->w1, c->w2, c->red, c->wb, c->wbeta})
+    for (double* p : {c->d_tab, c->d_vmet, c->d_fxm, c->d_fym, c->d_csi, c->d_rinv, c->w1, c->w2, c->red, c->wb,
+                      c->wbeta})
         if (p) (void) hipFree(p);
     delete c;
     return FRSDG_OK;

@@ -27,6 +27,7 @@ struct frsdg_ctx
     double*    d_fxm  = nullptr;
     double*    d_fym  = nullptr;
     double*    d_csi  = nullptr;
+    double*    d_rinv = nullptr; // 1/radius per node (radial coordinate system)
     Res2DP     p2{};
     Res1DP     p1{};
     // lazily allocated scratch

@@ -46,6 +46,7 @@ namespace frsdg
         const double* __restrict__ tab;    // D[n*n], dL[n], dR[n], filt[n*n]
         GasP      gas;
         BCP       bc;
+        const double* __restrict__ rinv;   // [E][n*n] 1/radius of radial meshes, else nullptr
     };
 
     // blockIdx -> logical tile so that XCD x (= b % 8) processes one contiguous band
@@ -762,6 +763,13 @@ namespace frsdg
                 r[q] = r[q] + sm.Del[el][2][k][q] * sm.dL[l];
                 r[q] = r[q] + sm.Del[el][3][k][q] * sm.dR[l];
             }
+            if (P.rinv)
+            {
+                S src[4];
+                radial_source2(P.gas, own, P.rinv[e * NN + node], src);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) r[q] = r[q] + src[q];
+            }
         }
         if constexpr (FILT && IO::can_filter)
         {
@@ -832,6 +840,7 @@ namespace frsdg
         const double* __restrict__ tab;
         GasP   gas;
         BCP    bc;
+        const double* rinv; // [ne][n] 1/radius of radial meshes, else nullptr
     };
     template <int N, class IO>
     __global__ __launch_bounds__(64) void k_res1d(const Res1DP P, const IO io)
@@ -888,6 +897,19 @@ namespace frsdg
             const S dl = FsL[q] - FdL[q], dr = FsR[q] - FdR[q];
             for (int in = 0; in < N; ++in)
                 res[q][in] = res[q][in] + P.inv_jac * (dl * dL[in] + dr * dR[in]);
+        }
+        // radial source, added to the interior elements only (the reference's
+        // boundary elements are evaluated before its source loop,
+        // physics_impl_1D.hpp:29-94)
+        if (P.rinv && (P.periodic || (e > 0 && e < P.ne - 1)))
+        {
+            for (int m = 0; m < N; ++m)
+            {
+                S nd[3] = {u[0][m], u[1][m], u[2][m]}, src[3];
+                radial_source1(P.gas, nd, P.rinv[static_cast<long long>(e) * N + m], src);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) res[q][m] = res[q][m] + src[q];
+            }
         }
 #pragma unroll
         for (int q = 0; q < 3; ++q)

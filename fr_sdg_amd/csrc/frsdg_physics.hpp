@@ -81,11 +81,12 @@ namespace frsdg
         q.p          = gas_pressure(g, q.rho, q.e);
         return q;
     }
-    // euler_fluxes_2d_w_p: F, G at a node
+    // euler_fluxes_2d_w_p: F, G (and p) at a node
     template <class S>
-    FR_HD void flux2(const GasP& g, const S U[4], S F[4], S G[4])
+    FR_HD void flux2p(const GasP& g, const S U[4], S F[4], S G[4], S& p)
     {
         const Prim2<S> q  = prim2(g, U);
+        p                 = q.p;
         const S        Ep = U[3] + q.p;
         const S        uv = U[1] * q.v;
         F[0]              = U[1];
@@ -96,6 +97,35 @@ namespace frsdg
         G[1]              = uv;
         G[2]              = U[2] * q.v + q.p;
         G[3]              = q.v * Ep;
+    }
+    template <class S>
+    FR_HD void flux2(const GasP& g, const S U[4], S F[4], S G[4])
+    {
+        S p;
+        flux2p(g, U, F, G, p);
+    }
+
+    // radial (axisymmetric) source of a 2D node, physics_impl_2D.hpp:306-319:
+    // G / r for every equation, then -p / r on the radial momentum (ir = 1/r,
+    // 0 on the axis)
+    template <class S>
+    FR_HD void radial_source2(const GasP& g, const S U[4], double ir, S src[4])
+    {
+        S F[4], G[4], p;
+        flux2p(g, U, F, G, p);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) src[q] = G[q] * ir;
+        src[2] = src[2] - p * ir;
+    }
+    // radial source of a 1D node, physics_impl_1D.hpp:70-90 (no density floor)
+    template <class S>
+    FR_HD void radial_source1(const GasP& g, const S U[3], double ir, S src[3])
+    {
+        const S vel  = U[1] / U[0];
+        const S pres = gas_pressure(g, U[0], U[2] / U[0] - 0.5 * vel * vel);
+        src[0]       = ir * U[1];
+        src[1]       = ir * U[1] * vel;
+        src[2]       = ir * vel * (U[2] + pres);
     }
 
     // rotated 2D numerical flux; kind 0 Rusanov, 1 HLLC

@@ -84,6 +84,11 @@ typedef struct frsdg_desc
      * frsdg_residual/frsdg_rhs see the stored vector as conservative, as the
      * reference's rhs_func does. */
     int    cond_sys;
+    /* coordinate system (Traits::coord_sys, macro_and_libc.hpp:154): 0
+     * Cartesian, 1 radial — the residual gains the source terms of
+     * physics_impl_1D.hpp:70-90 (r = x; interior elements only, as there) and
+     * physics_impl_2D.hpp:306-319 (axisymmetric, r = y/2 at the mapped node) */
+    int    coord_sys;
 } frsdg_desc;
 
 /* PhysicsEvaluator ctor (physics_evaluator.hpp:188-201) + the once-only

@@ -395,6 +395,14 @@ namespace fro
                         F[idx(eq, k, l)] = fx[eq];
                         G[idx(eq, k, l)] = fy[eq];
                     }
+                    if (d.coord_sys == 1) // radial source :306-319 (r = y/2 at the mapped node)
+                    {
+                        const auto   pos    = geo.phys(comp_x(i, k), comp_y(j, l));
+                        const double radius = 0.5 * pos.second;
+                        const double inv_r  = (radius > 1e-8) ? 1.0 / radius : 0.0;
+                        for (int eq = 0; eq < 4; ++eq) res[idx(eq, k, l)] += G[idx(eq, k, l)] * inv_r;
+                        res[idx(2, k, l)] -= pres * inv_r;
+                    }
                 }
             const auto& D = tb.D;
             for (int l = 0; l < n; ++l)
@@ -806,7 +814,27 @@ namespace fro
                     local(sp(e), bc_state(0, node(sp(e), 0)), node(sp(e + 1), 0), f + e * nd);
                 }
             }
-            for (long e = first; e < last; ++e) local(sp(e), node(sp(e - 1), n - 1), node(sp(e + 1), 0), f + e * nd);
+            for (long e = first; e < last; ++e)
+            {
+                local(sp(e), node(sp(e - 1), n - 1), node(sp(e + 1), 0), f + e * nd);
+                if (d.coord_sys == 1) // radial source :70-90, interior loop only
+                {
+                    const double h  = d.lx / d.nx;
+                    const U*     s_ = sp(e);
+                    U*           r_ = f + e * nd;
+                    for (int m = 0; m < n; ++m)
+                    {
+                        const double r     = (static_cast<double>(e) + 0.5 * (tb.csi[m] + 1.0)) * h;
+                        const double inv_r = (r > 1e-8) ? 1.0 / r : 0.0;
+                        const U      rho = s_[m], rhoU = s_[n + m], rhoE = s_[2 * n + m];
+                        const U      vel  = rhoU / rho;
+                        const U      pres = gas.pressure(rho, rhoE / rho - U(0.5) * vel * vel);
+                        r_[m] += inv_r * rhoU;
+                        r_[n + m] += inv_r * rhoU * vel;
+                        r_[2 * n + m] += inv_r * vel * (rhoE + pres);
+                    }
+                }
+            }
         }
         // physics_impl_1D.hpp:96-121 limit_solution (neighbours beyond a
         // non-periodic end: the zero ghost layer, as in 2D)

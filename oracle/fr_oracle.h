@@ -36,6 +36,7 @@ typedef struct fro_desc
     double shock_theta;      /* -shock_sens_theta */
     double shock_threshold;  /* -shock_sens_activation_value */
     int    cond_sys;         /* -cond_sys: implicit path on (log rho, u, [v,] log e) */
+    int    coord_sys;        /* 0 Cartesian, 1 radial (source terms) */
 } fro_desc;
 
 void* fro_create(const fro_desc* d);

@@ -28,7 +28,7 @@ class FroDesc(ctypes.Structure):
         ("filt_strength", ctypes.c_int), ("filt_alpha", ctypes.c_double),
         ("apply_weno", ctypes.c_int), ("weno_gamma", ctypes.c_double),
         ("shock_theta", ctypes.c_double), ("shock_threshold", ctypes.c_double),
-        ("cond_sys", ctypes.c_int),
+        ("cond_sys", ctypes.c_int), ("coord_sys", ctypes.c_int),
     ]
 
 
@@ -112,6 +112,7 @@ def make_desc(cfg):
     d.shock_theta = cfg.get("shock_theta", 1.0)
     d.shock_threshold = cfg.get("shock_threshold", 0.8)
     d.cond_sys = cfg.get("cond_sys", 0)
+    d.coord_sys = cfg.get("coord_sys", 0)
     return d
 
 
