@@ -83,7 +83,7 @@ EXPORTS = [
     "frsdg_beuler_step", "frsdg_batch_create", "frsdg_batch_destroy", "frsdg_batch_ndofs",
     "frsdg_batch_offsets", "frsdg_batch_instance", "frsdg_batch_set_dt", "frsdg_batch_set_tuning",
     "frsdg_batch_ssprk_step", "frsdg_batch_ssprk_run", "frsdg_batch_ssprk_step_each",
-    "frsdg_filter_transpose", "frsdg_mass", "frsdg_boundary_weights", "frsdg_adjoint_solve",
+    "frsdg_filter_transpose", "frsdg_mass", "frsdg_boundary_weights", "frsdg_adjoint_solve", "frsdg_pseudo_run",
 ]
 
 _lib = None
@@ -119,6 +119,9 @@ def load_library(path=LIB_PATH):
     L.frsdg_l2_error.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double), d, dp,
                                  ctypes.POINTER(ctypes.c_double), vp]
     L.frsdg_beuler_step.argtypes = [vp, d, d, dp, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats), vp]
+    L.frsdg_pseudo_run.argtypes = [vp, d, d, d, i, d, d, i, dp, ctypes.POINTER(NewtonOpts),
+                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(ctypes.c_int), vp]
     L.frsdg_batch_create.argtypes = [ctypes.POINTER(FrsdgDesc), i, ctypes.POINTER(ctypes.c_void_p)]
     L.frsdg_batch_destroy.argtypes = [vp]
     L.frsdg_batch_ndofs.argtypes = [vp]
@@ -306,6 +309,21 @@ class PhysicsEvaluator:
         st = NewtonStats()
         _check(self.lib.frsdg_beuler_step(self.h, t, dt, _ptr(u), ctypes.byref(no), ctypes.byref(st), _stream()))
         return st.as_dict()
+
+    def pseudo_run(self, u, dt0, increment, max_steps, from_initial_dt=True, dt_max=0.0, t_max=0.0, t=0.0,
+                   **opts):
+        """Pseudo-transient continuation (TSPSEUDO, TSPseudoTimeStepDefault);
+        u updated in place.  Returns the step sizes and the steady-residual
+        norms after each step."""
+        o = dict(NEWTON_DEFAULTS)
+        o.update(opts)
+        no = NewtonOpts(**o)
+        dts = (ctypes.c_double * max(1, max_steps))()
+        fns = (ctypes.c_double * max(1, max_steps))()
+        n = ctypes.c_int(0)
+        _check(self.lib.frsdg_pseudo_run(self.h, t, dt0, increment, int(from_initial_dt), dt_max, t_max, max_steps,
+                                         _ptr(u), ctypes.byref(no), dts, fns, ctypes.byref(n), _stream()))
+        return {"dt": list(dts[:n.value]), "fnorm": list(fns[:n.value]), "steps": n.value}
 
     def adjoint_solve(self, u, rhs, sigma, t=0.0, out=None, **opts):
         """(sigma I + J(u))^T lambda = rhs by GMRES on the adjoint kernel."""

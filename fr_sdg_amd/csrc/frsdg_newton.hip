@@ -578,6 +578,60 @@ extern "C" int frsdg_beuler_step(frsdg_ctx* c, double t, double dt, double* u, c
     return FRSDG_OK;
 }
 
+// Pseudo-transient continuation (TSPSEUDO with TSPseudoTimeStepDefault):
+// backward-Euler steps with the switched-evolution-relaxation step size.
+extern "C" int frsdg_pseudo_run(frsdg_ctx* c, double t, double dt0, double increment, int from_initial_dt,
+                                double dt_max, double t_max, int max_steps, double* u,
+                                const frsdg_newton_opts* o, double* dt_hist, double* fnorm_hist, int* steps,
+                                void* stream)
+{
+    if (!c || !u || !o || !steps || !(dt0 > 0.0) || !(increment > 0.0) || max_steps < 0) return FRSDG_ERR_INVALID;
+    if (!aligned(u)) return FRSDG_ERR_ALIGN;
+    *steps = 0;
+    int rc = ensure_ws(c, o->ksp_restart);
+    if (rc) return rc;
+    SolverWs*   s  = c->ws;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    double      dt = dt0, f_first = -1.0, f_prev = -1.0;
+    const double t_end_tol = 1e-12 * (t_max > 0.0 ? t_max : 1.0);
+    for (int k = 0; k < max_steps; ++k)
+    {
+        if (t_max > 0.0)
+        {
+            if (t >= t_max - t_end_tol) break;
+            if (t + dt > t_max) dt = t_max - t; // -ts_exact_final_time matchstep
+        }
+        frsdg_newton_stats ns{};
+        if ((rc = frsdg_beuler_step(c, t, dt, u, o, &ns, stream))) return rc;
+        t += dt;
+        // f = ||F(u, U_t = 0)||: the IFunction with a zero time derivative
+        FR_TRY(hipMemsetAsync(s->ud, 0, sizeof(double) * s->N, st));
+        if ((rc = frsdg_ifunction(c, t, u, s->ud, s->w, stream))) return rc;
+        Ctx K{c, s, st, o, t, 0.0, u, &ns};
+        const double f = K.norm(s->w);
+        if (dt_hist) dt_hist[k] = dt;
+        if (fnorm_hist) fnorm_hist[k] = f;
+        *steps = k + 1;
+        if (f_first < 0.0)
+        {
+            f_first = f;
+            f_prev  = f;
+        }
+        double nd;
+        if (f == 0.0)
+            nd = 1.e12 * increment * dt;
+        else if (from_initial_dt)
+            nd = increment * dt0 * f_first / f;
+        else
+            nd = increment * dt * f_prev / f;
+        if (dt_max > 0.0) nd = std::min(nd, dt_max);
+        f_prev = f;
+        dt     = nd;
+    }
+    FR_TRY(hipStreamSynchronize(st));
+    return FRSDG_OK;
+}
+
 // Adjoint linear solve (sigma I + dR/du(u))^T lambda = rhs by GMRES with the
 // adjoint kernel as operator (and the transposed block-Jacobi preconditioner).
 extern "C" int frsdg_adjoint_solve(frsdg_ctx* c, double t, double sigma, const double* u, const double* rhs,

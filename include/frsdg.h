@@ -198,6 +198,24 @@ typedef struct frsdg_newton_stats
 int frsdg_beuler_step(frsdg_ctx* ctx, double t, double dt, double* u, const frsdg_newton_opts* opts,
                       frsdg_newton_stats* stats, void* stream);
 
+/* Pseudo-transient continuation, PETSc TSPSEUDO as the reference's steady
+ * harness drives it (-ts_type pseudo -ts_pseudo_increment inc
+ * [-ts_pseudo_increment_dt_from_initial_dt] -ts_pseudo_fatol 0
+ * -ts_pseudo_frtol 0, tools/boundary_conforming_claude.py:113-127):
+ * backward-Euler steps (frsdg_beuler_step) whose size follows
+ * TSPseudoTimeStepDefault: the first step takes dt0; after every step
+ * f = ||F(u, U_t = 0)||_2 (the IFunction, i.e. the steady residual) and the
+ * next step takes inc * dt0 * f_first / f (from_initial_dt) or
+ * inc * dt * f_previous / f, capped at dt_max when dt_max > 0, f_first being
+ * the first f computed.  Stops after max_steps steps or when t reaches t_max
+ * (t_max > 0; the last step is shortened to land on it, -ts_exact_final_time
+ * matchstep).  dt_hist and fnorm_hist (max_steps entries, may be NULL) get
+ * each step's dt and the f after it; *steps the number of steps taken.
+ * u is updated in place.  Synchronises the stream. */
+int frsdg_pseudo_run(frsdg_ctx* ctx, double t, double dt0, double increment, int from_initial_dt, double dt_max,
+                     double t_max, int max_steps, double* u, const frsdg_newton_opts* opts, double* dt_hist,
+                     double* fnorm_hist, int* steps, void* stream);
+
 /* Adjoint linear solve (sigma I + dR/du(u))^T lambda = rhs by GMRES on the
  * adjoint kernel (opts: ksp_* fields, precond, jac_type for the block probes).
  * No reference counterpart.  Synchronises the stream. */

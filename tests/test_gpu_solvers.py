@@ -105,3 +105,43 @@ def test_wall_pressure_adjoint_vs_fd(precond):
     ad = float(torch.dot(grad0, v))
     print(f"[wall] adjoint {ad:.12e} fd {fd:.12e}", flush=True)
     assert abs(ad - fd) <= 1e-6 * abs(fd) + 1e-9
+
+
+@pytest.mark.parametrize("from_initial", [True, False])
+def test_pseudo_transient_step_rule_and_replay(from_initial):
+    """frsdg_pseudo_run follows TSPseudoTimeStepDefault (SER rule) and equals
+    the same backward-Euler steps driven one by one; the steady residual
+    falls over the run (8x4 p=2 bump from the free stream)."""
+    import torch
+
+    import fr_sdg_amd as fr
+    from fr_sdg_amd import configs as CF
+    cfg = CF.bump(8, 4, 2)
+    ev = fr.PhysicsEvaluator(cfg)
+    u0 = ev.impose_ic(fr.IC_UNIFORM, CF.bump_freestream())
+    h = min(cfg["lx"] / cfg["nx"], cfg["ly"] / cfg["ny"])
+    dt0 = 5.0 * h / CF.sound_speed_max(u0, cfg)
+    opts = dict(jac_type=0, precond=1, ksp_restart=60, ksp_max_it=600, ksp_rtol=1e-4, snes_rtol=1e-6,
+                snes_max_it=8)
+    inc = 1.5
+    u = u0.clone()
+    res = ev.pseudo_run(u, dt0, inc, 6, from_initial_dt=from_initial, **opts)
+    dts, fs = res["dt"], res["fnorm"]
+    print("[pseudo]", from_initial, dts, fs, flush=True)
+    assert res["steps"] == 6 and dts[0] == dt0
+    for k in range(1, 6):
+        want = inc * dt0 * fs[0] / fs[k - 1] if from_initial else inc * dts[k - 1] * (fs[k - 2] if k >= 2 else fs[0]) / fs[k - 1]
+        assert dts[k] == pytest.approx(want, rel=1e-14)
+    # replay the same steps through frsdg_beuler_step
+    w = u0.clone()
+    t = 0.0
+    for dt in dts:
+        ev.beuler_step(w, dt, t=t, **opts)
+        t += dt
+    torch.cuda.synchronize()
+    assert torch.equal(w, u)
+    assert fs[-1] < fs[0]
+    # t_max: the last step lands on it
+    v = u0.clone()
+    r2 = ev.pseudo_run(v, dt0, inc, 50, from_initial_dt=from_initial, t_max=2.5 * dt0, **opts)
+    assert abs(sum(r2["dt"]) - 2.5 * dt0) <= 1e-12 * dt0
