@@ -151,9 +151,10 @@ namespace
     // the same with one workgroup per tile (grid = tiles of the group) and the
     // tile shape fixed at compile time (DIV = 1 default tile, 2 halved): no
     // loop and no run-time shape arithmetic, so the register budget is the
-    // single-problem kernel's
+    // single-problem kernel's (≤ 128 VGPRs, 4 waves per SIMD; a cap of 96 for
+    // 5 waves spills 60-88 bytes per thread at n = 5, 7, 8)
     template <int N, bool CURVED, bool FILT, int DIV>
-    __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_batch_tile_n(const BatchInst* __restrict__ tab, const WorkList W,
+    __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_tile_n(const BatchInst* __restrict__ tab, const WorkList W,
                                                           StageArgs A)
     {
         using T       = Tile2D<N, double>;
