@@ -875,10 +875,12 @@ namespace frsdg
 #pragma unroll
         for (int q = 0; q < 3; ++q)
             for (int m = 0; m < N; ++m) res[q][m] = Sc<S>::c(0.0);
+#pragma unroll
         for (int jn = 0; jn < N; ++jn)
         {
             S nd[3] = {u[0][jn], u[1][jn], u[2][jn]}, F[3];
             flux1(P.gas, nd, F);
+#pragma unroll
             for (int in = 0; in < N; ++in)
             {
                 const double Dij = P.inv_jac * D[in * N + jn];
@@ -895,6 +897,7 @@ namespace frsdg
         for (int q = 0; q < 3; ++q)
         {
             const S dl = FsL[q] - FdL[q], dr = FsR[q] - FdR[q];
+#pragma unroll
             for (int in = 0; in < N; ++in)
                 res[q][in] = res[q][in] + P.inv_jac * (dl * dL[in] + dr * dR[in]);
         }
@@ -903,6 +906,7 @@ namespace frsdg
         // physics_impl_1D.hpp:29-94)
         if (P.rinv && (P.periodic || (e > 0 && e < P.ne - 1)))
         {
+#pragma unroll
             for (int m = 0; m < N; ++m)
             {
                 S nd[3] = {u[0][m], u[1][m], u[2][m]}, src[3];

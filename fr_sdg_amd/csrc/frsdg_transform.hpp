@@ -63,11 +63,23 @@ namespace frsdg
     {
         const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
         if (i >= M.nodes) return;
-        const int nv = M.dims == 2 ? 4 : 3;
-        double    w[4], c[4];
-        for (int q = 0; q < nv; ++q) w[q] = W[M.at(i, q)];
-        p2c_node<double>(M.dims, w, c);
-        for (int q = 0; q < nv; ++q) C[M.at(i, q)] = c[q];
+        double w[4], c[4];
+        if (M.dims == 2)
+        {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) w[q] = W[M.at(i, q)];
+            p2c_node<double>(2, w, c);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) C[M.at(i, q)] = c[q];
+        }
+        else
+        {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) w[q] = W[M.at(i, q)];
+            p2c_node<double>(1, w, c);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) C[M.at(i, q)] = c[q];
+        }
     }
 
     // cons_to_prim_elem (problem_def.hpp:55-70); may run in place (W == C):
@@ -77,8 +89,10 @@ namespace frsdg
         const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
         if (i >= M.nodes) return;
         const int nv = M.dims == 2 ? 4 : 3;
-        double    c[4] = {0.0, 0.0, 0.0, 0.0}, w[4];
-        for (int q = 0; q < nv; ++q) c[q] = C[M.at(i, q)];
+        double    c[4] = {0.0, 0.0, 0.0, 0.0}, w[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q < nv) c[q] = C[M.at(i, q)];
         const double inv = 1.0 / c[0];
         w[0]             = ::log(c[0]);
         if (M.dims == 2)
@@ -96,7 +110,9 @@ namespace frsdg
             w[1]             = u;
             w[2]             = ::log(c[2] * inv - kin);
         }
-        for (int q = 0; q < nv; ++q) W[M.at(i, q)] = w[q];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (q < nv) W[M.at(i, q)] = w[q];
     }
 
     static __global__ __launch_bounds__(256) void k_p2c_tan(const NodeMap M, const double* __restrict__ W,
@@ -105,14 +121,31 @@ namespace frsdg
     {
         const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
         if (i >= M.nodes) return;
-        const int nv = M.dims == 2 ? 4 : 3;
-        Dual      w[4], c[4];
-        for (int q = 0; q < nv; ++q) w[q] = Dual{W[M.at(i, q)], V[M.at(i, q)]};
-        p2c_node<Dual>(M.dims, w, c);
-        for (int q = 0; q < nv; ++q)
+        // fixed trip counts keep the Dual arrays in registers
+        Dual w[4], c[4];
+        if (M.dims == 2)
         {
-            C[M.at(i, q)]  = c[q].v;
-            CV[M.at(i, q)] = c[q].d;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) w[q] = Dual{W[M.at(i, q)], V[M.at(i, q)]};
+            p2c_node<Dual>(2, w, c);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                C[M.at(i, q)]  = c[q].v;
+                CV[M.at(i, q)] = c[q].d;
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) w[q] = Dual{W[M.at(i, q)], V[M.at(i, q)]};
+            p2c_node<Dual>(1, w, c);
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+            {
+                C[M.at(i, q)]  = c[q].v;
+                CV[M.at(i, q)] = c[q].d;
+            }
         }
     }
 
@@ -124,14 +157,30 @@ namespace frsdg
     {
         const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
         if (i >= M.nodes) return;
-        const int nv = M.dims == 2 ? 4 : 3;
-        Cplx      w[4], c[4];
-        for (int q = 0; q < nv; ++q) w[q] = Cplx{W[M.at(i, q)], V[M.at(i, q)] * h};
-        p2c_node<Cplx>(M.dims, w, c);
-        for (int q = 0; q < nv; ++q)
+        Cplx w[4], c[4];
+        if (M.dims == 2)
         {
-            Cre[M.at(i, q)]   = c[q].re;
-            Cim_h[M.at(i, q)] = c[q].im * inv_h;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) w[q] = Cplx{W[M.at(i, q)], V[M.at(i, q)] * h};
+            p2c_node<Cplx>(2, w, c);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                Cre[M.at(i, q)]   = c[q].re;
+                Cim_h[M.at(i, q)] = c[q].im * inv_h;
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) w[q] = Cplx{W[M.at(i, q)], V[M.at(i, q)] * h};
+            p2c_node<Cplx>(1, w, c);
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+            {
+                Cre[M.at(i, q)]   = c[q].re;
+                Cim_h[M.at(i, q)] = c[q].im * inv_h;
+            }
         }
     }
 
