@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--no-large", action="store_true", help="skip the large curved residual")
     ap.add_argument("--no-newton", action="store_true", help="skip the implicit (JFNK) comparison")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep comparison")
+    ap.add_argument("--newton-pseudo", action="store_true",
+                    help="add the pseudo-transient tangent vs complex-step comparison (about 45 s)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--adjoint-large", action="store_true",
                     help="run only the unsteady adjoint of int(rho^2) on the 1024^2 p=7 warped mesh")
@@ -187,6 +189,11 @@ def main():
             line["newton"] = bench_newton(torch, fr)
         except Exception as exc:
             line["newton"] = {"error": repr(exc)}
+        if args.newton_pseudo:
+            try:
+                line["newton_pseudo"] = bench_newton_pseudo(torch, fr)
+            except Exception as exc:
+                line["newton_pseudo"] = {"error": repr(exc)}
     if not args.no_batch:
         try:
             line["batched"] = bench_batch(torch, fr)
@@ -284,6 +291,48 @@ def bench_large(torch, fr):
     del evl, u, out
     torch.cuda.empty_cache()
     return res
+
+
+def bench_newton_pseudo(torch, fr, steps=12):
+    """The bump 256x64 p=5 driven to steady state by pseudo-transient
+    continuation (frsdg_pseudo_run: SER step rule from CFL 1, x1.5 per step,
+    one backward-Euler JFNK solve per step: GMRES(60) to 1e-3 with element
+    block-Jacobi, <= 3 Newton iterations to 1e-3).  The tangent kernel
+    linearises the wall and Dirichlet boundary states exactly; the complex
+    step sees them frozen, as the reference's lambda BCs are.  Reported: the
+    steady-residual history of both and the time each takes to reach the
+    reduction the complex step ends with (equal nonlinear-residual
+    reduction)."""
+    from fr_sdg_amd import configs as CF
+    cfg = CF.bump(tangent_bc_mode=0)
+    ev = fr.PhysicsEvaluator(cfg)
+    u0 = ev.impose_ic(fr.IC_UNIFORM, CF.bump_freestream())
+    h = min(cfg["lx"] / cfg["nx"], cfg["ly"] / cfg["ny"])
+    dt0 = 1.0 * h / CF.sound_speed_max(u0, cfg)
+    opts = dict(precond=1, ksp_restart=60, ksp_max_it=1000, ksp_rtol=1e-3, snes_rtol=1e-3, snes_max_it=3)
+    runs = {}
+    for name, jt in (("tangent", 0), ("complex_step", 1)):
+        u = u0.clone()
+        r = ev.pseudo_run(u, dt0, 1.5, steps, jac_type=jt, **opts)
+        red = [f / r["fnorm"][0] for f in r["fnorm"]]
+        runs[name] = {"fnorm": r["fnorm"], "reduction": red, "seconds": r["seconds"],
+                      "cfl": [dt / (dt0 / 1.0) for dt in r["dt"]], "finite": bool(torch.isfinite(u).all())}
+    target = min(runs["complex_step"]["reduction"])
+
+    def t_to(run):
+        for red, t in zip(run["reduction"], run["seconds"]):
+            if red <= target:
+                return t
+        return None
+
+    tt, tc = t_to(runs["tangent"]), t_to(runs["complex_step"])
+    del ev, u0
+    torch.cuda.empty_cache()
+    return {"workload": f"bump 256x64 p=5 HLLC, pseudo-transient continuation (TSPSEUDO SER rule) from CFL 1, "
+                        f"x1.5 per step, {steps} steps; tangent with the exact BC linearisation, complex step "
+                        "with the reference's frozen lambda BCs", **runs, "target_reduction": target,
+            "tangent_s_to_target": tt, "complex_step_s_to_target": tc,
+            "speedup_at_equal_reduction": (tc / tt) if (tt and tc) else None}
 
 
 def bench_newton(torch, fr):

@@ -121,7 +121,7 @@ def load_library(path=LIB_PATH):
     L.frsdg_beuler_step.argtypes = [vp, d, d, dp, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats), vp]
     L.frsdg_pseudo_run.argtypes = [vp, d, d, d, i, d, d, i, dp, ctypes.POINTER(NewtonOpts),
                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
-                                   ctypes.POINTER(ctypes.c_int), vp]
+                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), vp]
     L.frsdg_batch_create.argtypes = [ctypes.POINTER(FrsdgDesc), i, ctypes.POINTER(ctypes.c_void_p)]
     L.frsdg_batch_destroy.argtypes = [vp]
     L.frsdg_batch_ndofs.argtypes = [vp]
@@ -313,17 +313,19 @@ class PhysicsEvaluator:
     def pseudo_run(self, u, dt0, increment, max_steps, from_initial_dt=True, dt_max=0.0, t_max=0.0, t=0.0,
                    **opts):
         """Pseudo-transient continuation (TSPSEUDO, TSPseudoTimeStepDefault);
-        u updated in place.  Returns the step sizes and the steady-residual
-        norms after each step."""
+        u updated in place.  Returns the step sizes, the steady-residual norms
+        after each step and the cumulative wall seconds."""
         o = dict(NEWTON_DEFAULTS)
         o.update(opts)
         no = NewtonOpts(**o)
         dts = (ctypes.c_double * max(1, max_steps))()
         fns = (ctypes.c_double * max(1, max_steps))()
+        tms = (ctypes.c_double * max(1, max_steps))()
         n = ctypes.c_int(0)
         _check(self.lib.frsdg_pseudo_run(self.h, t, dt0, increment, int(from_initial_dt), dt_max, t_max, max_steps,
-                                         _ptr(u), ctypes.byref(no), dts, fns, ctypes.byref(n), _stream()))
-        return {"dt": list(dts[:n.value]), "fnorm": list(fns[:n.value]), "steps": n.value}
+                                         _ptr(u), ctypes.byref(no), dts, fns, tms, ctypes.byref(n), _stream()))
+        return {"dt": list(dts[:n.value]), "fnorm": list(fns[:n.value]), "seconds": list(tms[:n.value]),
+                "steps": n.value}
 
     def adjoint_solve(self, u, rhs, sigma, t=0.0, out=None, **opts):
         """(sigma I + J(u))^T lambda = rhs by GMRES on the adjoint kernel."""

@@ -582,8 +582,8 @@ extern "C" int frsdg_beuler_step(frsdg_ctx* c, double t, double dt, double* u, c
 // backward-Euler steps with the switched-evolution-relaxation step size.
 extern "C" int frsdg_pseudo_run(frsdg_ctx* c, double t, double dt0, double increment, int from_initial_dt,
                                 double dt_max, double t_max, int max_steps, double* u,
-                                const frsdg_newton_opts* o, double* dt_hist, double* fnorm_hist, int* steps,
-                                void* stream)
+                                const frsdg_newton_opts* o, double* dt_hist, double* fnorm_hist, double* time_hist,
+                                int* steps, void* stream)
 {
     if (!c || !u || !o || !steps || !(dt0 > 0.0) || !(increment > 0.0) || max_steps < 0) return FRSDG_ERR_INVALID;
     if (!aligned(u)) return FRSDG_ERR_ALIGN;
@@ -593,6 +593,8 @@ extern "C" int frsdg_pseudo_run(frsdg_ctx* c, double t, double dt0, double incre
     SolverWs*   s  = c->ws;
     hipStream_t st = static_cast<hipStream_t>(stream);
     double      dt = dt0, f_first = -1.0, f_prev = -1.0;
+    FR_TRY(hipStreamSynchronize(st));
+    const auto t_start = std::chrono::steady_clock::now();
     const double t_end_tol = 1e-12 * (t_max > 0.0 ? t_max : 1.0);
     for (int k = 0; k < max_steps; ++k)
     {
@@ -611,6 +613,7 @@ extern "C" int frsdg_pseudo_run(frsdg_ctx* c, double t, double dt0, double incre
         const double f = K.norm(s->w);
         if (dt_hist) dt_hist[k] = dt;
         if (fnorm_hist) fnorm_hist[k] = f;
+        if (time_hist) time_hist[k] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
         *steps = k + 1;
         if (f_first < 0.0)
         {

@@ -209,12 +209,13 @@ int frsdg_beuler_step(frsdg_ctx* ctx, double t, double dt, double* u, const frsd
  * inc * dt * f_previous / f, capped at dt_max when dt_max > 0, f_first being
  * the first f computed.  Stops after max_steps steps or when t reaches t_max
  * (t_max > 0; the last step is shortened to land on it, -ts_exact_final_time
- * matchstep).  dt_hist and fnorm_hist (max_steps entries, may be NULL) get
- * each step's dt and the f after it; *steps the number of steps taken.
+ * matchstep).  dt_hist, fnorm_hist and time_hist (max_steps entries, may be
+ * NULL) get each step's dt, the f after it and the wall seconds since the
+ * start (stream-synchronised); *steps the number of steps taken.
  * u is updated in place.  Synchronises the stream. */
 int frsdg_pseudo_run(frsdg_ctx* ctx, double t, double dt0, double increment, int from_initial_dt, double dt_max,
                      double t_max, int max_steps, double* u, const frsdg_newton_opts* opts, double* dt_hist,
-                     double* fnorm_hist, int* steps, void* stream);
+                     double* fnorm_hist, double* time_hist, int* steps, void* stream);
 
 /* Adjoint linear solve (sigma I + dR/du(u))^T lambda = rhs by GMRES on the
  * adjoint kernel (opts: ksp_* fields, precond, jac_type for the block probes).

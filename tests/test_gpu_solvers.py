@@ -126,6 +126,7 @@ def test_pseudo_transient_step_rule_and_replay(from_initial):
     inc = 1.5
     u = u0.clone()
     res = ev.pseudo_run(u, dt0, inc, 6, from_initial_dt=from_initial, **opts)
+    assert len(res["seconds"]) == 6 and all(b >= a for a, b in zip(res["seconds"], res["seconds"][1:]))
     dts, fs = res["dt"], res["fnorm"]
     print("[pseudo]", from_initial, dts, fs, flush=True)
     assert res["steps"] == 6 and dts[0] == dt0
