@@ -298,6 +298,34 @@ namespace
         (void) E;
     }
 
+    // z_e = B_e^-1 x_e on the elements of colour col (checkerboard (i + j) & 1);
+    // the other colour is zeroed when zero_other is set, left as is otherwise
+    __global__ void k_block_apply_color(long long E, int nx, int nd_e, const float* __restrict__ Binv,
+                                        const double* __restrict__ x, double* __restrict__ z, int transpose, int col,
+                                        int zero_other)
+    {
+        const long long e = blockIdx.x;
+        const int       i = static_cast<int>(e % nx), j = static_cast<int>(e / nx);
+        if (((i + j) & 1) != col)
+        {
+            if (zero_other)
+                for (int r = threadIdx.x; r < nd_e; r += blockDim.x) z[e * nd_e + r] = 0.0;
+            return;
+        }
+        const float*  A  = Binv + e * nd_e * nd_e;
+        const double* xe = x + e * nd_e;
+        for (int r = threadIdx.x; r < nd_e; r += blockDim.x)
+        {
+            double s = 0.0;
+            if (transpose)
+                for (int c = 0; c < nd_e; ++c) s += A[static_cast<long long>(r) * nd_e + c] * xe[c];
+            else
+                for (int c = 0; c < nd_e; ++c) s += A[static_cast<long long>(c) * nd_e + r] * xe[c];
+            z[e * nd_e + r] = s;
+        }
+        (void) E;
+    }
+
     struct Ctx
     {
         frsdg_ctx* c;
@@ -320,6 +348,23 @@ namespace
         }
         int prec(const double* x, double* z)
         {
+            if (o->precond == 2)
+            {
+                // red-black element block Gauss-Seidel, one forward sweep, matrix-free:
+                // z_R = D_R^-1 x_R;  z_B = D_B^-1 (x - A z)_B  (A = sigma I + J, or its
+                // transpose for the adjoint; A z restricted to black needs only the
+                // red entries of z, which is all z holds at that point)
+                const int nd_e = static_cast<int>(c->nd / c->E);
+                const dim3 bs(nd_e <= 64 ? 64 : (nd_e <= 128 ? 128 : 256));
+                hipLaunchKernelGGL(k_block_apply_color, dim3(static_cast<unsigned>(c->E)), bs, 0, st, c->E, c->d.nx,
+                                   nd_e, s->Bf, x, z, transpose, 0, 1);
+                int rc = jv(z, s->pjv);
+                if (rc) return rc;
+                hipLaunchKernelGGL(k_waxpby, dim3(grid(s->N)), dim3(256), 0, st, s->N, 1.0, x, -1.0, s->pjv, s->pv);
+                hipLaunchKernelGGL(k_block_apply_color, dim3(static_cast<unsigned>(c->E)), bs, 0, st, c->E, c->d.nx,
+                                   nd_e, s->Bf, s->pv, z, transpose, 1, 0);
+                return hip_rc(hipGetLastError());
+            }
             if (o->precond == 1)
             {
                 const int nd_e = static_cast<int>(c->nd / c->E);
@@ -497,9 +542,9 @@ extern "C" int frsdg_beuler_step(frsdg_ctx* c, double t, double dt, double* u, c
 {
     if (!c || !u || !o || !(dt > 0.0)) return FRSDG_ERR_INVALID;
     if (!aligned(u)) return FRSDG_ERR_ALIGN;
-    if (o->precond < 0 || o->precond > 1 || o->jac_type < 0 || o->jac_type > 1 || o->ksp_restart < 1)
+    if (o->precond < 0 || o->precond > 2 || o->jac_type < 0 || o->jac_type > 1 || o->ksp_restart < 1)
         return FRSDG_ERR_INVALID;
-    if (o->precond == 1 && c->d.dims != 2) return FRSDG_ERR_UNSUPPORTED;
+    if (o->precond >= 1 && c->d.dims != 2) return FRSDG_ERR_UNSUPPORTED;
     frsdg_newton_stats local{};
     frsdg_newton_stats* S = stats ? stats : &local;
     *S = frsdg_newton_stats{};
@@ -543,7 +588,7 @@ extern "C" int frsdg_beuler_step(frsdg_ctx* c, double t, double dt, double* u, c
     const double target = std::max(o->snes_rtol * fn, o->snes_atol);
     while (fn > target && S->newton_its < o->snes_max_it && std::isfinite(fn))
     {
-        if (o->precond == 1)
+        if (o->precond >= 1)
             if ((rc = K.build_block_jacobi())) return rc;
         // J du = -F
         hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, st, N, 0.0, s->F, -1.0, s->F);
@@ -643,7 +688,7 @@ extern "C" int frsdg_adjoint_solve(frsdg_ctx* c, double t, double sigma, const d
 {
     if (!c || !u || !rhs || !lambda || !o) return FRSDG_ERR_INVALID;
     if (!aligned(u) || !aligned(rhs) || !aligned(lambda)) return FRSDG_ERR_ALIGN;
-    if (o->precond == 1 && c->d.dims != 2) return FRSDG_ERR_UNSUPPORTED;
+    if (o->precond >= 1 && c->d.dims != 2) return FRSDG_ERR_UNSUPPORTED;
     frsdg_newton_stats local{};
     frsdg_newton_stats* S = stats ? stats : &local;
     *S = frsdg_newton_stats{};
@@ -653,8 +698,9 @@ extern "C" int frsdg_adjoint_solve(frsdg_ctx* c, double t, double sigma, const d
     hipStream_t st = static_cast<hipStream_t>(stream);
     FR_TRY(hipStreamSynchronize(st));
     const auto t0 = std::chrono::steady_clock::now();
+    if (o->precond < 0 || o->precond > 2) return FRSDG_ERR_INVALID;
     Ctx K{c, s, st, o, t, sigma, u, S};
-    if (o->precond == 1)
+    if (o->precond >= 1)
         if ((rc = K.build_block_jacobi())) return rc;
     K.transpose = 1;
     FR_TRY(hipMemcpyAsync(s->F, rhs, sizeof(double) * s->N, hipMemcpyDeviceToDevice, st));

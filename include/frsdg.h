@@ -176,7 +176,9 @@ int frsdg_l2_error(frsdg_ctx* ctx, int kind, const double* prm, double t, const 
 typedef struct frsdg_newton_opts
 {
     int    jac_type;    /* 0 forward tangent, 1 complex step (h = 1e-7) */
-    int    precond;     /* 0 none (-pc_type none), 1 element block-Jacobi */
+    int    precond;     /* 0 none (-pc_type none), 1 element block-Jacobi, 2 red-black
+                           element block Gauss-Seidel (one forward sweep, matrix-free
+                           off-diagonal coupling: one extra J*v per application) */
     double snes_rtol;   /* stop when ||F|| <= snes_rtol * ||F_0|| (PETSc default 1e-8) */
     double snes_atol;   /* or ||F|| <= snes_atol (PETSc default 1e-50) */
     int    snes_max_it; /* PETSc default 50 */

@@ -363,7 +363,7 @@ def dense_beuler_oracle(o, u_n, dt, tol=1e-12, maxit=30):
 
 
 @pytest.mark.parametrize("jac_type", [0, 1])
-@pytest.mark.parametrize("precond", [0, 1])
+@pytest.mark.parametrize("precond", [0, 1, 2])
 def test_beuler_jfnk_vs_dense_newton(jac_type, precond):
     """One implicit step on a tiny bump mesh (reference-compatible BCs so complex
     step and tangent linearise the same map) against a dense Newton solve."""
