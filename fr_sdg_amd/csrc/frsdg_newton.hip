@@ -15,6 +15,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <limits>
 #include <vector>
 
 using namespace frsdg;
@@ -595,20 +596,70 @@ extern "C" int frsdg_beuler_step(frsdg_ctx* c, double t, double dt, double* u, c
         int its = 0;
         if ((rc = K.gmres(s->F, s->du, its))) return rc;
         S->ksp_its += its;
-        // backtracking line search on ||F||
-        double lam = 1.0, fnew = 0.0;
-        bool   decreased = false;
-        for (int ls = 0; ls < 8; ++ls)
+        // PETSc's default line search (SNESLINESEARCHBT, order 3): Armijo
+        // test with alpha = 1e-4 on 0.5 ||F||^2, a quadratic then cubic
+        // backtracking model, each new lambda clamped to [0.1, 0.5] of the
+        // previous one, at most 40 cubic steps, minlambda 1e-12.  The initial
+        // slope is F . (J du) from one more J*v product, made negative as
+        // PETSc does (-|F . J du|, -1 when zero).
+        double initslope = 0.0;
+        {
+            const int keep_t = K.transpose;
+            K.transpose      = 0;
+            rc               = K.jv(s->du, s->w);
+            K.transpose      = keep_t;
+            if (rc || (rc = K.mdot(s->F, 1, s->w))) return rc; // s->F holds -F here: only |.| is used
+            FR_TRY(hipMemcpyAsync(&initslope, s->red, sizeof(double), hipMemcpyDeviceToHost, st));
+            FR_TRY(hipStreamSynchronize(st));
+        }
+        initslope = -std::fabs(initslope);
+        if (initslope == 0.0) initslope = -1.0;
+        const double alpha = 1e-4, minlambda = 1e-12;
+        auto try_step = [&](double lam, double& gnorm) -> int
         {
             hipLaunchKernelGGL(k_waxpby, dim3(g), dim3(256), 0, st, N, 1.0, u, lam, s->du, s->ut);
-            if ((rc = eval_F(s->ut, s->w))) return rc;
-            fnew = K.norm(s->w);
-            if (std::isfinite(fnew) && fnew <= (1.0 - 1e-4 * lam) * fn)
-            {
+            int r2 = eval_F(s->ut, s->w);
+            if (r2) return r2;
+            gnorm = K.norm(s->w);
+            if (!std::isfinite(gnorm)) gnorm = std::numeric_limits<double>::infinity();
+            return FRSDG_OK;
+        };
+        auto armijo = [&](double lam, double gnorm)
+        { return 0.5 * gnorm * gnorm <= 0.5 * fn * fn + lam * alpha * initslope; };
+        double lam = 1.0, fnew = 0.0;
+        bool   decreased = false;
+        if ((rc = try_step(lam, fnew))) return rc;
+        if (armijo(lam, fnew))
+            decreased = true;
+        else
+        {
+            // quadratic
+            double lamprev = lam, gprev = fnew;
+            double lt      = -initslope / (fnew * fnew - fn * fn - 2.0 * lam * initslope);
+            if (!std::isfinite(lt)) lt = 0.1 * lam;
+            if (lt > 0.5 * lam) lt = 0.5 * lam;
+            lam = (lt <= 0.1 * lam) ? 0.1 * lam : lt;
+            if ((rc = try_step(lam, fnew))) return rc;
+            if (armijo(lam, fnew))
                 decreased = true;
-                break;
+            // cubic
+            for (int count = 0; !decreased && count < 40 && lam > minlambda; ++count)
+            {
+                const double t1 = 0.5 * (fnew * fnew - fn * fn) - lam * initslope;
+                const double t2 = 0.5 * (gprev * gprev - fn * fn) - lamprev * initslope;
+                const double a  = (t1 / (lam * lam) - t2 / (lamprev * lamprev)) / (lam - lamprev);
+                const double b  = (-lamprev * t1 / (lam * lam) + lam * t2 / (lamprev * lamprev)) / (lam - lamprev);
+                double       d  = b * b - 3.0 * a * initslope;
+                if (d < 0.0) d = 0.0;
+                lt = (a == 0.0) ? -initslope / (2.0 * b) : (-b + std::sqrt(d)) / (3.0 * a);
+                if (!std::isfinite(lt)) lt = 0.1 * lam;
+                lamprev = lam;
+                gprev   = fnew;
+                if (lt > 0.5 * lam) lt = 0.5 * lam;
+                lam = (lt <= 0.1 * lam) ? 0.1 * lam : lt;
+                if ((rc = try_step(lam, fnew))) return rc;
+                if (armijo(lam, fnew)) decreased = true;
             }
-            lam *= 0.5;
         }
         if (!decreased) break; // line search failed (SNES_DIVERGED_LINE_SEARCH): keep u
         FR_TRY(hipMemcpyAsync(u, s->ut, sizeof(double) * N, hipMemcpyDeviceToDevice, st));

@@ -169,8 +169,9 @@ int frsdg_l2_error(frsdg_ctx* ctx, int kind, const double* prm, double t, const 
 /* ---------------- implicit steps: Jacobian-free Newton-Krylov ----------------
  * The TSSetIFunction / TSSetIJacobian path of the reference
  * (petsc_solver_impl.hpp:572-579, 626-655, 168-267) with PETSc's SNES newtonls
- * + KSP GMRES replaced by a device solver: Newton with backtracking line
- * search, restarted GMRES (classical Gram-Schmidt, right preconditioning) whose
+ * + KSP GMRES replaced by a device solver: Newton with PETSc's default
+ * cubic backtracking line search (SNESLINESEARCHBT: Armijo alpha 1e-4,
+ * steps clamped to [0.1, 0.5] of the previous, 40 tries), restarted GMRES (classical Gram-Schmidt, right preconditioning) whose
  * operator is the MatShell product sigma*v + J v from the forward-tangent or
  * the complex-step kernel. */
 typedef struct frsdg_newton_opts
