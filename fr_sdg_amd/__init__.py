@@ -84,6 +84,7 @@ EXPORTS = [
     "frsdg_batch_offsets", "frsdg_batch_instance", "frsdg_batch_set_dt", "frsdg_batch_set_tuning",
     "frsdg_batch_ssprk_step", "frsdg_batch_ssprk_run", "frsdg_batch_ssprk_step_each",
     "frsdg_filter_transpose", "frsdg_mass", "frsdg_boundary_weights", "frsdg_adjoint_solve", "frsdg_pseudo_run",
+    "frsdg_color_jacobian", "frsdg_ilu0_factor", "frsdg_ilu0_apply",
 ]
 
 _lib = None
@@ -136,6 +137,9 @@ def load_library(path=LIB_PATH):
     L.frsdg_filter_transpose.argtypes = [vp, dp, dp, vp]
     L.frsdg_mass.argtypes = [vp, dp, vp]
     L.frsdg_boundary_weights.argtypes = [vp, i, dp, vp]
+    L.frsdg_color_jacobian.argtypes = [vp, d, dp, dp, d, i, dp, vp]
+    L.frsdg_ilu0_factor.argtypes = [vp, dp, dp, vp]
+    L.frsdg_ilu0_apply.argtypes = [vp, dp, dp, dp, dp, vp]
     L.frsdg_adjoint_solve.argtypes = [vp, d, d, dp, dp, dp, ctypes.POINTER(NewtonOpts),
                                       ctypes.POINTER(NewtonStats), vp]
     L.frsdg_strerror.argtypes = [i]
@@ -326,6 +330,37 @@ class PhysicsEvaluator:
                                          _ptr(u), ctypes.byref(no), dts, fns, tms, ctypes.byref(n), _stream()))
         return {"dt": list(dts[:n.value]), "fnorm": list(fns[:n.value]), "seconds": list(tms[:n.value]),
                 "steps": n.value}
+
+    def color_layout(self):
+        """(E, S, nd): block rows, slots per row (9 in 2D, 3 in 1D), DOFs per element."""
+        d = self._desc
+        E = d.nx * (d.ny if d.dims == 2 else 1)
+        return E, (9 if d.dims == 2 else 3), self.ndofs // E
+
+    def color_jacobian(self, u, sigma, u_n=None, method=0, t=0.0):
+        """The reference's preconditioning matrix dF/du (DMDA box-stencil
+        colouring + MatFDColoringApply, petsc_solver_impl.hpp:72-95,126-160)
+        of F(x) = IFunction(t, x, sigma (T(x) - T(u_n))).  Returns a tensor
+        [E, S, nd(col), nd(row)]: block (e, slot) = dF_e/du_neighbour.
+        method 0 FD with PETSc's default "wp" step, 1 "ds" step, 2 exact."""
+        E, S, nd = self.color_layout()
+        P = torch.empty((E, S, nd, nd), dtype=torch.float64, device=self.device)
+        _check(self.lib.frsdg_color_jacobian(self.h, t, _ptr(u), None if u_n is None else _ptr(u_n), sigma,
+                                             int(method), _ptr(P), _stream()))
+        return P
+
+    def ilu0_factor(self, P):
+        """Block ILU(0) of P (color_jacobian layout) in place; returns the
+        diagonal blocks' triangular inverses [E, 2, nd, nd]."""
+        E, S, nd = self.color_layout()
+        Dinv = torch.empty((E, 2, nd, nd), dtype=torch.float64, device=self.device)
+        _check(self.lib.frsdg_ilu0_factor(self.h, _ptr(P), _ptr(Dinv), _stream()))
+        return Dinv
+
+    def ilu0_apply(self, P, Dinv, x, out=None):
+        out = self.empty() if out is None else out
+        _check(self.lib.frsdg_ilu0_apply(self.h, _ptr(P), _ptr(Dinv), _ptr(x), _ptr(out), _stream()))
+        return out
 
     def adjoint_solve(self, u, rhs, sigma, t=0.0, out=None, **opts):
         """(sigma I + J(u))^T lambda = rhs by GMRES on the adjoint kernel."""

@@ -16,7 +16,7 @@ SOURCES = [os.path.join(HERE, "csrc", f) for f in ("frsdg_capi.hip", "frsdg_newt
 HEADERS = [
     os.path.join(HERE, "csrc", f)
     for f in ("frsdg_scalar.hpp", "frsdg_physics.hpp", "frsdg_geometry.hpp", "frsdg_kernels.hpp",
-              "frsdg_tables.hpp", "frsdg_adjoint.hpp", "frsdg_reverse.hpp", "frsdg_limiter.hpp", "frsdg_transform.hpp", "frsdg_ctx.hpp")
+              "frsdg_tables.hpp", "frsdg_adjoint.hpp", "frsdg_reverse.hpp", "frsdg_limiter.hpp", "frsdg_transform.hpp", "frsdg_color.hpp", "frsdg_ctx.hpp")
 ] + [os.path.join(ROOT, "include", "frsdg.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

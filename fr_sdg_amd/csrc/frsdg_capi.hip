@@ -604,6 +604,7 @@ const char* frsdg_strerror(int code)
         case FRSDG_ERR_HIP: return "HIP runtime error";
         case FRSDG_ERR_UNSUPPORTED: return "unsupported on this path";
         case FRSDG_ERR_ALIGN: return "device pointer not 16-byte aligned";
+        case FRSDG_ERR_NUMERIC: return "zero pivot in a factorisation";
     }
     return "unknown error";
 }

@@ -11,6 +11,7 @@
 // neighbours never share a colour, so each probe isolates one column of every
 // diagonal block of its colour) and inverted in place by Gauss-Jordan.
 // The small Hessenberg least-squares problem runs on the host (Givens).
+#include "frsdg_color.hpp"
 #include "frsdg_ctx.hpp"
 
 #include <chrono>
@@ -41,6 +42,8 @@ struct SolverWs
     double* pv   = nullptr; // probe vector
     double* pjv  = nullptr; // probe product
     int*    info = nullptr; // per-block singularity flag
+    // coloured preconditioning matrix and its ILU(0)
+    frsdg::ColorPlan* cp = nullptr;
 };
 
 namespace frsdg
@@ -53,6 +56,14 @@ namespace frsdg
             if (p) (void) hipFree(p);
         if (s->info) (void) hipFree(s->info);
         if (s->Bf) (void) hipFree(s->Bf);
+        if (frsdg::ColorPlan* cp = s->cp)
+        {
+            for (int* p : {cp->nbr, cp->cls, cp->low, cp->upd, cp->upp, cp->lvl, cp->info})
+                if (p) (void) hipFree(p);
+            for (double* p : {cp->P, cp->Dinv, cp->y, cp->v, cp->f0, cp->f1})
+                if (p) (void) hipFree(p);
+            delete cp;
+        }
         delete s;
         c->ws = nullptr;
     }
@@ -327,6 +338,71 @@ namespace
         (void) E;
     }
 
+    // Tables and buffers of the coloured matrix and its ILU(0), once per context.
+    int color_plan_ensure(frsdg_ctx* c, SolverWs* s)
+    {
+        if (s->cp) return FRSDG_OK;
+        const int dims = c->d.dims;
+        const int nd   = static_cast<int>(c->nd / c->E);
+        if (nd > frsdg::CMAXND) return FRSDG_ERR_UNSUPPORTED;
+        auto* cp = new frsdg::ColorPlan{};
+        std::vector<int> nbr, cls, low, upd, upp, lvl;
+        int rc = frsdg::color_plan_tables(dims, c->d.nx, dims == 2 ? c->d.ny : 1, dims == 2 ? c->p2.per_x : c->p1.periodic,
+                                          dims == 2 ? c->p2.per_y : 0, nd, *cp, nbr, cls, low, upd, upp, lvl);
+        if (rc)
+        {
+            delete cp;
+            return rc;
+        }
+        s->cp = cp; // owned by the workspace from here on, also if an allocation below fails
+        auto up = [](int*& d, const std::vector<int>& h) -> int
+        {
+            FR_TRY(hipMalloc(&d, sizeof(int) * std::max<size_t>(1, h.size())));
+            FR_TRY(hipMemcpy(d, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
+            return FRSDG_OK;
+        };
+        if ((rc = up(cp->nbr, nbr)) || (rc = up(cp->cls, cls)) || (rc = up(cp->low, low)) || (rc = up(cp->upd, upd)) ||
+            (rc = up(cp->upp, upp)) || (rc = up(cp->lvl, lvl)))
+            return rc;
+        FR_TRY(hipMalloc(&cp->info, sizeof(int)));
+        const long long bsz = static_cast<long long>(nd) * nd;
+        FR_TRY(hipMalloc(&cp->P, sizeof(double) * cp->E * cp->S * bsz));
+        FR_TRY(hipMalloc(&cp->Dinv, sizeof(double) * cp->E * 2 * bsz));
+        for (double** p : {&cp->y, &cp->v, &cp->f0, &cp->f1}) FR_TRY(hipMalloc(p, sizeof(double) * s->N));
+        return FRSDG_OK;
+    }
+
+    // Block ILU(0) of P in place (frsdg_color.hpp), level by level.
+    int ilu_factor(SolverWs* s, hipStream_t st, double* P, double* Dinv)
+    {
+        const frsdg::ColorPlan& cp = *s->cp;
+        FR_TRY(hipMemsetAsync(cp.info, 0, sizeof(int), st));
+        const int nlev = static_cast<int>(cp.lvl_ptr.size()) - 1;
+        for (int l = 0; l < nlev; ++l)
+            hipLaunchKernelGGL(frsdg::k_ilu_row, dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l])),
+                               dim3(256), 0, st, cp.S, cp.nd, cp.nbr, cp.low, cp.upd, cp.upp, cp.lvl, cp.lvl_ptr[l], P,
+                               Dinv, cp.info);
+        FR_TRY(hipGetLastError());
+        int info = 0;
+        FR_TRY(hipMemcpyAsync(&info, cp.info, sizeof(int), hipMemcpyDeviceToHost, st));
+        FR_TRY(hipStreamSynchronize(st));
+        return info ? FRSDG_ERR_NUMERIC : FRSDG_OK;
+    }
+
+    // z = (L U)^-1 x: forward sweep over the levels, then backward (z may alias x).
+    int ilu_apply(SolverWs* s, hipStream_t st, const double* P, const double* Dinv, const double* x, double* z)
+    {
+        const frsdg::ColorPlan& cp = *s->cp;
+        const int nlev = static_cast<int>(cp.lvl_ptr.size()) - 1;
+        for (int l = 0; l < nlev; ++l)
+            hipLaunchKernelGGL(frsdg::k_ilu_fwd, dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l])),
+                               dim3(256), 0, st, cp.S, cp.nd, cp.nbr, cp.low, cp.lvl, cp.lvl_ptr[l], P, Dinv, x, cp.y);
+        for (int l = nlev - 1; l >= 0; --l)
+            hipLaunchKernelGGL(frsdg::k_ilu_bwd, dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l])),
+                               dim3(256), 0, st, cp.S, cp.nd, cp.nbr, cp.upp, cp.lvl, cp.lvl_ptr[l], P, Dinv, cp.y, z);
+        return hip_rc(hipGetLastError());
+    }
+
     struct Ctx
     {
         frsdg_ctx* c;
@@ -349,6 +425,7 @@ namespace
         }
         int prec(const double* x, double* z)
         {
+            if (o->precond >= 3) return ilu_apply(s, st, s->cp->P, s->cp->Dinv, x, z);
             if (o->precond == 2)
             {
                 // red-black element block Gauss-Seidel, one forward sweep, matrix-free:
@@ -389,6 +466,67 @@ namespace
             (void) hipMemcpyAsync(&h, s->red, sizeof(double), hipMemcpyDeviceToHost, st);
             (void) hipStreamSynchronize(st);
             return std::sqrt(h);
+        }
+        // The coloured matrix dF/du of F at u (frsdg_color.hpp): method 0 coloured
+        // finite differences with PETSc's default "wp" step, 1 the "ds" step,
+        // 2 exact columns from tangent products of the unit colour vectors.
+        template <class EvalF>
+        int color_assemble(int method, EvalF&& evalF, double* P)
+        {
+            frsdg::ColorPlan& cp = *s->cp;
+            const long long N = s->N;
+            const double eps = 1.4901161193847656e-08; // sqrt(DBL_EPSILON): MatFDColoring error_rel
+            const double umin = 100.0 * eps;             // MatFDColoring umin
+            FR_TRY(hipMemsetAsync(P, 0, sizeof(double) * cp.E * cp.S * cp.nd * cp.nd, st));
+            double h_wp = 0.0, dxinv = 0.0;
+            if (method == 2)
+                FR_TRY(hipMemsetAsync(cp.v, 0, sizeof(double) * N, st));
+            else
+            {
+                int rc = evalF(u, cp.f0);
+                if (rc) return rc;
+                FR_TRY(hipMemcpyAsync(cp.v, u, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+                if (method == 0)
+                {
+                    const double unorm = norm(u);
+                    dxinv = 1.0 / (std::sqrt(1.0 + unorm) * eps);
+                    h_wp  = 1.0 / dxinv;
+                }
+            }
+            const unsigned ge = static_cast<unsigned>((cp.E + 255) / 256);
+            const int keep_t = transpose;
+            for (int cc = 0; cc < cp.ncls; ++cc)
+            {
+                if (!cp.cls_count[cc]) continue;
+                for (int k = 0; k < cp.nd; ++k)
+                {
+                    hipLaunchKernelGGL(frsdg::k_color_perturb, dim3(ge), dim3(256), 0, st, cp.E, cp.nd, cp.cls, cc, k, u,
+                                       cp.v, h_wp, eps, umin, method, 0);
+                    int rc;
+                    if (method == 2)
+                    {
+                        transpose = 0;
+                        rc        = jv(cp.v, cp.f1);
+                        transpose = keep_t;
+                    }
+                    else
+                        rc = evalF(cp.v, cp.f1);
+                    if (rc) return rc;
+                    hipLaunchKernelGGL(frsdg::k_color_store, dim3(grid(N)), dim3(256), 0, st, cp.E, cp.S, cp.nd, cp.nbr,
+                                       cp.cls, cc, k, cp.f1, cp.f0, u, dxinv, eps, umin, method, P);
+                    hipLaunchKernelGGL(frsdg::k_color_perturb, dim3(ge), dim3(256), 0, st, cp.E, cp.nd, cp.cls, cc, k, u,
+                                       cp.v, h_wp, eps, umin, method, 1);
+                }
+            }
+            return hip_rc(hipGetLastError());
+        }
+        template <class EvalF>
+        int build_ilu(int method, EvalF&& evalF)
+        {
+            int rc = color_plan_ensure(c, s);
+            if (rc) return rc;
+            if ((rc = color_assemble(method, evalF, s->cp->P))) return rc;
+            return ilu_factor(s, st, s->cp->P, s->cp->Dinv);
         }
         int build_block_jacobi()
         {
@@ -543,9 +681,9 @@ extern "C" int frsdg_beuler_step(frsdg_ctx* c, double t, double dt, double* u, c
 {
     if (!c || !u || !o || !(dt > 0.0)) return FRSDG_ERR_INVALID;
     if (!aligned(u)) return FRSDG_ERR_ALIGN;
-    if (o->precond < 0 || o->precond > 2 || o->jac_type < 0 || o->jac_type > 1 || o->ksp_restart < 1)
+    if (o->precond < 0 || o->precond > 4 || o->jac_type < 0 || o->jac_type > 1 || o->ksp_restart < 1)
         return FRSDG_ERR_INVALID;
-    if (o->precond >= 1 && c->d.dims != 2) return FRSDG_ERR_UNSUPPORTED;
+    if (o->precond >= 1 && o->precond <= 2 && c->d.dims != 2) return FRSDG_ERR_UNSUPPORTED;
     frsdg_newton_stats local{};
     frsdg_newton_stats* S = stats ? stats : &local;
     *S = frsdg_newton_stats{};
@@ -589,7 +727,13 @@ extern "C" int frsdg_beuler_step(frsdg_ctx* c, double t, double dt, double* u, c
     const double target = std::max(o->snes_rtol * fn, o->snes_atol);
     while (fn > target && S->newton_its < o->snes_max_it && std::isfinite(fn))
     {
-        if (o->precond >= 1)
+        if (o->precond >= 3)
+        {
+            // the reference rebuilds its preconditioning matrix at every Newton
+            // iteration (PETSc's default lag 1)
+            if ((rc = K.build_ilu(o->precond == 3 ? 0 : 2, eval_F))) return rc;
+        }
+        else if (o->precond >= 1)
             if ((rc = K.build_block_jacobi())) return rc;
         // J du = -F
         hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, st, N, 0.0, s->F, -1.0, s->F);
@@ -667,9 +811,14 @@ extern "C" int frsdg_beuler_step(frsdg_ctx* c, double t, double dt, double* u, c
         fn = fnew;
         S->newton_its++;
     }
-    FR_TRY(hipStreamSynchronize(st));
     S->fnorm     = fn;
     S->converged = fn <= target;
+    // With the filter or the WENO limiter on, the reference stabilises the
+    // solution once per implicit step (TSSetPostStep, or TSSetPostEvaluate
+    // under -do_post_ts false: petsc_solver_impl.hpp:582-596; both run at the
+    // end of a one-stage step), on the state vector as it is (W under cond_sys).
+    if ((rc = frsdg_apply_stabilization(c, u, stream))) return rc;
+    FR_TRY(hipStreamSynchronize(st));
     S->seconds   = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return FRSDG_OK;
 }
@@ -739,6 +888,8 @@ extern "C" int frsdg_adjoint_solve(frsdg_ctx* c, double t, double sigma, const d
 {
     if (!c || !u || !rhs || !lambda || !o) return FRSDG_ERR_INVALID;
     if (!aligned(u) || !aligned(rhs) || !aligned(lambda)) return FRSDG_ERR_ALIGN;
+    if (o->precond < 0 || o->precond > 4) return FRSDG_ERR_INVALID;
+    if (o->precond >= 3) return FRSDG_ERR_UNSUPPORTED; // no transposed ILU sweeps
     if (o->precond >= 1 && c->d.dims != 2) return FRSDG_ERR_UNSUPPORTED;
     frsdg_newton_stats local{};
     frsdg_newton_stats* S = stats ? stats : &local;
@@ -749,7 +900,6 @@ extern "C" int frsdg_adjoint_solve(frsdg_ctx* c, double t, double sigma, const d
     hipStream_t st = static_cast<hipStream_t>(stream);
     FR_TRY(hipStreamSynchronize(st));
     const auto t0 = std::chrono::steady_clock::now();
-    if (o->precond < 0 || o->precond > 2) return FRSDG_ERR_INVALID;
     Ctx K{c, s, st, o, t, sigma, u, S};
     if (o->precond >= 1)
         if ((rc = K.build_block_jacobi())) return rc;
@@ -766,4 +916,66 @@ extern "C" int frsdg_adjoint_solve(frsdg_ctx* c, double t, double sigma, const d
     S->converged = S->fnorm <= o->ksp_rtol * S->fnorm0 * 10.0;
     S->seconds   = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return FRSDG_OK;
+}
+
+// The coloured preconditioning matrix at u for F(x) = IFunction(t, x,
+// sigma (T(x) - T(u_n))) (frsdg_color.hpp).
+extern "C" int frsdg_color_jacobian(frsdg_ctx* c, double t, const double* u, const double* u_n, double sigma,
+                                    int method, double* P, void* stream)
+{
+    if (!c || !u || !P || method < 0 || method > 2) return FRSDG_ERR_INVALID;
+    if (!aligned(u) || !aligned(P) || (u_n && !aligned(u_n))) return FRSDG_ERR_ALIGN;
+    int rc = ensure_ws(c, c->ws ? c->ws->m : 1);
+    if (rc) return rc;
+    SolverWs* s = c->ws;
+    if ((rc = color_plan_ensure(c, s))) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const long long N = s->N;
+    const bool cs = c->d.cond_sys != 0;
+    const unsigned gn = static_cast<unsigned>((c->nm.nodes + 255) / 256);
+    const unsigned g  = static_cast<unsigned>((N + 255) / 256);
+    const double* base = u_n ? u_n : u;
+    if (cs)
+    {
+        if (!c->w2) FR_TRY(hipMalloc(&c->w2, sizeof(double) * c->nd));
+        hipLaunchKernelGGL(k_p2c, dim3(gn), dim3(256), 0, st, c->nm, base, s->un);
+    }
+    else
+        FR_TRY(hipMemcpyAsync(s->un, base, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+    frsdg_newton_opts oo{};
+    frsdg_newton_stats ns{};
+    Ctx K{c, s, st, &oo, t, sigma, u, &ns};
+    auto eval_F = [&](const double* x, double* F) -> int
+    {
+        if (cs)
+        {
+            hipLaunchKernelGGL(k_p2c, dim3(gn), dim3(256), 0, st, c->nm, x, c->w2);
+            hipLaunchKernelGGL(k_waxpby, dim3(g), dim3(256), 0, st, N, sigma, c->w2, -sigma, s->un, s->ud);
+        }
+        else
+            hipLaunchKernelGGL(k_waxpby, dim3(g), dim3(256), 0, st, N, sigma, x, -sigma, s->un, s->ud);
+        return frsdg_ifunction(c, t, x, s->ud, F, st);
+    };
+    if ((rc = K.color_assemble(method, eval_F, P))) return rc;
+    FR_TRY(hipStreamSynchronize(st));
+    return FRSDG_OK;
+}
+
+extern "C" int frsdg_ilu0_factor(frsdg_ctx* c, double* P, double* Dinv, void* stream)
+{
+    if (!c || !P || !Dinv) return FRSDG_ERR_INVALID;
+    if (!aligned(P) || !aligned(Dinv)) return FRSDG_ERR_ALIGN;
+    int rc = ensure_ws(c, c->ws ? c->ws->m : 1);
+    if (rc || (rc = color_plan_ensure(c, c->ws))) return rc;
+    return ilu_factor(c->ws, static_cast<hipStream_t>(stream), P, Dinv);
+}
+
+extern "C" int frsdg_ilu0_apply(frsdg_ctx* c, const double* P, const double* Dinv, const double* x, double* z,
+                                void* stream)
+{
+    if (!c || !P || !Dinv || !x || !z) return FRSDG_ERR_INVALID;
+    if (!aligned(P) || !aligned(Dinv) || !aligned(x) || !aligned(z)) return FRSDG_ERR_ALIGN;
+    int rc = ensure_ws(c, c->ws ? c->ws->m : 1);
+    if (rc || (rc = color_plan_ensure(c, c->ws))) return rc;
+    return ilu_apply(c->ws, static_cast<hipStream_t>(stream), P, Dinv, x, z);
 }

@@ -28,6 +28,7 @@ extern "C" {
 #define FRSDG_ERR_HIP 2         /* HIP runtime error */
 #define FRSDG_ERR_UNSUPPORTED 3 /* valid but not implemented on this path */
 #define FRSDG_ERR_ALIGN 4       /* pointer not 16-byte aligned */
+#define FRSDG_ERR_NUMERIC 5     /* zero pivot in a factorisation */
 
 /* boundary types: values of numPDE::BCType (boundary_conditions.hpp:8-16) plus
  * FRSDG_BC_SLIPWALL, the curvilinear slip wall of tests/bump.cpp:250-272 */
@@ -179,7 +180,11 @@ typedef struct frsdg_newton_opts
     int    jac_type;    /* 0 forward tangent, 1 complex step (h = 1e-7) */
     int    precond;     /* 0 none (-pc_type none), 1 element block-Jacobi, 2 red-black
                            element block Gauss-Seidel (one forward sweep, matrix-free
-                           off-diagonal coupling: one extra J*v per application) */
+                           off-diagonal coupling: one extra J*v per application),
+                           3 ILU(0) of the coloured finite-difference matrix (the
+                           reference's preconditioning matrix with PETSc's default
+                           one-process PC), 4 ILU(0) of the same pattern with exact
+                           tangent columns (frsdg_color_jacobian methods 0 and 2) */
     double snes_rtol;   /* stop when ||F|| <= snes_rtol * ||F_0|| (PETSc default 1e-8) */
     double snes_atol;   /* or ||F|| <= snes_atol (PETSc default 1e-50) */
     int    snes_max_it; /* PETSc default 50 */
@@ -197,7 +202,10 @@ typedef struct frsdg_newton_stats
 } frsdg_newton_stats;
 
 /* One backward-Euler step: solves F(u) = (u - u^n)/dt + R(u) = 0 for u^{n+1}.
- * u holds u^n on entry and u^{n+1} on return.  Synchronises the stream. */
+ * u holds u^n on entry and u^{n+1} on return.  With the context's filter or
+ * WENO limiter on, frsdg_apply_stabilization runs on u^{n+1} after the solve,
+ * as the reference's TSSetPostStep / TSSetPostEvaluate callback does once per
+ * implicit step (petsc_solver_impl.hpp:582-596).  Synchronises the stream. */
 int frsdg_beuler_step(frsdg_ctx* ctx, double t, double dt, double* u, const frsdg_newton_opts* opts,
                       frsdg_newton_stats* stats, void* stream);
 
@@ -220,9 +228,41 @@ int frsdg_pseudo_run(frsdg_ctx* ctx, double t, double dt0, double increment, int
                      double t_max, int max_steps, double* u, const frsdg_newton_opts* opts, double* dt_hist,
                      double* fnorm_hist, double* time_hist, int* steps, void* stream);
 
+/* The preconditioning matrix of the reference's implicit path
+ * (IJacobianStrategy::setup_colored_preconditioner and MatFDColoringApply
+ * through ts_coloring_wrapper, petsc_solver_impl.hpp:55-95, 126-160, 221-237):
+ * P = dF/du at u of F(x) = IFunction(t, x, sigma (T(x) - T(u_n))), T the
+ * transient variable (identity, or prim -> cons under cond_sys); u_n NULL
+ * means u_n = u.  The DMDA box stencil of width 1 makes P block sparse:
+ * P holds E * S blocks of nd x nd doubles (S = 9 in 2D, 3 in 1D; nd = DOFs per
+ * element), block (e, slot) = dF_e/du_m for the neighbour m at slot
+ * (dj+1)*3 + (di+1) (1D: di+1), column-major, zero where there is no
+ * neighbour.  method 0: coloured finite differences as MatFDColoringApply
+ * with PETSc's defaults (colouring of DMCreateColoring, step
+ * h = sqrt(eps) sqrt(1 + ||u||_2), "wp"); 1: per-column step
+ * h = sqrt(eps) u_i with |u_i| raised to 100 sqrt(eps) ("ds"); 2: exact
+ * columns from the tangent product, same colouring.  Periodic directions
+ * need a multiple of 3 elements (PETSc's colouring requires it), else
+ * FRSDG_ERR_UNSUPPORTED.  Synchronises the stream. */
+int frsdg_color_jacobian(frsdg_ctx* ctx, double t, const double* u, const double* u_n, double sigma, int method,
+                         double* P, void* stream);
+
+/* ILU(0) of a matrix in the frsdg_color_jacobian layout, in place, natural
+ * element order, no pivoting: PETSc's default PC of a one-process run
+ * (-pc_type ilu, levels 0) on the reference's preconditioning matrix.  Dinv
+ * (E * 2 * nd * nd doubles) receives the inverses of the unit-lower and upper
+ * factors of each diagonal block.  FRSDG_ERR_NUMERIC on a zero pivot (PETSc's
+ * MAT_FACTOR_NUMERIC_ZEROPIVOT).  Synchronises the stream. */
+int frsdg_ilu0_factor(frsdg_ctx* ctx, double* P, double* Dinv, void* stream);
+
+/* z = (L U)^-1 x with the factors of frsdg_ilu0_factor (PCApply of ILU);
+ * z may alias x. */
+int frsdg_ilu0_apply(frsdg_ctx* ctx, const double* P, const double* Dinv, const double* x, double* z,
+                     void* stream);
+
 /* Adjoint linear solve (sigma I + dR/du(u))^T lambda = rhs by GMRES on the
- * adjoint kernel (opts: ksp_* fields, precond, jac_type for the block probes).
- * No reference counterpart.  Synchronises the stream. */
+ * adjoint kernel (opts: ksp_* fields, precond 0..2, jac_type for the block
+ * probes).  No reference counterpart.  Synchronises the stream. */
 int frsdg_adjoint_solve(frsdg_ctx* ctx, double t, double sigma, const double* u, const double* rhs,
                         double* lambda, const frsdg_newton_opts* opts, frsdg_newton_stats* stats,
                         void* stream);

@@ -383,6 +383,29 @@ def test_beuler_jfnk_vs_dense_newton(jac_type, precond):
     assert C.rel_l2(_host(u), ref) <= 1e-9
 
 
+@pytest.mark.parametrize("stab", ["filter", "weno"])
+def test_beuler_applies_post_step_stabilization(stab):
+    """With the filter (or WENO limiter) on, the implicit step is the Newton
+    solve followed by one stabilisation of the new state (TSSetPostStep /
+    TSSetPostEvaluate, petsc_solver_impl.hpp:582-596)."""
+    torch, fr, Oracle = _mods()
+    extra = (dict(apply_filter=1, filt_cutoff=0, filt_strength=8, filt_alpha=36.0) if stab == "filter"
+             else dict(apply_weno=1, shock_threshold=0.0))
+    cfg = dict(C.bump_cfg(4, 2, 3), tangent_bc_mode=1, **extra)
+    ev = fr.PhysicsEvaluator(cfg)
+    o = Oracle(cfg)
+    u0 = o.impose_ic(0, [1.0, 0.5 * 1.4 ** 0.5, 0.0, 1.0])
+    u0 = u0 * (1.0 + 0.02 * np.sin(0.37 * np.arange(u0.size)))
+    dt = 0.05
+    ref = o.filter(dense_beuler_oracle(o, u0, dt))
+    assert C.rel_l2(ref, dense_beuler_oracle(o, u0, dt)) > 1e-6  # the stabilisation changes the state
+    u = _dev(torch, u0)
+    st = ev.beuler_step(u, dt, jac_type=0, precond=1, snes_rtol=1e-10, ksp_rtol=1e-8)
+    print("[newton stab]", stab, st, flush=True)
+    assert st["converged"]
+    assert C.rel_l2(_host(u), ref) <= 1e-9
+
+
 def test_batched_sweep_matches_per_instance_and_oracle():
     """8 instances drawn like the 64-instance sweep (mesh scaled down): the
     persistent batched stage kernels give bitwise the per-instance launches,
