@@ -28,8 +28,9 @@
 // (unpivoted), then U_IJ = L_II^-1 A_IJ for its upper neighbours J.  Rows
 // whose lower neighbours are all factored form a level; levels run as
 // separate launches (i + 2j for a non-periodic 2D mesh: nx + 2 ny - 2 levels).
-// The diagonal blocks' triangular inverses are kept so that both sweeps of
-// the apply are dense block products.
+// The diagonal blocks' triangular inverses are kept: the triangular solves of
+// the factorisation become block products (LDS-tiled, blk_gemm) and both
+// sweeps of the apply are dense block products.
 #pragma once
 #include "../../include/frsdg.h"
 
@@ -64,6 +65,7 @@ namespace frsdg
         double* v    = nullptr; // perturbed state / unit colour vector
         double* f0   = nullptr; // F at the base state
         double* f1   = nullptr; // F at the perturbed state
+        double* T    = nullptr; // factorisation scratch: one block per row of the widest level
     };
 
     // Neighbour tables, colour classes and the ILU(0) level schedule.
@@ -235,53 +237,257 @@ namespace frsdg
 
     // ---- block ILU(0) -----------------------------------------------------------
 
-    // One block row per workgroup, rows of one level per launch.
-    static __global__ __launch_bounds__(256) void k_ilu_row(int S, int nd, const int* __restrict__ nbr,
-                                                            const int* __restrict__ low, const int* __restrict__ upd,
-                                                            const int* __restrict__ upp, const int* __restrict__ lvl,
-                                                            int off, double* P, double* __restrict__ Dinv,
-                                                            int* __restrict__ info)
+    // C = A B (sub = false) or C -= A B (sub = true) for nd x nd column-major
+    // blocks, by the whole workgroup (256 threads): output tiles of 144 x 144,
+    // each thread a 9 x 9 register tile (rows tx + 16 i, columns ty + 16 j),
+    // k in slabs of 16 staged through LDS, the next slab's loads in flight
+    // while the current one is multiplied.  C must not alias A or B.
+    constexpr int GT = 144, GKB = 16, GST = GT * GKB / 256; // 9 staged values of A and of B per thread
+    __device__ inline void gemm_fetch(int nd, const double* __restrict__ A, const double* __restrict__ B, int r0,
+                                      int c0, int k0, double (&ra)[GST], double (&rb)[GST])
     {
-        const long long I   = lvl[off + blockIdx.x];
-        const long long bsz = static_cast<long long>(nd) * nd;
-        const int       C   = S / 2; // centre slot
-        const int       tid = threadIdx.x;
-        auto blk = [&](long long e, int s) { return P + (e * S + s) * bsz; };
-        for (int q = 0; q < CMAXN; ++q)
+#pragma unroll
+        for (int m = 0; m < GST; ++m)
         {
-            const int sK = low[I * CMAXN + q];
-            if (sK < 0) break;
-            const long long K = nbr[I * S + sK];
-            double*       A = blk(I, sK);
-            const double* U = blk(K, C);
-            // L_IK = A_IK U_KK^-1, each row by forward substitution over the columns
-            for (int r = tid; r < nd; r += blockDim.x)
-                for (int c = 0; c < nd; ++c)
-                {
-                    double acc = A[r + static_cast<long long>(c) * nd];
-                    for (int k = 0; k < c; ++k) acc -= A[r + static_cast<long long>(k) * nd] * U[k + static_cast<long long>(c) * nd];
-                    A[r + static_cast<long long>(c) * nd] = acc / U[c + static_cast<long long>(c) * nd];
-                }
-            __syncthreads();
-            for (int p = 0; p < CMAXP; ++p)
-            {
-                const long long at = ((I * CMAXN + q) * CMAXP + p) * 2;
-                const int sj = upd[at];
-                if (sj < 0) break;
-                double*       B  = blk(I, sj);
-                const double* UK = blk(K, upd[at + 1]);
-                for (long long o = tid; o < bsz; o += blockDim.x)
-                {
-                    const int r = static_cast<int>(o % nd), c = static_cast<int>(o / nd);
-                    double acc = B[o];
-                    for (int k = 0; k < nd; ++k) acc -= A[r + static_cast<long long>(k) * nd] * UK[k + static_cast<long long>(c) * nd];
-                    B[o] = acc;
-                }
-                __syncthreads();
-            }
+            const int o  = threadIdx.x + 256 * m;
+            const int kk = o / GT, rr = o - kk * GT; // A: rows contiguous per k
+            const int r = r0 + rr, k = k0 + kk;
+            ra[m] = (r < nd && k < nd) ? A[r + static_cast<long long>(k) * nd] : 0.0;
+            const int cc = o / GKB, kb = o - cc * GKB; // B: k contiguous per column
+            const int c = c0 + cc, k2 = k0 + kb;
+            rb[m] = (c < nd && k2 < nd) ? B[k2 + static_cast<long long>(c) * nd] : 0.0;
         }
-        // A_II = L_II U_II, unpivoted, right-looking
-        double* D = blk(I, C);
+    }
+    __device__ inline void blk_gemm(int nd, const double* __restrict__ A, const double* __restrict__ B,
+                                    double* __restrict__ C, bool sub, double* __restrict__ As,
+                                    double* __restrict__ Bs)
+    {
+        const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+        for (int r0 = 0; r0 < nd; r0 += GT)
+            for (int c0 = 0; c0 < nd; c0 += GT)
+            {
+                double acc[9][9];
+#pragma unroll
+                for (int i = 0; i < 9; ++i)
+#pragma unroll
+                    for (int j = 0; j < 9; ++j) acc[i][j] = 0.0;
+                double ra[GST], rb[GST];
+                gemm_fetch(nd, A, B, r0, c0, 0, ra, rb);
+                for (int k0 = 0; k0 < nd; k0 += GKB)
+                {
+#pragma unroll
+                    for (int m = 0; m < GST; ++m)
+                    {
+                        const int o = tid + 256 * m;
+                        const int kk = o / GT, cc = o / GKB;
+                        As[kk * GT + (o - kk * GT)] = ra[m];
+                        Bs[(o - cc * GKB) * GT + cc] = rb[m];
+                    }
+                    __syncthreads();
+                    if (k0 + GKB < nd) gemm_fetch(nd, A, B, r0, c0, k0 + GKB, ra, rb);
+#pragma unroll 2
+                    for (int kk = 0; kk < GKB; ++kk)
+                    {
+                        double av[9], bv[9];
+#pragma unroll
+                        for (int i = 0; i < 9; ++i) av[i] = As[kk * GT + tx + 16 * i];
+#pragma unroll
+                        for (int j = 0; j < 9; ++j) bv[j] = Bs[kk * GT + ty + 16 * j];
+#pragma unroll
+                        for (int i = 0; i < 9; ++i)
+#pragma unroll
+                            for (int j = 0; j < 9; ++j) acc[i][j] = ::fma(av[i], bv[j], acc[i][j]);
+                    }
+                    __syncthreads();
+                }
+#pragma unroll
+                for (int j = 0; j < 9; ++j)
+#pragma unroll
+                    for (int i = 0; i < 9; ++i)
+                    {
+                        const int r = r0 + tx + 16 * i, c = c0 + ty + 16 * j;
+                        if (r < nd && c < nd)
+                        {
+                            const long long at = r + static_cast<long long>(c) * nd;
+                            C[at] = sub ? C[at] - acc[i][j] : acc[i][j];
+                        }
+                    }
+            }
+    }
+
+    __device__ inline void blk_copy(long long bsz, const double* __restrict__ src, double* __restrict__ dst)
+    {
+        constexpr int U = 8; // loads in flight per thread
+        long long o = threadIdx.x;
+        for (; o + (U - 1) * 256 < bsz; o += U * 256)
+        {
+            double v[U];
+#pragma unroll
+            for (int m = 0; m < U; ++m) v[m] = src[o + 256 * m];
+#pragma unroll
+            for (int m = 0; m < U; ++m) dst[o + 256 * m] = v[m];
+        }
+        for (; o < bsz; o += 256) dst[o] = src[o];
+    }
+
+    // A_II = L_II U_II (unpivoted, right-looking) and the inverses of its unit-
+    // lower and upper factors, for nd <= 144: the block lives in registers,
+    // thread (tx, ty) holding rows tx + 16 i and columns ty + 16 j, and each
+    // elimination step broadcasts the pivot row and column through LDS.
+    __device__ inline void diag_lu_inv_reg(int nd, double* __restrict__ D, double* __restrict__ Li,
+                                           double* __restrict__ Ui, double* __restrict__ bufA,
+                                           double* __restrict__ bufB, int* __restrict__ info)
+    {
+        const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+        double x[9][9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+#pragma unroll
+            for (int j = 0; j < 9; ++j)
+            {
+                const int r = tx + 16 * i, c = ty + 16 * j;
+                x[i][j] = (r < nd && c < nd) ? D[r + static_cast<long long>(c) * nd] : 0.0;
+            }
+        double* rowk = bufA; // pivot row k
+        double* colk = bufB; // column k below the pivot, then the multipliers
+        for (int k = 0; k < nd; ++k)
+        {
+            const int kt = k & 15, kb = k >> 4;
+            if (tx == kt)
+#pragma unroll
+                for (int i = 0; i < 9; ++i)
+                    if (i == kb)
+#pragma unroll
+                        for (int j = 0; j < 9; ++j) rowk[ty + 16 * j] = x[i][j];
+            if (ty == kt)
+#pragma unroll
+                for (int j = 0; j < 9; ++j)
+                    if (j == kb)
+#pragma unroll
+                        for (int i = 0; i < 9; ++i) colk[tx + 16 * i] = x[i][j];
+            __syncthreads();
+            const double piv = rowk[k];
+            if (tid == 0 && ::fabs(piv) <= 1e-12) info[0] = 1; // PETSc's default zero-pivot tolerance
+            if (tid > k && tid < nd) colk[tid] /= piv;
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+            {
+                const int r = tx + 16 * i;
+                if (r <= k || r >= nd) continue;
+                const double l = colk[r];
+#pragma unroll
+                for (int j = 0; j < 9; ++j)
+                {
+                    const int c = ty + 16 * j;
+                    if (c == k)
+                        x[i][j] = l;
+                    else if (c > k)
+                        x[i][j] = ::fma(-l, rowk[c], x[i][j]);
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+#pragma unroll
+            for (int j = 0; j < 9; ++j)
+            {
+                const int r = tx + 16 * i, c = ty + 16 * j;
+                if (r < nd && c < nd) D[r + static_cast<long long>(c) * nd] = x[i][j];
+            }
+        __syncthreads();
+        // L^-1: X = I; for k ascending, rows r > k: X_r -= L_rk X_k
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+#pragma unroll
+            for (int j = 0; j < 9; ++j) x[i][j] = (tx + 16 * i == ty + 16 * j) ? 1.0 : 0.0;
+        double nxt = (tid < nd) ? D[tid] : 0.0; // column k of D, loaded one step ahead
+        for (int k = 0; k < nd - 1; ++k)
+        {
+            const int kt = k & 15, kb = k >> 4;
+            if (tx == kt)
+#pragma unroll
+                for (int i = 0; i < 9; ++i)
+                    if (i == kb)
+#pragma unroll
+                        for (int j = 0; j < 9; ++j) rowk[ty + 16 * j] = x[i][j];
+            if (tid > k && tid < nd) colk[tid] = nxt;
+            if (tid < nd) nxt = D[tid + static_cast<long long>(k + 1) * nd];
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+            {
+                const int r = tx + 16 * i;
+                if (r <= k || r >= nd) continue;
+                const double l = colk[r];
+#pragma unroll
+                for (int j = 0; j < 9; ++j)
+                    if (ty + 16 * j <= k) x[i][j] = ::fma(-l, rowk[ty + 16 * j], x[i][j]);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+#pragma unroll
+            for (int j = 0; j < 9; ++j)
+            {
+                const int r = tx + 16 * i, c = ty + 16 * j;
+                if (r < nd && c < nd) Li[r + static_cast<long long>(c) * nd] = x[i][j];
+            }
+        // U^-1: X = I; for k descending, X_k /= U_kk, rows r < k: X_r -= U_rk X_k
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+#pragma unroll
+            for (int j = 0; j < 9; ++j) x[i][j] = (tx + 16 * i == ty + 16 * j) ? 1.0 : 0.0;
+        nxt = (tid < nd) ? D[tid + static_cast<long long>(nd - 1) * nd] : 0.0;
+        for (int k = nd - 1; k >= 0; --k)
+        {
+            const int kt = k & 15, kb = k >> 4;
+            if (tid < nd) colk[tid] = nxt; // column k of U (the diagonal at colk[k])
+            if (k > 0 && tid < nd) nxt = D[tid + static_cast<long long>(k - 1) * nd];
+            __syncthreads();
+            const double ukk = colk[k];
+            if (tx == kt)
+#pragma unroll
+                for (int i = 0; i < 9; ++i)
+                    if (i == kb)
+#pragma unroll
+                        for (int j = 0; j < 9; ++j)
+                        {
+                            x[i][j] /= ukk;
+                            rowk[ty + 16 * j] = x[i][j];
+                        }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+            {
+                const int r = tx + 16 * i;
+                if (r >= k) continue;
+                const double u = colk[r];
+#pragma unroll
+                for (int j = 0; j < 9; ++j)
+                    if (ty + 16 * j >= k) x[i][j] = ::fma(-u, rowk[ty + 16 * j], x[i][j]);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+#pragma unroll
+            for (int j = 0; j < 9; ++j)
+            {
+                const int r = tx + 16 * i, c = ty + 16 * j;
+                if (r < nd && c < nd) Ui[r + static_cast<long long>(c) * nd] = x[i][j];
+            }
+        __syncthreads();
+    }
+
+    // A_II = L_II U_II and the inverses of its factors for blocks larger than
+    // 144,, in global memory (one column per thread for
+    // the inverses).
+    __device__ inline void diag_lu_inv_global(int nd, double* D, double* Li, double* Ui, int* info)
+    {
+        const int tid = threadIdx.x;
         for (int k = 0; k < nd; ++k)
         {
             const double piv = D[k + static_cast<long long>(k) * nd];
@@ -296,23 +502,6 @@ namespace frsdg
             }
             __syncthreads();
         }
-        // U_IJ = L_II^-1 A_IJ, each column by forward substitution
-        for (int q = 0; q < CMAXN; ++q)
-        {
-            const int sJ = upp[I * CMAXN + q];
-            if (sJ < 0) break;
-            double* B = blk(I, sJ);
-            for (int c = tid; c < nd; c += blockDim.x)
-                for (int r = 1; r < nd; ++r)
-                {
-                    double acc = B[r + static_cast<long long>(c) * nd];
-                    for (int k = 0; k < r; ++k) acc -= D[r + static_cast<long long>(k) * nd] * B[k + static_cast<long long>(c) * nd];
-                    B[r + static_cast<long long>(c) * nd] = acc;
-                }
-        }
-        // triangular inverses of the diagonal block, one column per thread
-        double* Li = Dinv + I * 2 * bsz;
-        double* Ui = Li + bsz;
         for (int c = tid; c < nd; c += blockDim.x)
         {
             const long long cc = static_cast<long long>(c) * nd;
@@ -321,6 +510,7 @@ namespace frsdg
             for (int r = c + 1; r < nd; ++r)
             {
                 double acc = 0.0;
+#pragma unroll 8
                 for (int k = c; k < r; ++k) acc -= D[r + static_cast<long long>(k) * nd] * Li[k + cc];
                 Li[r + cc] = acc;
             }
@@ -329,79 +519,171 @@ namespace frsdg
             for (int r = c - 1; r >= 0; --r)
             {
                 double acc = 0.0;
+#pragma unroll 8
                 for (int k = r + 1; k <= c; ++k) acc -= D[r + static_cast<long long>(k) * nd] * Ui[k + cc];
                 Ui[r + cc] = acc / D[r + static_cast<long long>(r) * nd];
             }
+        }
+        __syncthreads();
+    }
+
+    // One block row per workgroup, rows of one level per launch.  The
+    // triangular solves with factored diagonal blocks are products with their
+    // stored inverses: L_IK = A_IK U_KK^-1 and U_IJ = L_II^-1 A_IJ.  T is
+    // scratch of one block per workgroup of the launch.
+    static __global__ __launch_bounds__(256) void k_ilu_row(int S, int nd, const int* __restrict__ nbr,
+                                                            const int* __restrict__ low, const int* __restrict__ upd,
+                                                            const int* __restrict__ upp, const int* __restrict__ lvl,
+                                                            int off, double* P, double* Dinv, double* T,
+                                                            int* __restrict__ info)
+    {
+        __shared__ double As[GT * GKB], Bs[GT * GKB];
+        const long long I   = lvl[off + blockIdx.x];
+        const long long bsz = static_cast<long long>(nd) * nd;
+        const int       C   = S / 2; // centre slot
+        const int       tid = threadIdx.x;
+        double*         Tw  = T + static_cast<long long>(blockIdx.x) * bsz;
+        auto blk = [&](long long e, int s) { return P + (e * S + s) * bsz; };
+        for (int q = 0; q < CMAXN; ++q)
+        {
+            const int sK = low[I * CMAXN + q];
+            if (sK < 0) break;
+            const long long K = nbr[I * S + sK];
+            double* A = blk(I, sK);
+            blk_copy(bsz, A, Tw);
+            __syncthreads();
+            blk_gemm(nd, Tw, Dinv + K * 2 * bsz + bsz, A, false, As, Bs); // L_IK = A_IK U_KK^-1
+            __syncthreads();
+            for (int p = 0; p < CMAXP; ++p)
+            {
+                const long long at = ((I * CMAXN + q) * CMAXP + p) * 2;
+                const int sj = upd[at];
+                if (sj < 0) break;
+                blk_gemm(nd, A, blk(K, upd[at + 1]), blk(I, sj), true, As, Bs); // A_IJ -= L_IK U_KJ
+                __syncthreads();
+            }
+        }
+        double* D  = blk(I, C);
+        double* Li = Dinv + I * 2 * bsz;
+        double* Ui = Li + bsz;
+        if (nd <= GT)
+            diag_lu_inv_reg(nd, D, Li, Ui, As, Bs, info);
+        else
+            diag_lu_inv_global(nd, D, Li, Ui, info);
+        // U_IJ = L_II^-1 A_IJ
+        for (int q = 0; q < CMAXN; ++q)
+        {
+            const int sJ = upp[I * CMAXN + q];
+            if (sJ < 0) break;
+            double* B = blk(I, sJ);
+            blk_copy(bsz, B, Tw);
+            __syncthreads();
+            blk_gemm(nd, Li, Tw, B, false, As, Bs);
+            __syncthreads();
         }
     }
 
     constexpr int CMAXND = 336; // largest element block (2D p = 8: 324 DOFs)
 
-    // forward sweep of one level: y_I = L_II^-1 (x_I - sum_K L_IK y_K)
-    static __global__ __launch_bounds__(256) void k_ilu_fwd(int S, int nd, const int* __restrict__ nbr,
-                                                            const int* __restrict__ low, const int* __restrict__ lvl,
-                                                            int off, const double* __restrict__ P,
-                                                            const double* __restrict__ Dinv,
-                                                            const double* __restrict__ x, double* y)
+    // One level of the forward (FWD: out_I = L_II^-1 (in_I - sum_K L_IK out_K),
+    // K below) or backward (out_I = U_II^-1 (in_I - sum_J U_IJ out_J), J above)
+    // sweep.  One element per workgroup of 16 waves: wave w takes columns
+    // [w cb, (w+1) cb) of every block, lanes take rows, and the 16 partial
+    // sums meet in LDS.  The sweeps are latency-bound (a level holds few
+    // elements), so the block size is a template parameter (0: run-time nd)
+    // and a wave's loads of one block are all issued before their products.
+    constexpr int SWT = 1024, SWW = SWT / 64, SWR = (CMAXND + 63) / 64;
+    template <int ND>
+    __device__ inline void sweep_block(int nd, int c_lo, int c_hi, int lane, const double* __restrict__ B,
+                                       const double* __restrict__ v, double (&part)[SWR])
     {
-        __shared__ double t[CMAXND];
-        const long long I   = lvl[off + blockIdx.x];
-        const long long bsz = static_cast<long long>(nd) * nd;
-        for (int r = threadIdx.x; r < nd; r += blockDim.x)
+        constexpr int CB  = ND ? (ND + SWW - 1) / SWW : 0;
+        constexpr int RPL = ND ? (ND + 63) / 64 : SWR;
+        if constexpr (ND > 0)
         {
-            double acc = x[I * nd + r];
-            for (int q = 0; q < CMAXN; ++q)
+#pragma unroll
+            for (int cc = 0; cc < CB; ++cc)
             {
-                const int sK = low[I * CMAXN + q];
-                if (sK < 0) break;
-                const long long K  = nbr[I * S + sK];
-                const double*   L  = P + (I * S + sK) * bsz;
-                const double*   yk = y + K * nd;
-                for (int c = 0; c < nd; ++c) acc -= L[r + static_cast<long long>(c) * nd] * yk[c];
+                const int  c  = c_lo + cc;
+                const bool ok = c < c_hi;
+                const double vc = ok ? v[c] : 0.0;
+#pragma unroll
+                for (int rr = 0; rr < RPL; ++rr)
+                {
+                    const int r = lane + 64 * rr;
+                    if (ok && r < ND) part[rr] = ::fma(B[r + c * ND], vc, part[rr]);
+                }
             }
-            t[r] = acc;
         }
-        __syncthreads();
-        const double* Li = Dinv + I * 2 * bsz;
-        for (int r = threadIdx.x; r < nd; r += blockDim.x)
+        else
         {
-            double acc = 0.0;
-            for (int c = 0; c <= r; ++c) acc += Li[r + static_cast<long long>(c) * nd] * t[c];
-            y[I * nd + r] = acc;
+            for (int c = c_lo; c < c_hi; ++c)
+            {
+                const double vc = v[c];
+#pragma unroll
+                for (int rr = 0; rr < RPL; ++rr)
+                {
+                    const int r = lane + 64 * rr;
+                    if (r < nd) part[rr] = ::fma(B[r + static_cast<long long>(c) * nd], vc, part[rr]);
+                }
+            }
         }
     }
 
-    // backward sweep of one level: z_I = U_II^-1 (y_I - sum_J U_IJ z_J)
-    static __global__ __launch_bounds__(256) void k_ilu_bwd(int S, int nd, const int* __restrict__ nbr,
-                                                            const int* __restrict__ upp, const int* __restrict__ lvl,
-                                                            int off, const double* __restrict__ P,
-                                                            const double* __restrict__ Dinv,
-                                                            const double* __restrict__ y, double* z)
+    template <bool FWD, int ND>
+    __global__ __launch_bounds__(SWT) void k_ilu_sweep(int S, int nd, const int* __restrict__ nbr,
+                                                       const int* __restrict__ list, const int* __restrict__ lvl,
+                                                       int off, const double* __restrict__ P,
+                                                       const double* __restrict__ Dinv,
+                                                       const double* __restrict__ in, double* out)
     {
+        __shared__ double red[SWW][CMAXND];
         __shared__ double t[CMAXND];
-        const long long I   = lvl[off + blockIdx.x];
-        const long long bsz = static_cast<long long>(nd) * nd;
-        for (int r = threadIdx.x; r < nd; r += blockDim.x)
+        if constexpr (ND > 0) nd = ND;
+        const long long I    = lvl[off + blockIdx.x];
+        const long long bsz  = static_cast<long long>(nd) * nd;
+        const int       lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        const int       cb   = (nd + SWW - 1) / SWW;
+        const int       c_lo = w * cb, c_hi = min(nd, c_lo + cb);
+        double part[SWR];
+#pragma unroll
+        for (int rr = 0; rr < SWR; ++rr) part[rr] = 0.0;
+        for (int q = 0; q < CMAXN; ++q)
         {
-            double acc = y[I * nd + r];
-            for (int q = 0; q < CMAXN; ++q)
-            {
-                const int sJ = upp[I * CMAXN + q];
-                if (sJ < 0) break;
-                const long long J  = nbr[I * S + sJ];
-                const double*   U  = P + (I * S + sJ) * bsz;
-                const double*   zj = z + J * nd;
-                for (int c = 0; c < nd; ++c) acc -= U[r + static_cast<long long>(c) * nd] * zj[c];
-            }
+            const int s = list[I * CMAXN + q];
+            if (s < 0) break;
+            sweep_block<ND>(nd, c_lo, c_hi, lane, P + (I * S + s) * bsz, out + static_cast<long long>(nbr[I * S + s]) * nd,
+                            part);
+        }
+#pragma unroll
+        for (int rr = 0; rr < SWR; ++rr)
+        {
+            const int r = lane + 64 * rr;
+            if (r < nd) red[w][r] = part[rr];
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r < nd; r += SWT)
+        {
+            double acc = in[I * nd + r];
+            for (int k = 0; k < SWW; ++k) acc -= red[k][r];
             t[r] = acc;
         }
         __syncthreads();
-        const double* Ui = Dinv + I * 2 * bsz + bsz;
-        for (int r = threadIdx.x; r < nd; r += blockDim.x)
+#pragma unroll
+        for (int rr = 0; rr < SWR; ++rr) part[rr] = 0.0;
+        sweep_block<ND>(nd, c_lo, c_hi, lane, Dinv + I * 2 * bsz + (FWD ? 0 : bsz), t, part);
+#pragma unroll
+        for (int rr = 0; rr < SWR; ++rr)
+        {
+            const int r = lane + 64 * rr;
+            if (r < nd) red[w][r] = part[rr];
+        }
+        __syncthreads();
+        for (int r = threadIdx.x; r < nd; r += SWT)
         {
             double acc = 0.0;
-            for (int c = r; c < nd; ++c) acc += Ui[r + static_cast<long long>(c) * nd] * t[c];
-            z[I * nd + r] = acc;
+            for (int k = 0; k < SWW; ++k) acc += red[k][r];
+            out[I * nd + r] = acc;
         }
     }
 } // namespace frsdg

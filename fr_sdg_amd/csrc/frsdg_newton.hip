@@ -60,7 +60,7 @@ namespace frsdg
         {
             for (int* p : {cp->nbr, cp->cls, cp->low, cp->upd, cp->upp, cp->lvl, cp->info})
                 if (p) (void) hipFree(p);
-            for (double* p : {cp->P, cp->Dinv, cp->y, cp->v, cp->f0, cp->f1})
+            for (double* p : {cp->P, cp->Dinv, cp->y, cp->v, cp->f0, cp->f1, cp->T})
                 if (p) (void) hipFree(p);
             delete cp;
         }
@@ -368,6 +368,9 @@ namespace
         const long long bsz = static_cast<long long>(nd) * nd;
         FR_TRY(hipMalloc(&cp->P, sizeof(double) * cp->E * cp->S * bsz));
         FR_TRY(hipMalloc(&cp->Dinv, sizeof(double) * cp->E * 2 * bsz));
+        int width = 1;
+        for (size_t l = 0; l + 1 < cp->lvl_ptr.size(); ++l) width = std::max(width, cp->lvl_ptr[l + 1] - cp->lvl_ptr[l]);
+        FR_TRY(hipMalloc(&cp->T, sizeof(double) * width * bsz));
         for (double** p : {&cp->y, &cp->v, &cp->f0, &cp->f1}) FR_TRY(hipMalloc(p, sizeof(double) * s->N));
         return FRSDG_OK;
     }
@@ -381,7 +384,7 @@ namespace
         for (int l = 0; l < nlev; ++l)
             hipLaunchKernelGGL(frsdg::k_ilu_row, dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l])),
                                dim3(256), 0, st, cp.S, cp.nd, cp.nbr, cp.low, cp.upd, cp.upp, cp.lvl, cp.lvl_ptr[l], P,
-                               Dinv, cp.info);
+                               Dinv, cp.T, cp.info);
         FR_TRY(hipGetLastError());
         int info = 0;
         FR_TRY(hipMemcpyAsync(&info, cp.info, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -390,16 +393,36 @@ namespace
     }
 
     // z = (L U)^-1 x: forward sweep over the levels, then backward (z may alias x).
+    template <int ND>
+    void ilu_sweeps(const frsdg::ColorPlan& cp, hipStream_t st, const double* P, const double* Dinv, const double* x,
+                    double* z)
+    {
+        const int nlev = static_cast<int>(cp.lvl_ptr.size()) - 1;
+        for (int l = 0; l < nlev; ++l)
+            hipLaunchKernelGGL((frsdg::k_ilu_sweep<true, ND>), dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l])),
+                               dim3(frsdg::SWT), 0, st, cp.S, cp.nd, cp.nbr, cp.low, cp.lvl, cp.lvl_ptr[l], P, Dinv, x,
+                               cp.y);
+        for (int l = nlev - 1; l >= 0; --l)
+            hipLaunchKernelGGL((frsdg::k_ilu_sweep<false, ND>), dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l])),
+                               dim3(frsdg::SWT), 0, st, cp.S, cp.nd, cp.nbr, cp.upp, cp.lvl, cp.lvl_ptr[l], P, Dinv,
+                               cp.y, z);
+    }
     int ilu_apply(SolverWs* s, hipStream_t st, const double* P, const double* Dinv, const double* x, double* z)
     {
         const frsdg::ColorPlan& cp = *s->cp;
-        const int nlev = static_cast<int>(cp.lvl_ptr.size()) - 1;
-        for (int l = 0; l < nlev; ++l)
-            hipLaunchKernelGGL(frsdg::k_ilu_fwd, dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l])),
-                               dim3(256), 0, st, cp.S, cp.nd, cp.nbr, cp.low, cp.lvl, cp.lvl_ptr[l], P, Dinv, x, cp.y);
-        for (int l = nlev - 1; l >= 0; --l)
-            hipLaunchKernelGGL(frsdg::k_ilu_bwd, dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l])),
-                               dim3(256), 0, st, cp.S, cp.nd, cp.nbr, cp.upp, cp.lvl, cp.lvl_ptr[l], P, Dinv, cp.y, z);
+        switch (cp.nd)
+        {
+            // 2D element blocks 4 (p+1)^2, 1D 3 (p+1)
+            case 16: ilu_sweeps<16>(cp, st, P, Dinv, x, z); break;
+            case 36: ilu_sweeps<36>(cp, st, P, Dinv, x, z); break;
+            case 64: ilu_sweeps<64>(cp, st, P, Dinv, x, z); break;
+            case 100: ilu_sweeps<100>(cp, st, P, Dinv, x, z); break;
+            case 144: ilu_sweeps<144>(cp, st, P, Dinv, x, z); break;
+            case 196: ilu_sweeps<196>(cp, st, P, Dinv, x, z); break;
+            case 256: ilu_sweeps<256>(cp, st, P, Dinv, x, z); break;
+            case 324: ilu_sweeps<324>(cp, st, P, Dinv, x, z); break;
+            default: ilu_sweeps<0>(cp, st, P, Dinv, x, z); break;
+        }
         return hip_rc(hipGetLastError());
     }
 
