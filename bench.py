@@ -39,7 +39,10 @@ def parse():
     ap.add_argument("--no-newton", action="store_true", help="skip the implicit (JFNK) comparison")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep comparison")
     ap.add_argument("--newton-pseudo", action="store_true",
-                    help="add the pseudo-transient tangent vs complex-step comparison (about 45 s)")
+                    help="add the pseudo-transient tangent vs complex-step comparison with ILU(0) of the "
+                         "coloured matrix (about 2 minutes)")
+    ap.add_argument("--ilu", action="store_true",
+                    help="add the coloured-matrix / ILU(0) preconditioner timings on the bump (about 30 s)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--adjoint-large", action="store_true",
                     help="run only the unsteady adjoint of int(rho^2) on the 1024^2 p=7 warped mesh")
@@ -189,6 +192,11 @@ def main():
             line["newton"] = bench_newton(torch, fr)
         except Exception as exc:
             line["newton"] = {"error": repr(exc)}
+        if args.ilu:
+            try:
+                line["ilu"] = bench_ilu(torch, fr)
+            except Exception as exc:
+                line["ilu"] = {"error": repr(exc)}
         if args.newton_pseudo:
             try:
                 line["newton_pseudo"] = bench_newton_pseudo(torch, fr)
@@ -293,11 +301,86 @@ def bench_large(torch, fr):
     return res
 
 
+def ilu_flops(nx, ny, nd):
+    """FP64 flops of one block ILU(0) factorisation on a non-periodic nx x ny
+    box-stencil block pattern: per block row, one product with U_KK^-1 per
+    lower neighbour K, one update product per (K, J) pair, one product with
+    L_II^-1 per upper neighbour (2 nd^3 each), the unpivoted LU (2/3 nd^3) and
+    the two triangular inverses (2/3 nd^3)."""
+    def nb(i, j):
+        return [(i + di, j + dj) for dj in (-1, 0, 1) for di in (-1, 0, 1)
+                if 0 <= i + di < nx and 0 <= j + dj < ny]
+    idx = lambda e: e[1] * nx + e[0]
+    gemms = 0
+    for j in range(ny):
+        for i in range(nx):
+            me = idx((i, j))
+            pat = {idx(e) for e in nb(i, j)}
+            for K in (e for e in nb(i, j) if idx(e) < me):
+                gemms += 1 + len([J for J in pat if J > idx(K) and J in {idx(f) for f in nb(*K)}])
+            gemms += len([J for J in pat if J > me])
+    return gemms * 2.0 * nd ** 3 + nx * ny * (4.0 / 3.0) * nd ** 3
+
+
+def bench_ilu(torch, fr):
+    """The reference's preconditioning matrix and its ILU(0) on the bump
+    256x64 p=5 (frsdg_color_jacobian: DMDA box colouring, 9 x 144 coloured
+    residual evaluations; frsdg_ilu0_factor / frsdg_ilu0_apply): wall times,
+    the factorisation's FP64 rate and the apply's HBM rate (it reads every
+    stored block once: E*9 blocks of P plus 2 E inverse blocks), and the
+    GMRES iterations one CFL-10 backward-Euler Newton iteration from free
+    stream needs to reach 1e-3 with each preconditioner (GMRES(60), at most
+    3000 iterations)."""
+    from fr_sdg_amd import configs as CF
+    cfg = CF.bump(tangent_bc_mode=0)
+    ev = fr.PhysicsEvaluator(cfg)
+    u0 = ev.impose_ic(fr.IC_UNIFORM, CF.bump_freestream())
+    h = min(cfg["lx"] / cfg["nx"], cfg["ly"] / cfg["ny"])
+    sigma = CF.sound_speed_max(u0, cfg) / (10.0 * h)
+    E, S, nd = ev.color_layout()
+
+    def wall(f, n=1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            r = f()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n, r
+
+    out = {"workload": "bump 256x64 p=5 HLLC (2.36 M DOFs), P at free stream, sigma = 1/dt at CFL 10",
+           "blocks": E * S, "block": [nd, nd], "P_GB": E * S * nd * nd * 8 / 1e9}
+    t_fd, _ = wall(lambda: ev.color_jacobian(u0, sigma, None, method=0))
+    t_ex, P = wall(lambda: ev.color_jacobian(u0, sigma, None, method=2))
+    t_f, Dinv = wall(lambda: ev.ilu0_factor(P))
+    x = torch.randn(ev.ndofs, dtype=torch.float64, device="cuda")
+    ev.ilu0_apply(P, Dinv, x)
+    t_a, _ = wall(lambda: ev.ilu0_apply(P, Dinv, x), 5)
+    fl = ilu_flops(cfg["nx"], cfg["ny"], nd)
+    by = (E * S + 2 * E) * nd * nd * 8.0 + 4.0 * ev.ndofs * 8.0
+    out.update({"assemble_fd_s": t_fd, "assemble_exact_s": t_ex, "factor_s": t_f, "factor_gflop": fl / 1e9,
+                "factor_tflops": fl / t_f / 1e12, "factor_frac_fp64_peak": fl / t_f / 78.6e12,
+                "apply_ms": t_a * 1e3, "apply_GBps": by / t_a / 1e9, "apply_frac_hbm": by / t_a / 8.0e12,
+                "levels": cfg["nx"] + 2 * cfg["ny"] - 2})
+    del P, Dinv
+    torch.cuda.empty_cache()
+    its = {}
+    for name, pc in (("block_jacobi", 1), ("gauss_seidel", 2), ("ilu_fd", 3), ("ilu_exact", 4)):
+        u = u0.clone()
+        t, st = wall(lambda: ev.beuler_step(u, 1.0 / sigma, jac_type=0, precond=pc, ksp_restart=60,
+                                            ksp_max_it=3000, ksp_rtol=1e-3, snes_rtol=1e-3, snes_max_it=1))
+        its[name] = {"gmres_its": st["ksp_its"], "s": t}
+    out["one_newton_iteration"] = its
+    del ev, u0
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_newton_pseudo(torch, fr, steps=12):
     """The bump 256x64 p=5 driven to steady state by pseudo-transient
     continuation (frsdg_pseudo_run: SER step rule from CFL 1, x1.5 per step,
-    one backward-Euler JFNK solve per step: GMRES(60) to 1e-3 with element
-    block-Jacobi, <= 3 Newton iterations to 1e-3).  The tangent kernel
+    one backward-Euler JFNK solve per step: GMRES(60) to 1e-3 preconditioned
+    by ILU(0) of the coloured finite-difference matrix, as the reference's
+    PETSc setup does, <= 3 Newton iterations to 1e-3).  The tangent kernel
     linearises the wall and Dirichlet boundary states exactly; the complex
     step sees them frozen, as the reference's lambda BCs are.  Reported: the
     steady-residual history of both and the time each takes to reach the
@@ -309,7 +392,7 @@ def bench_newton_pseudo(torch, fr, steps=12):
     u0 = ev.impose_ic(fr.IC_UNIFORM, CF.bump_freestream())
     h = min(cfg["lx"] / cfg["nx"], cfg["ly"] / cfg["ny"])
     dt0 = 1.0 * h / CF.sound_speed_max(u0, cfg)
-    opts = dict(precond=1, ksp_restart=60, ksp_max_it=1000, ksp_rtol=1e-3, snes_rtol=1e-3, snes_max_it=3)
+    opts = dict(precond=3, ksp_restart=60, ksp_max_it=1000, ksp_rtol=1e-3, snes_rtol=1e-3, snes_max_it=3)
     runs = {}
     for name, jt in (("tangent", 0), ("complex_step", 1)):
         u = u0.clone()
@@ -329,8 +412,9 @@ def bench_newton_pseudo(torch, fr, steps=12):
     del ev, u0
     torch.cuda.empty_cache()
     return {"workload": f"bump 256x64 p=5 HLLC, pseudo-transient continuation (TSPSEUDO SER rule) from CFL 1, "
-                        f"x1.5 per step, {steps} steps; tangent with the exact BC linearisation, complex step "
-                        "with the reference's frozen lambda BCs", **runs, "target_reduction": target,
+                        f"x1.5 per step, {steps} steps, GMRES(60) with ILU(0) of the coloured FD matrix; tangent "
+                        "with the exact BC linearisation, complex step with the reference's frozen lambda BCs",
+            **runs, "target_reduction": target,
             "tangent_s_to_target": tt, "complex_step_s_to_target": tc,
             "speedup_at_equal_reduction": (tc / tt) if (tt and tc) else None}
 
