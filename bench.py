@@ -21,6 +21,11 @@ import os
 import sys
 import time
 
+
+def progress(msg):
+    """One line per finished leg on stderr (the JSON line stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -50,7 +55,7 @@ def parse():
     ap.add_argument("--adj-snapshots", type=int, default=32)
     ap.add_argument("--adj-n", type=int, default=1024)
     ap.add_argument("--kernel-only", choices=["stage", "residual", "tangent", "cstep", "adjoint", "large", "limiter",
-                                                "copy", "batch", "batch_each"], default=None,
+                                                "copy", "batch", "batch_each", "ilu"], default=None,
                     help="profiling mode: launch only this kernel `--steps` times, print nothing else")
     return ap.parse_args()
 
@@ -182,31 +187,37 @@ def main():
         "wall_s_timed": t_wall,
     }
 
+    progress("headline done")
     if not args.no_large:
         try:
             line["large_curved"] = bench_large(torch, fr)
         except Exception as exc:  # report, keep the headline line valid
             line["large_curved"] = {"error": repr(exc)}
+        progress("large curved done")
     if not args.no_newton:
         try:
             line["newton"] = bench_newton(torch, fr)
         except Exception as exc:
             line["newton"] = {"error": repr(exc)}
+        progress("newton done")
         if args.ilu:
             try:
                 line["ilu"] = bench_ilu(torch, fr)
             except Exception as exc:
                 line["ilu"] = {"error": repr(exc)}
+            progress("ilu done")
         if args.newton_pseudo:
             try:
                 line["newton_pseudo"] = bench_newton_pseudo(torch, fr)
             except Exception as exc:
                 line["newton_pseudo"] = {"error": repr(exc)}
+            progress("newton pseudo done")
     if not args.no_batch:
         try:
             line["batched"] = bench_batch(torch, fr)
         except Exception as exc:
             line["batched"] = {"error": repr(exc)}
+        progress("batched done")
         try:
             # the same sweep on meshes 1/4 the size (N in 8..48): launch-bound instances
             # (20 steps: within 100 steps shocks form and the filtered p=5,6
@@ -231,6 +242,21 @@ def kernel_only(torch, fr, args):
             step(u)
         torch.cuda.synchronize()
         print(json.dumps({"kernel_only": args.kernel_only, "steps": args.steps, "ndofs": b.ndofs}))
+        return
+    if args.kernel_only == "ilu":
+        # the bump 256x64 p=5: one coloured assembly (FD), one ILU(0) factorisation,
+        # then `steps` applies
+        cfg = CF.bump(tangent_bc_mode=0)
+        ev = fr.PhysicsEvaluator(cfg)
+        u = ev.impose_ic(fr.IC_UNIFORM, CF.bump_freestream())
+        sigma = CF.sound_speed_max(u, cfg) / (10.0 * min(cfg["lx"] / cfg["nx"], cfg["ly"] / cfg["ny"]))
+        P = ev.color_jacobian(u, sigma, None, method=0)
+        Dinv = ev.ilu0_factor(P)
+        x = torch.randn(ev.ndofs, dtype=torch.float64, device="cuda")
+        for _ in range(args.steps):
+            ev.ilu0_apply(P, Dinv, x)
+        torch.cuda.synchronize()
+        print(json.dumps({"kernel_only": "ilu", "applies": args.steps, "ndofs": ev.ndofs}))
         return
     if args.kernel_only in ("large", "limiter"):
         cfg = dict(dims=2, nx=1024, ny=1024, p=7, lx=2.0, ly=2.0, riemann=1, geometry=fr.GEOM_WARPED,
@@ -369,6 +395,7 @@ def bench_ilu(torch, fr):
         t, st = wall(lambda: ev.beuler_step(u, 1.0 / sigma, jac_type=0, precond=pc, ksp_restart=60,
                                             ksp_max_it=3000, ksp_rtol=1e-3, snes_rtol=1e-3, snes_max_it=1))
         its[name] = {"gmres_its": st["ksp_its"], "s": t}
+        progress(f"ilu: one Newton iteration with {name}: {st['ksp_its']} GMRES iterations")
     out["one_newton_iteration"] = its
     del ev, u0
     torch.cuda.empty_cache()
@@ -397,6 +424,7 @@ def bench_newton_pseudo(torch, fr, steps=12):
     for name, jt in (("tangent", 0), ("complex_step", 1)):
         u = u0.clone()
         r = ev.pseudo_run(u, dt0, 1.5, steps, jac_type=jt, **opts)
+        progress(f"pseudo-transient {name}: {r['steps']} steps, {r['seconds'][-1]:.1f} s")
         red = [f / r["fnorm"][0] for f in r["fnorm"]]
         runs[name] = {"fnorm": r["fnorm"], "reduction": red, "seconds": r["seconds"],
                       "cfl": [dt / (dt0 / 1.0) for dt in r["dt"]], "finite": bool(torch.isfinite(u).all())}

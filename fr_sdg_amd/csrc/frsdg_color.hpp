@@ -29,8 +29,9 @@
 // whose lower neighbours are all factored form a level; levels run as
 // separate launches (i + 2j for a non-periodic 2D mesh: nx + 2 ny - 2 levels).
 // The diagonal blocks' triangular inverses are kept: the triangular solves of
-// the factorisation become block products (LDS-tiled, blk_gemm) and both
-// sweeps of the apply are dense block products.
+// the factorisation become block products (LDS-tiled, blk_gemm).  A last,
+// fully parallel pass scales every off-diagonal block by its row's diagonal
+// inverse (k_ilu_scale), which makes the rows of both sweeps independent.
 #pragma once
 #include "../../include/frsdg.h"
 
@@ -58,6 +59,8 @@ namespace frsdg
         int* upd  = nullptr; // [E][CMAXN][2 CMAXP] (slot of J in row I, slot of J in row K) per lower slot
         int* upp  = nullptr; // [E][CMAXN] upper slots
         int* lvl  = nullptr; // [E] elements grouped by level
+        int* lpos = nullptr; // [E][2][CMAXN] per level position: lower (slot, element) pairs, slot -1 ends
+        int* upos = nullptr; // [E][2][CMAXN] per level position: upper (slot, element) pairs
         int* info = nullptr; // zero-pivot flag
         double* P    = nullptr; // E*S*nd*nd, factored in place by the preconditioner
         double* Dinv = nullptr; // E*2*nd*nd: L_II^-1, U_II^-1
@@ -65,7 +68,8 @@ namespace frsdg
         double* v    = nullptr; // perturbed state / unit colour vector
         double* f0   = nullptr; // F at the base state
         double* f1   = nullptr; // F at the perturbed state
-        double* T    = nullptr; // factorisation scratch: one block per row of the widest level
+        double* T    = nullptr; // factorisation scratch: tcap blocks
+        long long tcap = 0;
     };
 
     // Neighbour tables, colour classes and the ILU(0) level schedule.
@@ -73,7 +77,8 @@ namespace frsdg
     // mesh (a periodic direction whose element count is not a multiple of 3).
     inline int color_plan_tables(int dims, int nx, int ny, int per_x, int per_y, int nd, ColorPlan& cp,
                                  std::vector<int>& h_nbr, std::vector<int>& h_cls, std::vector<int>& h_low,
-                                 std::vector<int>& h_upd, std::vector<int>& h_upp, std::vector<int>& h_lvl)
+                                 std::vector<int>& h_upd, std::vector<int>& h_upp, std::vector<int>& h_lvl,
+                                 std::vector<int>& h_lpos, std::vector<int>& h_upos)
     {
         if (dims == 1) ny = 1, per_y = 0;
         if ((per_x && nx % 3) || (per_y && ny % 3)) return FRSDG_ERR_UNSUPPORTED;
@@ -157,6 +162,21 @@ namespace frsdg
         h_lvl.assign(E, 0);
         std::vector<int> fill(cp.lvl_ptr.begin(), cp.lvl_ptr.end() - 1);
         for (long long e = 0; e < E; ++e) h_lvl[fill[level[e]]++] = static_cast<int>(e);
+        // the sweeps read a row's neighbours by level position, in one round trip
+        h_lpos.assign(E * 2 * CMAXN, -1);
+        h_upos.assign(E * 2 * CMAXN, -1);
+        for (long long pos = 0; pos < E; ++pos)
+        {
+            const long long e = h_lvl[pos];
+            for (int q = 0; q < CMAXN; ++q)
+            {
+                const int sl = h_low[e * CMAXN + q], su = h_upp[e * CMAXN + q];
+                h_lpos[(pos * 2) * CMAXN + q]     = sl;
+                h_lpos[(pos * 2 + 1) * CMAXN + q] = sl >= 0 ? h_nbr[e * S + sl] : 0;
+                h_upos[(pos * 2) * CMAXN + q]     = su;
+                h_upos[(pos * 2 + 1) * CMAXN + q] = su >= 0 ? h_nbr[e * S + su] : 0;
+            }
+        }
         return FRSDG_OK;
     }
 
@@ -241,7 +261,10 @@ namespace frsdg
     // blocks, by the whole workgroup (256 threads): output tiles of 144 x 144,
     // each thread a 9 x 9 register tile (rows tx + 16 i, columns ty + 16 j),
     // k in slabs of 16 staged through LDS, the next slab's loads in flight
-    // while the current one is multiplied.  C must not alias A or B.
+    // while the current one is multiplied (the prefetch registers on top of
+    // the 81 accumulators spill a little to scratch; measured on the 256x64
+    // p=5 bump this is still faster than exposing each slab's load latency).
+    // C must not alias A or B.
     constexpr int GT = 144, GKB = 16, GST = GT * GKB / 256; // 9 staged values of A and of B per thread
     __device__ inline void gemm_fetch(int nd, const double* __restrict__ A, const double* __restrict__ B, int r0,
                                       int c0, int k0, double (&ra)[GST], double (&rb)[GST])
@@ -583,107 +606,98 @@ namespace frsdg
         }
     }
 
+    // After the last level: every off-diagonal block is replaced by its
+    // product with the row's diagonal inverse, M_IK = L_II^-1 L_IK below and
+    // N_IJ = U_II^-1 U_IJ above, so that the sweeps' rows are independent:
+    // y_I = L_II^-1 x_I - sum_K M_IK y_K, z_I = U_II^-1 y_I - sum_J N_IJ z_J.
+    // One workgroup per (row, off-diagonal slot) of a chunk starting at block
+    // `first`; T holds one scratch block per workgroup.
+    static __global__ __launch_bounds__(256) void k_ilu_scale(int S, int nd, long long nblk, long long first,
+                                                              const int* __restrict__ nbr, double* P,
+                                                              const double* __restrict__ Dinv, double* T)
+    {
+        __shared__ double As[GT * GKB], Bs[GT * GKB];
+        const long long g = first + blockIdx.x;
+        if (g >= nblk) return; // uniform per workgroup
+        const long long e   = g / (S - 1);
+        int             s   = static_cast<int>(g - e * (S - 1));
+        if (s >= S / 2) ++s; // skip the centre slot
+        const int m = nbr[e * S + s];
+        if (m < 0) return;
+        const long long bsz = static_cast<long long>(nd) * nd;
+        double* B  = P + (e * S + s) * bsz;
+        double* Tw = T + static_cast<long long>(blockIdx.x) * bsz;
+        blk_copy(bsz, B, Tw);
+        __syncthreads();
+        blk_gemm(nd, Dinv + e * 2 * bsz + (m < e ? 0 : bsz), Tw, B, false, As, Bs);
+    }
+
     constexpr int CMAXND = 336; // largest element block (2D p = 8: 324 DOFs)
 
-    // One level of the forward (FWD: out_I = L_II^-1 (in_I - sum_K L_IK out_K),
-    // K below) or backward (out_I = U_II^-1 (in_I - sum_J U_IJ out_J), J above)
-    // sweep.  One element per workgroup of 16 waves: wave w takes columns
-    // [w cb, (w+1) cb) of every block, lanes take rows, and the 16 partial
-    // sums meet in LDS.  The sweeps are latency-bound (a level holds few
-    // elements), so the block size is a template parameter (0: run-time nd)
-    // and a wave's loads of one block are all issued before their products.
-    constexpr int SWT = 1024, SWW = SWT / 64, SWR = (CMAXND + 63) / 64;
+    // One level of the forward (out_I = L_II^-1 in_I - sum_K M_IK out_K, K
+    // below) or backward (out_I = U_II^-1 in_I - sum_J N_IJ out_J, J above)
+    // sweep on the scaled factors.  Rows are independent, so a row is split
+    // over workgroups: blockIdx.y picks 16 rows, thread (row, g) sums the
+    // columns c = g (mod 16) of every block and 16 partial sums meet in LDS.
+    // A level holds few rows; spreading each over nd/16 workgroups puts the
+    // whole chip on it.  ND = block size as a template parameter (0: run time)
+    // so that a thread's loads are all in flight together.
+    constexpr int SWR = 16; // rows per workgroup
     template <int ND>
-    __device__ inline void sweep_block(int nd, int c_lo, int c_hi, int lane, const double* __restrict__ B,
-                                       const double* __restrict__ v, double (&part)[SWR])
+    __device__ inline double sweep_dot(int nd, int r, int g, const double* __restrict__ B,
+                                       const double* __restrict__ v)
     {
-        constexpr int CB  = ND ? (ND + SWW - 1) / SWW : 0;
-        constexpr int RPL = ND ? (ND + 63) / 64 : SWR;
+        double acc = 0.0;
         if constexpr (ND > 0)
         {
 #pragma unroll
-            for (int cc = 0; cc < CB; ++cc)
-            {
-                const int  c  = c_lo + cc;
-                const bool ok = c < c_hi;
-                const double vc = ok ? v[c] : 0.0;
-#pragma unroll
-                for (int rr = 0; rr < RPL; ++rr)
-                {
-                    const int r = lane + 64 * rr;
-                    if (ok && r < ND) part[rr] = ::fma(B[r + c * ND], vc, part[rr]);
-                }
-            }
+            for (int c = g; c < ND; c += 16) acc = ::fma(B[r + c * ND], v[c], acc);
         }
         else
         {
-            for (int c = c_lo; c < c_hi; ++c)
-            {
-                const double vc = v[c];
-#pragma unroll
-                for (int rr = 0; rr < RPL; ++rr)
-                {
-                    const int r = lane + 64 * rr;
-                    if (r < nd) part[rr] = ::fma(B[r + static_cast<long long>(c) * nd], vc, part[rr]);
-                }
-            }
+            for (int c = g; c < nd; c += 16) acc = ::fma(B[r + static_cast<long long>(c) * nd], v[c], acc);
         }
+        return acc;
     }
 
     template <bool FWD, int ND>
-    __global__ __launch_bounds__(SWT) void k_ilu_sweep(int S, int nd, const int* __restrict__ nbr,
-                                                       const int* __restrict__ list, const int* __restrict__ lvl,
-                                                       int off, const double* __restrict__ P,
+    __global__ __launch_bounds__(256) void k_ilu_sweep(int S, int nd, const int* __restrict__ pos_tab,
+                                                       const int* __restrict__ lvl, int off,
+                                                       const double* __restrict__ P,
                                                        const double* __restrict__ Dinv,
                                                        const double* __restrict__ in, double* out)
     {
-        __shared__ double red[SWW][CMAXND];
-        __shared__ double t[CMAXND];
+        __shared__ double red[16][SWR + 1];
         if constexpr (ND > 0) nd = ND;
-        const long long I    = lvl[off + blockIdx.x];
-        const long long bsz  = static_cast<long long>(nd) * nd;
-        const int       lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-        const int       cb   = (nd + SWW - 1) / SWW;
-        const int       c_lo = w * cb, c_hi = min(nd, c_lo + cb);
-        double part[SWR];
+        const long long pos = off + blockIdx.x;
+        const long long I   = lvl[pos];
+        const long long bsz = static_cast<long long>(nd) * nd;
+        const int       rl  = threadIdx.x & (SWR - 1), g = threadIdx.x >> 4;
+        const int       r   = blockIdx.y * SWR + rl;
+        int sl[CMAXN], el[CMAXN];
 #pragma unroll
-        for (int rr = 0; rr < SWR; ++rr) part[rr] = 0.0;
         for (int q = 0; q < CMAXN; ++q)
         {
-            const int s = list[I * CMAXN + q];
-            if (s < 0) break;
-            sweep_block<ND>(nd, c_lo, c_hi, lane, P + (I * S + s) * bsz, out + static_cast<long long>(nbr[I * S + s]) * nd,
-                            part);
+            sl[q] = pos_tab[pos * 2 * CMAXN + q];
+            el[q] = pos_tab[pos * 2 * CMAXN + CMAXN + q];
         }
+        double acc = 0.0;
+        if (r < nd)
+        {
+            acc = sweep_dot<ND>(nd, r, g, Dinv + I * 2 * bsz + (FWD ? 0 : bsz), in + I * nd);
 #pragma unroll
-        for (int rr = 0; rr < SWR; ++rr)
-        {
-            const int r = lane + 64 * rr;
-            if (r < nd) red[w][r] = part[rr];
+            for (int q = 0; q < CMAXN; ++q)
+                if (sl[q] >= 0)
+                    acc -= sweep_dot<ND>(nd, r, g, P + (I * S + sl[q]) * bsz, out + static_cast<long long>(el[q]) * nd);
         }
+        red[g][rl] = acc;
         __syncthreads();
-        for (int r = threadIdx.x; r < nd; r += SWT)
+        if (g == 0 && r < nd)
         {
-            double acc = in[I * nd + r];
-            for (int k = 0; k < SWW; ++k) acc -= red[k][r];
-            t[r] = acc;
-        }
-        __syncthreads();
+            double sum = 0.0;
 #pragma unroll
-        for (int rr = 0; rr < SWR; ++rr) part[rr] = 0.0;
-        sweep_block<ND>(nd, c_lo, c_hi, lane, Dinv + I * 2 * bsz + (FWD ? 0 : bsz), t, part);
-#pragma unroll
-        for (int rr = 0; rr < SWR; ++rr)
-        {
-            const int r = lane + 64 * rr;
-            if (r < nd) red[w][r] = part[rr];
-        }
-        __syncthreads();
-        for (int r = threadIdx.x; r < nd; r += SWT)
-        {
-            double acc = 0.0;
-            for (int k = 0; k < SWW; ++k) acc += red[k][r];
-            out[I * nd + r] = acc;
+            for (int k = 0; k < 16; ++k) sum += red[k][rl];
+            out[I * nd + r] = sum;
         }
     }
 } // namespace frsdg

@@ -58,7 +58,7 @@ namespace frsdg
         if (s->Bf) (void) hipFree(s->Bf);
         if (frsdg::ColorPlan* cp = s->cp)
         {
-            for (int* p : {cp->nbr, cp->cls, cp->low, cp->upd, cp->upp, cp->lvl, cp->info})
+            for (int* p : {cp->nbr, cp->cls, cp->low, cp->upd, cp->upp, cp->lvl, cp->lpos, cp->upos, cp->info})
                 if (p) (void) hipFree(p);
             for (double* p : {cp->P, cp->Dinv, cp->y, cp->v, cp->f0, cp->f1, cp->T})
                 if (p) (void) hipFree(p);
@@ -346,9 +346,9 @@ namespace
         const int nd   = static_cast<int>(c->nd / c->E);
         if (nd > frsdg::CMAXND) return FRSDG_ERR_UNSUPPORTED;
         auto* cp = new frsdg::ColorPlan{};
-        std::vector<int> nbr, cls, low, upd, upp, lvl;
+        std::vector<int> nbr, cls, low, upd, upp, lvl, lpos, upos;
         int rc = frsdg::color_plan_tables(dims, c->d.nx, dims == 2 ? c->d.ny : 1, dims == 2 ? c->p2.per_x : c->p1.periodic,
-                                          dims == 2 ? c->p2.per_y : 0, nd, *cp, nbr, cls, low, upd, upp, lvl);
+                                          dims == 2 ? c->p2.per_y : 0, nd, *cp, nbr, cls, low, upd, upp, lvl, lpos, upos);
         if (rc)
         {
             delete cp;
@@ -362,7 +362,7 @@ namespace
             return FRSDG_OK;
         };
         if ((rc = up(cp->nbr, nbr)) || (rc = up(cp->cls, cls)) || (rc = up(cp->low, low)) || (rc = up(cp->upd, upd)) ||
-            (rc = up(cp->upp, upp)) || (rc = up(cp->lvl, lvl)))
+            (rc = up(cp->upp, upp)) || (rc = up(cp->lvl, lvl)) || (rc = up(cp->lpos, lpos)) || (rc = up(cp->upos, upos)))
             return rc;
         FR_TRY(hipMalloc(&cp->info, sizeof(int)));
         const long long bsz = static_cast<long long>(nd) * nd;
@@ -370,7 +370,8 @@ namespace
         FR_TRY(hipMalloc(&cp->Dinv, sizeof(double) * cp->E * 2 * bsz));
         int width = 1;
         for (size_t l = 0; l + 1 < cp->lvl_ptr.size(); ++l) width = std::max(width, cp->lvl_ptr[l + 1] - cp->lvl_ptr[l]);
-        FR_TRY(hipMalloc(&cp->T, sizeof(double) * width * bsz));
+        cp->tcap = std::max(width, 256); // scratch blocks: the widest level, at least 256 for the scaling pass
+        FR_TRY(hipMalloc(&cp->T, sizeof(double) * cp->tcap * bsz));
         for (double** p : {&cp->y, &cp->v, &cp->f0, &cp->f1}) FR_TRY(hipMalloc(p, sizeof(double) * s->N));
         return FRSDG_OK;
     }
@@ -385,6 +386,12 @@ namespace
             hipLaunchKernelGGL(frsdg::k_ilu_row, dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l])),
                                dim3(256), 0, st, cp.S, cp.nd, cp.nbr, cp.low, cp.upd, cp.upp, cp.lvl, cp.lvl_ptr[l], P,
                                Dinv, cp.T, cp.info);
+        // scale the off-diagonal blocks (rows independent in the sweeps), in
+        // chunks of as many workgroups as the scratch holds blocks
+        const long long nblk = cp.E * (cp.S - 1);
+        for (long long first = 0; first < nblk; first += cp.tcap)
+            hipLaunchKernelGGL(frsdg::k_ilu_scale, dim3(static_cast<unsigned>(std::min<long long>(cp.tcap, nblk - first))),
+                               dim3(256), 0, st, cp.S, cp.nd, nblk, first, cp.nbr, P, Dinv, cp.T);
         FR_TRY(hipGetLastError());
         int info = 0;
         FR_TRY(hipMemcpyAsync(&info, cp.info, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -398,14 +405,15 @@ namespace
                     double* z)
     {
         const int nlev = static_cast<int>(cp.lvl_ptr.size()) - 1;
+        const unsigned rch = static_cast<unsigned>((cp.nd + frsdg::SWR - 1) / frsdg::SWR);
         for (int l = 0; l < nlev; ++l)
-            hipLaunchKernelGGL((frsdg::k_ilu_sweep<true, ND>), dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l])),
-                               dim3(frsdg::SWT), 0, st, cp.S, cp.nd, cp.nbr, cp.low, cp.lvl, cp.lvl_ptr[l], P, Dinv, x,
-                               cp.y);
+            hipLaunchKernelGGL((frsdg::k_ilu_sweep<true, ND>),
+                               dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l]), rch), dim3(256), 0, st, cp.S,
+                               cp.nd, cp.lpos, cp.lvl, cp.lvl_ptr[l], P, Dinv, x, cp.y);
         for (int l = nlev - 1; l >= 0; --l)
-            hipLaunchKernelGGL((frsdg::k_ilu_sweep<false, ND>), dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l])),
-                               dim3(frsdg::SWT), 0, st, cp.S, cp.nd, cp.nbr, cp.upp, cp.lvl, cp.lvl_ptr[l], P, Dinv,
-                               cp.y, z);
+            hipLaunchKernelGGL((frsdg::k_ilu_sweep<false, ND>),
+                               dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l]), rch), dim3(256), 0, st, cp.S,
+                               cp.nd, cp.upos, cp.lvl, cp.lvl_ptr[l], P, Dinv, cp.y, z);
     }
     int ilu_apply(SolverWs* s, hipStream_t st, const double* P, const double* Dinv, const double* x, double* z)
     {

@@ -249,10 +249,14 @@ int frsdg_color_jacobian(frsdg_ctx* ctx, double t, const double* u, const double
 
 /* ILU(0) of a matrix in the frsdg_color_jacobian layout, in place, natural
  * element order, no pivoting: PETSc's default PC of a one-process run
- * (-pc_type ilu, levels 0) on the reference's preconditioning matrix.  Dinv
- * (E * 2 * nd * nd doubles) receives the inverses of the unit-lower and upper
- * factors of each diagonal block.  FRSDG_ERR_NUMERIC on a zero pivot (PETSc's
- * MAT_FACTOR_NUMERIC_ZEROPIVOT).  Synchronises the stream. */
+ * (-pc_type ilu, levels 0) on the reference's preconditioning matrix.  On
+ * return each diagonal block holds its LU factors (unit-lower L_II below the
+ * diagonal, U_II on and above), each block below the diagonal L_II^-1 L_IK
+ * and each block above U_II^-1 U_IJ (scaled so that the apply's rows are
+ * independent); Dinv (E * 2 * nd * nd doubles) receives L_II^-1 and U_II^-1.
+ * FRSDG_ERR_NUMERIC on a pivot of magnitude <= 1e-12 (PETSc's default
+ * zero-pivot tolerance, MAT_FACTOR_NUMERIC_ZEROPIVOT).  Synchronises the
+ * stream. */
 int frsdg_ilu0_factor(frsdg_ctx* ctx, double* P, double* Dinv, void* stream);
 
 /* z = (L U)^-1 x with the factors of frsdg_ilu0_factor (PCApply of ILU);

@@ -117,10 +117,26 @@ def test_ilu0_factor_and_apply(name):
     LU = ilu0(A, pat)
     Dinv = ev.ilu0_factor(P)
     LUg = _host(P)
+    # P now holds the diagonal blocks' LU and the off-diagonal factor blocks
+    # scaled by their row's diagonal inverses: L_II^-1 L_IK below, U_II^-1 U_IJ above
     ref = dense_to_blocks(LU, nbr, nd)
+    Lr = np.zeros_like(LUg)  # the device factors unscaled again
+    for e in range(E):
+        sl = slice(e * nd, (e + 1) * nd)
+        blk, gblk = LU[sl, sl], LUg[e, S // 2].T
+        Lii, Uii = np.tril(blk, -1) + np.eye(nd), np.triu(blk)
+        gL, gU = np.tril(gblk, -1) + np.eye(nd), np.triu(gblk)
+        Lr[e, S // 2] = LUg[e, S // 2]
+        for s_ in range(S):
+            m = nbr[e, s_]
+            if m < 0 or m == e:
+                continue
+            B = LU[sl, m * nd:(m + 1) * nd]
+            ref[e, s_] = (np.linalg.solve(Lii, B) if m < e else np.linalg.solve(Uii, B)).T
+            Lr[e, s_] = ((gL if m < e else gU) @ LUg[e, s_].T).T
     assert C.rel_l2(LUg, ref) <= 1e-10
     # the defining property, straight from the device factors
-    LUd, _ = blocks_to_dense(LUg, nbr)
+    LUd, _ = blocks_to_dense(Lr, nbr)
     L = np.tril(LUd, -1) + np.eye(A.shape[0])
     U = np.triu(LUd)
     assert np.max(np.abs((L @ U - A)[pat])) <= 1e-10 * np.abs(A).max()
