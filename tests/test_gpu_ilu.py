@@ -35,6 +35,11 @@ CASES = {
     "sod1d_p3": C.sod_cfg(10, 3),
     "sod1d_periodic_p2": dict(C.sod_cfg(9, 2), bc_type=C.PER),
     "bump_logvars_p2": dict(C.bump_cfg(3, 2, 2), tangent_bc_mode=1, cond_sys=1),
+    # the configured bump's order (144-DOF blocks: register-tile diagonal LU,
+    # the 144 sweep instantiation) and a larger block (196: the global-memory
+    # diagonal path, GEMMs over several 144 x 144 output tiles)
+    "bump_p5": dict(C.bump_cfg(3, 2, 5), tangent_bc_mode=1),
+    "warped_vdw_p6": C.warped_cfg(2, 2, 6, filt=0),
 }
 
 
