@@ -26,12 +26,14 @@
 //   L_IK = A_IK U_KK^-1  and  A_IJ -= L_IK U_KJ  (J > K in both patterns)
 // for its lower neighbours K in increasing order, then A_II = L_II U_II
 // (unpivoted), then U_IJ = L_II^-1 A_IJ for its upper neighbours J.  Rows
-// whose lower neighbours are all factored form a level; levels run as
-// separate launches (i + 2j for a non-periodic 2D mesh: nx + 2 ny - 2 levels).
-// The diagonal blocks' triangular inverses are kept: the triangular solves of
-// the factorisation become block products (LDS-tiled, blk_gemm).  A last,
-// fully parallel pass scales every off-diagonal block by its row's diagonal
-// inverse (k_ilu_scale), which makes the rows of both sweeps independent.
+// whose lower neighbours are all factored form a level (i + 2j for a
+// non-periodic 2D mesh: nx + 2 ny - 2 levels).  A level runs as a short
+// sequence of launches whose workgroups are 48 x 48 tiles of the block
+// products (the triangular solves use the stored inverses of the factored
+// diagonal blocks), one workgroup per row for the diagonal LU and one per
+// inverse.  A last, fully parallel pass scales every off-diagonal block by
+// its row's diagonal inverse (k_ilu_scale), which makes the rows of both
+// sweeps of the apply independent.
 #pragma once
 #include "../../include/frsdg.h"
 
@@ -52,6 +54,8 @@ namespace frsdg
         int       ncls = 0; // colour classes of elements (9 or 3)
         long long E    = 0;
         std::vector<int> lvl_ptr;   // level offsets into lvl (host)
+        std::vector<int> lvl_nlow;  // most lower neighbours of a row, per level (host)
+        std::vector<int> lvl_nupp;  // most upper neighbours of a row, per level (host)
         std::vector<int> cls_count; // elements per colour class (host)
         int* nbr  = nullptr; // [E][S] neighbour element per slot, -1 none
         int* cls  = nullptr; // [E] colour class
@@ -162,6 +166,16 @@ namespace frsdg
         h_lvl.assign(E, 0);
         std::vector<int> fill(cp.lvl_ptr.begin(), cp.lvl_ptr.end() - 1);
         for (long long e = 0; e < E; ++e) h_lvl[fill[level[e]]++] = static_cast<int>(e);
+        cp.lvl_nlow.assign(nlev, 0);
+        cp.lvl_nupp.assign(nlev, 0);
+        for (long long e = 0; e < E; ++e)
+        {
+            int nl = 0, nu = 0;
+            while (nl < CMAXN && h_low[e * CMAXN + nl] >= 0) ++nl;
+            while (nu < CMAXN && h_upp[e * CMAXN + nu] >= 0) ++nu;
+            cp.lvl_nlow[level[e]] = std::max(cp.lvl_nlow[level[e]], nl);
+            cp.lvl_nupp[level[e]] = std::max(cp.lvl_nupp[level[e]], nu);
+        }
         // the sweeps read a row's neighbours by level position, in one round trip
         h_lpos.assign(E * 2 * CMAXN, -1);
         h_upos.assign(E * 2 * CMAXN, -1);
@@ -257,109 +271,14 @@ namespace frsdg
 
     // ---- block ILU(0) -----------------------------------------------------------
 
-    // C = A B (sub = false) or C -= A B (sub = true) for nd x nd column-major
-    // blocks, by the whole workgroup (256 threads): output tiles of 144 x 144,
-    // each thread a 9 x 9 register tile (rows tx + 16 i, columns ty + 16 j),
-    // k in slabs of 16 staged through LDS, the next slab's loads in flight
-    // while the current one is multiplied (the prefetch registers on top of
-    // the 81 accumulators spill a little to scratch; measured on the 256x64
-    // p=5 bump this is still faster than exposing each slab's load latency).
-    // C must not alias A or B.
-    constexpr int GT = 144, GKB = 16, GST = GT * GKB / 256; // 9 staged values of A and of B per thread
-    __device__ inline void gemm_fetch(int nd, const double* __restrict__ A, const double* __restrict__ B, int r0,
-                                      int c0, int k0, double (&ra)[GST], double (&rb)[GST])
-    {
-#pragma unroll
-        for (int m = 0; m < GST; ++m)
-        {
-            const int o  = threadIdx.x + 256 * m;
-            const int kk = o / GT, rr = o - kk * GT; // A: rows contiguous per k
-            const int r = r0 + rr, k = k0 + kk;
-            ra[m] = (r < nd && k < nd) ? A[r + static_cast<long long>(k) * nd] : 0.0;
-            const int cc = o / GKB, kb = o - cc * GKB; // B: k contiguous per column
-            const int c = c0 + cc, k2 = k0 + kb;
-            rb[m] = (c < nd && k2 < nd) ? B[k2 + static_cast<long long>(c) * nd] : 0.0;
-        }
-    }
-    __device__ inline void blk_gemm(int nd, const double* __restrict__ A, const double* __restrict__ B,
-                                    double* __restrict__ C, bool sub, double* __restrict__ As,
-                                    double* __restrict__ Bs)
-    {
-        const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-        for (int r0 = 0; r0 < nd; r0 += GT)
-            for (int c0 = 0; c0 < nd; c0 += GT)
-            {
-                double acc[9][9];
-#pragma unroll
-                for (int i = 0; i < 9; ++i)
-#pragma unroll
-                    for (int j = 0; j < 9; ++j) acc[i][j] = 0.0;
-                double ra[GST], rb[GST];
-                gemm_fetch(nd, A, B, r0, c0, 0, ra, rb);
-                for (int k0 = 0; k0 < nd; k0 += GKB)
-                {
-#pragma unroll
-                    for (int m = 0; m < GST; ++m)
-                    {
-                        const int o = tid + 256 * m;
-                        const int kk = o / GT, cc = o / GKB;
-                        As[kk * GT + (o - kk * GT)] = ra[m];
-                        Bs[(o - cc * GKB) * GT + cc] = rb[m];
-                    }
-                    __syncthreads();
-                    if (k0 + GKB < nd) gemm_fetch(nd, A, B, r0, c0, k0 + GKB, ra, rb);
-#pragma unroll 2
-                    for (int kk = 0; kk < GKB; ++kk)
-                    {
-                        double av[9], bv[9];
-#pragma unroll
-                        for (int i = 0; i < 9; ++i) av[i] = As[kk * GT + tx + 16 * i];
-#pragma unroll
-                        for (int j = 0; j < 9; ++j) bv[j] = Bs[kk * GT + ty + 16 * j];
-#pragma unroll
-                        for (int i = 0; i < 9; ++i)
-#pragma unroll
-                            for (int j = 0; j < 9; ++j) acc[i][j] = ::fma(av[i], bv[j], acc[i][j]);
-                    }
-                    __syncthreads();
-                }
-#pragma unroll
-                for (int j = 0; j < 9; ++j)
-#pragma unroll
-                    for (int i = 0; i < 9; ++i)
-                    {
-                        const int r = r0 + tx + 16 * i, c = c0 + ty + 16 * j;
-                        if (r < nd && c < nd)
-                        {
-                            const long long at = r + static_cast<long long>(c) * nd;
-                            C[at] = sub ? C[at] - acc[i][j] : acc[i][j];
-                        }
-                    }
-            }
-    }
+    constexpr int GT = 144; // largest block factored in a register tile (256 threads x 9 x 9)
 
-    __device__ inline void blk_copy(long long bsz, const double* __restrict__ src, double* __restrict__ dst)
-    {
-        constexpr int U = 8; // loads in flight per thread
-        long long o = threadIdx.x;
-        for (; o + (U - 1) * 256 < bsz; o += U * 256)
-        {
-            double v[U];
-#pragma unroll
-            for (int m = 0; m < U; ++m) v[m] = src[o + 256 * m];
-#pragma unroll
-            for (int m = 0; m < U; ++m) dst[o + 256 * m] = v[m];
-        }
-        for (; o < bsz; o += 256) dst[o] = src[o];
-    }
-
-    // A_II = L_II U_II (unpivoted, right-looking) and the inverses of its unit-
-    // lower and upper factors, for nd <= 144: the block lives in registers,
-    // thread (tx, ty) holding rows tx + 16 i and columns ty + 16 j, and each
-    // elimination step broadcasts the pivot row and column through LDS.
-    __device__ inline void diag_lu_inv_reg(int nd, double* __restrict__ D, double* __restrict__ Li,
-                                           double* __restrict__ Ui, double* __restrict__ bufA,
-                                           double* __restrict__ bufB, int* __restrict__ info)
+    // A_II = L_II U_II (unpivoted, right-looking) for nd <= 144: the block
+    // lives in registers, thread (tx, ty) holding rows tx + 16 i and columns
+    // ty + 16 j, and each elimination step broadcasts the pivot row and column
+    // through LDS.
+    __device__ inline void diag_lu_reg(int nd, double* __restrict__ D, double* __restrict__ bufA,
+                                       double* __restrict__ bufB, int* __restrict__ info)
     {
         const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
         double x[9][9];
@@ -420,6 +339,17 @@ namespace frsdg
                 if (r < nd && c < nd) D[r + static_cast<long long>(c) * nd] = x[i][j];
             }
         __syncthreads();
+    }
+
+    // L_II^-1 of the factored block D (unit lower part), in a register tile;
+    // the columns of L are read from D one step ahead.
+    __device__ inline void diag_linv_reg(int nd, const double* __restrict__ D, double* __restrict__ Li,
+                                         double* __restrict__ bufA, double* __restrict__ bufB)
+    {
+        const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+        double x[9][9];
+        double* rowk = bufA;
+        double* colk = bufB;
         // L^-1: X = I; for k ascending, rows r > k: X_r -= L_rk X_k
 #pragma unroll
         for (int i = 0; i < 9; ++i)
@@ -458,6 +388,18 @@ namespace frsdg
                 const int r = tx + 16 * i, c = ty + 16 * j;
                 if (r < nd && c < nd) Li[r + static_cast<long long>(c) * nd] = x[i][j];
             }
+    }
+
+    // U_II^-1 of the factored block D (upper part, with the diagonal), in a
+    // register tile.
+    __device__ inline void diag_uinv_reg(int nd, const double* __restrict__ D, double* __restrict__ Ui,
+                                         double* __restrict__ bufA, double* __restrict__ bufB)
+    {
+        const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+        double x[9][9];
+        double* rowk = bufA;
+        double* colk = bufB;
+        double nxt;
         // U^-1: X = I; for k descending, X_k /= U_kk, rows r < k: X_r -= U_rk X_k
 #pragma unroll
         for (int i = 0; i < 9; ++i)
@@ -505,10 +447,9 @@ namespace frsdg
         __syncthreads();
     }
 
-    // A_II = L_II U_II and the inverses of its factors for blocks larger than
-    // 144,, in global memory (one column per thread for
-    // the inverses).
-    __device__ inline void diag_lu_inv_global(int nd, double* D, double* Li, double* Ui, int* info)
+    // The same for blocks larger than 144, in global memory: the LU, then one
+    // inverse (which = 0: L_II^-1, 1: U_II^-1) with one column per thread.
+    __device__ inline void diag_lu_global(int nd, double* D, int* info)
     {
         const int tid = threadIdx.x;
         for (int k = 0; k < nd; ++k)
@@ -525,111 +466,256 @@ namespace frsdg
             }
             __syncthreads();
         }
-        for (int c = tid; c < nd; c += blockDim.x)
+    }
+    __device__ inline void diag_inv_global(int nd, const double* D, double* X, int which)
+    {
+        for (int c = threadIdx.x; c < nd; c += blockDim.x)
         {
             const long long cc = static_cast<long long>(c) * nd;
-            for (int r = 0; r < c; ++r) Li[r + cc] = 0.0;
-            Li[c + cc] = 1.0;
-            for (int r = c + 1; r < nd; ++r)
+            if (which == 0)
             {
-                double acc = 0.0;
+                for (int r = 0; r < c; ++r) X[r + cc] = 0.0;
+                X[c + cc] = 1.0;
+                for (int r = c + 1; r < nd; ++r)
+                {
+                    double acc = 0.0;
 #pragma unroll 8
-                for (int k = c; k < r; ++k) acc -= D[r + static_cast<long long>(k) * nd] * Li[k + cc];
-                Li[r + cc] = acc;
+                    for (int k = c; k < r; ++k) acc -= D[r + static_cast<long long>(k) * nd] * X[k + cc];
+                    X[r + cc] = acc;
+                }
             }
-            for (int r = c + 1; r < nd; ++r) Ui[r + cc] = 0.0;
-            Ui[c + cc] = 1.0 / D[c + cc];
-            for (int r = c - 1; r >= 0; --r)
+            else
             {
-                double acc = 0.0;
+                for (int r = c + 1; r < nd; ++r) X[r + cc] = 0.0;
+                X[c + cc] = 1.0 / D[c + cc];
+                for (int r = c - 1; r >= 0; --r)
+                {
+                    double acc = 0.0;
 #pragma unroll 8
-                for (int k = r + 1; k <= c; ++k) acc -= D[r + static_cast<long long>(k) * nd] * Ui[k + cc];
-                Ui[r + cc] = acc / D[r + static_cast<long long>(r) * nd];
+                    for (int k = r + 1; k <= c; ++k) acc -= D[r + static_cast<long long>(k) * nd] * X[k + cc];
+                    X[r + cc] = acc / D[r + static_cast<long long>(r) * nd];
+                }
             }
         }
-        __syncthreads();
     }
 
-    // One block row per workgroup, rows of one level per launch.  The
-    // triangular solves with factored diagonal blocks are products with their
-    // stored inverses: L_IK = A_IK U_KK^-1 and U_IJ = L_II^-1 A_IJ.  T is
-    // scratch of one block per workgroup of the launch.
-    static __global__ __launch_bounds__(256) void k_ilu_row(int S, int nd, const int* __restrict__ nbr,
-                                                            const int* __restrict__ low, const int* __restrict__ upd,
-                                                            const int* __restrict__ upp, const int* __restrict__ lvl,
-                                                            int off, double* P, double* Dinv, double* T,
-                                                            int* __restrict__ info)
+    // ---- tiled factorisation: several workgroups per block row ----------------
+    //
+    // A level holds few block rows, and one workgroup per row leaves most CUs
+    // idle while each row runs a chain of dependent block products.  The
+    // tiled path runs a level as a sequence of launches whose workgroups are
+    // 48 x 48 output tiles of those products: for each lower neighbour K in
+    // order, T1 = A_IK U_KK^-1 (k_ilu_lstep), then A_IJ -= T1 U_KJ for every
+    // update pair together with the store of T1 as L_IK (k_ilu_pstep); then
+    // the diagonal block (k_ilu_diag: one workgroup per row); then
+    // T2_J = L_II^-1 A_IJ for the upper blocks (k_ilu_ustep) and their store
+    // (k_ilu_ucopy).  Scratch per row position b: T1 at T + b (1 + CMAXN)
+    // blocks, T2_q after it.
+    constexpr int TT = 48, TKB = 16;
+    __device__ inline void tile_gemm(int nd, int tr, int tc, const double* __restrict__ A,
+                                     const double* __restrict__ B, double* __restrict__ C, bool sub,
+                                     double* __restrict__ As, double* __restrict__ Bs)
     {
-        __shared__ double As[GT * GKB], Bs[GT * GKB];
+        const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+        const int r0 = tr * TT, c0 = tc * TT;
+        double acc[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) acc[i][j] = 0.0;
+        for (int k0 = 0; k0 < nd; k0 += TKB)
+        {
+#pragma unroll
+            for (int m = 0; m < 3; ++m) // 48 x 16 values of A and 16 x 48 of B, 3 each per thread
+            {
+                const int o  = tid + 256 * m;
+                const int kk = o / TT, rr = o - kk * TT;
+                const int r = r0 + rr, k = k0 + kk;
+                As[kk * TT + rr] = (r < nd && k < nd) ? A[r + static_cast<long long>(k) * nd] : 0.0;
+                const int cc = o / TKB, kb = o - cc * TKB;
+                const int c = c0 + cc, k2 = k0 + kb;
+                Bs[kb * TT + cc] = (c < nd && k2 < nd) ? B[k2 + static_cast<long long>(c) * nd] : 0.0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < TKB; ++kk)
+            {
+                double a[3], b[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) a[i] = As[kk * TT + tx + 16 * i];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) b[j] = Bs[kk * TT + ty + 16 * j];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) acc[i][j] = ::fma(a[i], b[j], acc[i][j]);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+            {
+                const int r = r0 + tx + 16 * i, c = c0 + ty + 16 * j;
+                if (r < nd && c < nd)
+                {
+                    const long long at = r + static_cast<long long>(c) * nd;
+                    C[at] = sub ? C[at] - acc[i][j] : acc[i][j];
+                }
+            }
+    }
+
+    __device__ inline void tile_copy(int nd, int tr, int tc, const double* __restrict__ src, double* __restrict__ dst)
+    {
+        for (int o = threadIdx.x; o < TT * TT; o += 256)
+        {
+            const int cc = o / TT, rr = o - cc * TT;
+            const int r = tr * TT + rr, c = tc * TT + cc;
+            if (r < nd && c < nd) dst[r + static_cast<long long>(c) * nd] = src[r + static_cast<long long>(c) * nd];
+        }
+    }
+
+    // blockIdx.x: row position in the level; blockIdx.y: output tile
+    static __global__ __launch_bounds__(256) void k_ilu_lstep(int S, int nd, int q, const int* __restrict__ lpos,
+                                                              const int* __restrict__ lvl, int off,
+                                                              const double* __restrict__ P,
+                                                              const double* __restrict__ Dinv, double* __restrict__ T)
+    {
+        __shared__ double As[TT * TKB], Bs[TT * TKB];
+        const long long pos = off + blockIdx.x;
+        const int       sK  = lpos[pos * 2 * CMAXN + q];
+        if (sK < 0) return;
+        const long long K = lpos[pos * 2 * CMAXN + CMAXN + q], I = lvl[pos];
+        const long long bsz = static_cast<long long>(nd) * nd;
+        const int nt = (nd + TT - 1) / TT, tr = blockIdx.y % nt, tc = blockIdx.y / nt;
+        tile_gemm(nd, tr, tc, P + (I * S + sK) * bsz, Dinv + K * 2 * bsz + bsz,
+                  T + static_cast<long long>(blockIdx.x) * (1 + CMAXN) * bsz, false, As, Bs);
+    }
+
+    // blockIdx.y = task * nt^2 + tile; tasks 0..CMAXP-1 the update pairs, task
+    // CMAXP the store of T1 as L_IK
+    static __global__ __launch_bounds__(256) void k_ilu_pstep(int S, int nd, int q, const int* __restrict__ lpos,
+                                                              const int* __restrict__ upd,
+                                                              const int* __restrict__ lvl, int off, double* P,
+                                                              const double* __restrict__ T)
+    {
+        __shared__ double As[TT * TKB], Bs[TT * TKB];
+        const long long pos = off + blockIdx.x;
+        const int       sK  = lpos[pos * 2 * CMAXN + q];
+        if (sK < 0) return;
+        const long long K = lpos[pos * 2 * CMAXN + CMAXN + q], I = lvl[pos];
+        const long long bsz = static_cast<long long>(nd) * nd;
+        const int nt = (nd + TT - 1) / TT, nt2 = nt * nt;
+        const int task = blockIdx.y / nt2, tile = blockIdx.y - task * nt2, tr = tile % nt, tc = tile / nt;
+        const double* L = T + static_cast<long long>(blockIdx.x) * (1 + CMAXN) * bsz;
+        if (task == CMAXP)
+        {
+            tile_copy(nd, tr, tc, L, P + (I * S + sK) * bsz);
+            return;
+        }
+        const long long at = ((I * CMAXN + q) * CMAXP + task) * 2;
+        const int sj = upd[at];
+        if (sj < 0) return;
+        tile_gemm(nd, tr, tc, L, P + (K * S + upd[at + 1]) * bsz, P + (I * S + sj) * bsz, true, As, Bs);
+    }
+
+    // REG: blocks of at most 144 (register tile), else the global-memory path
+    template <bool REG>
+    __global__ __launch_bounds__(256) void k_ilu_diag(int S, int nd, const int* __restrict__ lvl, int off, double* P,
+                                                      int* __restrict__ info)
+    {
+        __shared__ double bufA[GT], bufB[GT];
+        const long long I = lvl[off + blockIdx.x];
+        double* D = P + (I * S + S / 2) * static_cast<long long>(nd) * nd;
+        if constexpr (REG)
+            diag_lu_reg(nd, D, bufA, bufB, info);
+        else
+            diag_lu_global(nd, D, info);
+    }
+
+    // the two triangular inverses of the factored diagonal blocks, one
+    // workgroup each (blockIdx.y = 0: L_II^-1, 1: U_II^-1)
+    template <bool REG>
+    __global__ __launch_bounds__(256) void k_ilu_inv(int S, int nd, const int* __restrict__ lvl, int off,
+                                                     const double* __restrict__ P, double* __restrict__ Dinv)
+    {
+        __shared__ double bufA[GT], bufB[GT];
         const long long I   = lvl[off + blockIdx.x];
         const long long bsz = static_cast<long long>(nd) * nd;
-        const int       C   = S / 2; // centre slot
-        const int       tid = threadIdx.x;
-        double*         Tw  = T + static_cast<long long>(blockIdx.x) * bsz;
-        auto blk = [&](long long e, int s) { return P + (e * S + s) * bsz; };
-        for (int q = 0; q < CMAXN; ++q)
+        const double*   D   = P + (I * S + S / 2) * bsz;
+        double*         X   = Dinv + I * 2 * bsz + (blockIdx.y ? bsz : 0);
+        if constexpr (REG)
         {
-            const int sK = low[I * CMAXN + q];
-            if (sK < 0) break;
-            const long long K = nbr[I * S + sK];
-            double* A = blk(I, sK);
-            blk_copy(bsz, A, Tw);
-            __syncthreads();
-            blk_gemm(nd, Tw, Dinv + K * 2 * bsz + bsz, A, false, As, Bs); // L_IK = A_IK U_KK^-1
-            __syncthreads();
-            for (int p = 0; p < CMAXP; ++p)
-            {
-                const long long at = ((I * CMAXN + q) * CMAXP + p) * 2;
-                const int sj = upd[at];
-                if (sj < 0) break;
-                blk_gemm(nd, A, blk(K, upd[at + 1]), blk(I, sj), true, As, Bs); // A_IJ -= L_IK U_KJ
-                __syncthreads();
-            }
+            if (blockIdx.y == 0)
+                diag_linv_reg(nd, D, X, bufA, bufB);
+            else
+                diag_uinv_reg(nd, D, X, bufA, bufB);
         }
-        double* D  = blk(I, C);
-        double* Li = Dinv + I * 2 * bsz;
-        double* Ui = Li + bsz;
-        if (nd <= GT)
-            diag_lu_inv_reg(nd, D, Li, Ui, As, Bs, info);
         else
-            diag_lu_inv_global(nd, D, Li, Ui, info);
-        // U_IJ = L_II^-1 A_IJ
-        for (int q = 0; q < CMAXN; ++q)
-        {
-            const int sJ = upp[I * CMAXN + q];
-            if (sJ < 0) break;
-            double* B = blk(I, sJ);
-            blk_copy(bsz, B, Tw);
-            __syncthreads();
-            blk_gemm(nd, Li, Tw, B, false, As, Bs);
-            __syncthreads();
-        }
+            diag_inv_global(nd, D, X, blockIdx.y);
+    }
+
+    // blockIdx.y = q * nt^2 + tile: T2_q = L_II^-1 A_IJ for the q-th upper block
+    static __global__ __launch_bounds__(256) void k_ilu_ustep(int S, int nd, const int* __restrict__ upos,
+                                                              const int* __restrict__ lvl, int off,
+                                                              const double* __restrict__ P,
+                                                              const double* __restrict__ Dinv, double* __restrict__ T)
+    {
+        __shared__ double As[TT * TKB], Bs[TT * TKB];
+        const long long pos = off + blockIdx.x;
+        const long long bsz = static_cast<long long>(nd) * nd;
+        const int nt = (nd + TT - 1) / TT, nt2 = nt * nt;
+        const int q = blockIdx.y / nt2, tile = blockIdx.y - q * nt2, tr = tile % nt, tc = tile / nt;
+        const int sJ = upos[pos * 2 * CMAXN + q];
+        if (sJ < 0) return;
+        const long long I = lvl[pos];
+        tile_gemm(nd, tr, tc, Dinv + I * 2 * bsz, P + (I * S + sJ) * bsz,
+                  T + (static_cast<long long>(blockIdx.x) * (1 + CMAXN) + 1 + q) * bsz, false, As, Bs);
+    }
+
+    static __global__ __launch_bounds__(256) void k_ilu_ucopy(int S, int nd, const int* __restrict__ upos,
+                                                              const int* __restrict__ lvl, int off, double* P,
+                                                              const double* __restrict__ T)
+    {
+        const long long pos = off + blockIdx.x;
+        const long long bsz = static_cast<long long>(nd) * nd;
+        const int nt = (nd + TT - 1) / TT, nt2 = nt * nt;
+        const int q = blockIdx.y / nt2, tile = blockIdx.y - q * nt2, tr = tile % nt, tc = tile / nt;
+        const int sJ = upos[pos * 2 * CMAXN + q];
+        if (sJ < 0) return;
+        const long long I = lvl[pos];
+        tile_copy(nd, tr, tc, T + (static_cast<long long>(blockIdx.x) * (1 + CMAXN) + 1 + q) * bsz,
+                  P + (I * S + sJ) * bsz);
     }
 
     // After the last level: every off-diagonal block is replaced by its
     // product with the row's diagonal inverse, M_IK = L_II^-1 L_IK below and
     // N_IJ = U_II^-1 U_IJ above, so that the sweeps' rows are independent:
     // y_I = L_II^-1 x_I - sum_K M_IK y_K, z_I = U_II^-1 y_I - sum_J N_IJ z_J.
-    // One workgroup per (row, off-diagonal slot) of a chunk starting at block
-    // `first`; T holds one scratch block per workgroup.
-    static __global__ __launch_bounds__(256) void k_ilu_scale(int S, int nd, long long nblk, long long first,
+    // Blocks (row, off-diagonal slot) are taken in chunks starting at `first`
+    // (blockIdx.x: block of the chunk, blockIdx.y: 48 x 48 tile); phase 0
+    // copies the chunk's blocks to the scratch T, phase 1 writes the products.
+    static __global__ __launch_bounds__(256) void k_ilu_scale(int S, int nd, long long nblk, long long first, int phase,
                                                               const int* __restrict__ nbr, double* P,
                                                               const double* __restrict__ Dinv, double* T)
     {
-        __shared__ double As[GT * GKB], Bs[GT * GKB];
+        __shared__ double As[TT * TKB], Bs[TT * TKB];
         const long long g = first + blockIdx.x;
         if (g >= nblk) return; // uniform per workgroup
-        const long long e   = g / (S - 1);
-        int             s   = static_cast<int>(g - e * (S - 1));
+        const long long e = g / (S - 1);
+        int             s = static_cast<int>(g - e * (S - 1));
         if (s >= S / 2) ++s; // skip the centre slot
         const int m = nbr[e * S + s];
         if (m < 0) return;
         const long long bsz = static_cast<long long>(nd) * nd;
+        const int nt = (nd + TT - 1) / TT, tr = blockIdx.y % nt, tc = blockIdx.y / nt;
         double* B  = P + (e * S + s) * bsz;
         double* Tw = T + static_cast<long long>(blockIdx.x) * bsz;
-        blk_copy(bsz, B, Tw);
-        __syncthreads();
-        blk_gemm(nd, Dinv + e * 2 * bsz + (m < e ? 0 : bsz), Tw, B, false, As, Bs);
+        if (phase == 0)
+            tile_copy(nd, tr, tc, B, Tw);
+        else
+            tile_gemm(nd, tr, tc, Dinv + e * 2 * bsz + (m < e ? 0 : bsz), Tw, B, false, As, Bs);
     }
 
     constexpr int CMAXND = 336; // largest element block (2D p = 8: 324 DOFs)

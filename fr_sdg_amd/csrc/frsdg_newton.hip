@@ -370,7 +370,8 @@ namespace
         FR_TRY(hipMalloc(&cp->Dinv, sizeof(double) * cp->E * 2 * bsz));
         int width = 1;
         for (size_t l = 0; l + 1 < cp->lvl_ptr.size(); ++l) width = std::max(width, cp->lvl_ptr[l + 1] - cp->lvl_ptr[l]);
-        cp->tcap = std::max(width, 256); // scratch blocks: the widest level, at least 256 for the scaling pass
+        // scratch blocks: 1 + CMAXN per row of the widest level, at least 256 for the scaling pass
+        cp->tcap = std::max<long long>(static_cast<long long>(width) * (1 + frsdg::CMAXN), 256);
         FR_TRY(hipMalloc(&cp->T, sizeof(double) * cp->tcap * bsz));
         for (double** p : {&cp->y, &cp->v, &cp->f0, &cp->f1}) FR_TRY(hipMalloc(p, sizeof(double) * s->N));
         return FRSDG_OK;
@@ -382,16 +383,48 @@ namespace
         const frsdg::ColorPlan& cp = *s->cp;
         FR_TRY(hipMemsetAsync(cp.info, 0, sizeof(int), st));
         const int nlev = static_cast<int>(cp.lvl_ptr.size()) - 1;
+        const unsigned nt2 = static_cast<unsigned>(((cp.nd + frsdg::TT - 1) / frsdg::TT) * ((cp.nd + frsdg::TT - 1) / frsdg::TT));
         for (int l = 0; l < nlev; ++l)
-            hipLaunchKernelGGL(frsdg::k_ilu_row, dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l])),
-                               dim3(256), 0, st, cp.S, cp.nd, cp.nbr, cp.low, cp.upd, cp.upp, cp.lvl, cp.lvl_ptr[l], P,
-                               Dinv, cp.T, cp.info);
+        {
+            const unsigned rows = static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l]);
+            const int off = cp.lvl_ptr[l];
+            for (int q = 0; q < cp.lvl_nlow[l]; ++q)
+            {
+                hipLaunchKernelGGL(frsdg::k_ilu_lstep, dim3(rows, nt2), dim3(256), 0, st, cp.S, cp.nd, q, cp.lpos, cp.lvl,
+                                   off, P, Dinv, cp.T);
+                hipLaunchKernelGGL(frsdg::k_ilu_pstep, dim3(rows, (frsdg::CMAXP + 1) * nt2), dim3(256), 0, st, cp.S, cp.nd,
+                                   q, cp.lpos, cp.upd, cp.lvl, off, P, cp.T);
+            }
+            if (cp.nd <= frsdg::GT)
+            {
+                hipLaunchKernelGGL(frsdg::k_ilu_diag<true>, dim3(rows), dim3(256), 0, st, cp.S, cp.nd, cp.lvl, off, P,
+                                   cp.info);
+                hipLaunchKernelGGL(frsdg::k_ilu_inv<true>, dim3(rows, 2), dim3(256), 0, st, cp.S, cp.nd, cp.lvl, off, P,
+                                   Dinv);
+            }
+            else
+            {
+                hipLaunchKernelGGL(frsdg::k_ilu_diag<false>, dim3(rows), dim3(256), 0, st, cp.S, cp.nd, cp.lvl, off, P,
+                                   cp.info);
+                hipLaunchKernelGGL(frsdg::k_ilu_inv<false>, dim3(rows, 2), dim3(256), 0, st, cp.S, cp.nd, cp.lvl, off, P,
+                                   Dinv);
+            }
+            if (cp.lvl_nupp[l] > 0)
+            {
+                const dim3 g(rows, static_cast<unsigned>(cp.lvl_nupp[l]) * nt2);
+                hipLaunchKernelGGL(frsdg::k_ilu_ustep, g, dim3(256), 0, st, cp.S, cp.nd, cp.upos, cp.lvl, off, P, Dinv, cp.T);
+                hipLaunchKernelGGL(frsdg::k_ilu_ucopy, g, dim3(256), 0, st, cp.S, cp.nd, cp.upos, cp.lvl, off, P, cp.T);
+            }
+        }
         // scale the off-diagonal blocks (rows independent in the sweeps), in
         // chunks of as many workgroups as the scratch holds blocks
         const long long nblk = cp.E * (cp.S - 1);
         for (long long first = 0; first < nblk; first += cp.tcap)
-            hipLaunchKernelGGL(frsdg::k_ilu_scale, dim3(static_cast<unsigned>(std::min<long long>(cp.tcap, nblk - first))),
-                               dim3(256), 0, st, cp.S, cp.nd, nblk, first, cp.nbr, P, Dinv, cp.T);
+        {
+            const dim3 g(static_cast<unsigned>(std::min<long long>(cp.tcap, nblk - first)), nt2);
+            hipLaunchKernelGGL(frsdg::k_ilu_scale, g, dim3(256), 0, st, cp.S, cp.nd, nblk, first, 0, cp.nbr, P, Dinv, cp.T);
+            hipLaunchKernelGGL(frsdg::k_ilu_scale, g, dim3(256), 0, st, cp.S, cp.nd, nblk, first, 1, cp.nbr, P, Dinv, cp.T);
+        }
         FR_TRY(hipGetLastError());
         int info = 0;
         FR_TRY(hipMemcpyAsync(&info, cp.info, sizeof(int), hipMemcpyDeviceToHost, st));
