@@ -84,7 +84,7 @@ EXPORTS = [
     "frsdg_batch_offsets", "frsdg_batch_instance", "frsdg_batch_set_dt", "frsdg_batch_set_tuning",
     "frsdg_batch_ssprk_step", "frsdg_batch_ssprk_run", "frsdg_batch_ssprk_step_each",
     "frsdg_filter_transpose", "frsdg_mass", "frsdg_boundary_weights", "frsdg_adjoint_solve", "frsdg_pseudo_run",
-    "frsdg_color_jacobian", "frsdg_ilu0_factor", "frsdg_ilu0_apply",
+    "frsdg_color_jacobian", "frsdg_ilu0_factor", "frsdg_ilu0_apply", "frsdg_linear_solve",
 ]
 
 _lib = None
@@ -138,6 +138,8 @@ def load_library(path=LIB_PATH):
     L.frsdg_mass.argtypes = [vp, dp, vp]
     L.frsdg_boundary_weights.argtypes = [vp, i, dp, vp]
     L.frsdg_color_jacobian.argtypes = [vp, d, dp, dp, d, i, dp, vp]
+    L.frsdg_linear_solve.argtypes = [vp, d, d, dp, dp, dp, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats),
+                                     ctypes.POINTER(ctypes.c_double), i, ctypes.POINTER(ctypes.c_int), vp]
     L.frsdg_ilu0_factor.argtypes = [vp, dp, dp, vp]
     L.frsdg_ilu0_apply.argtypes = [vp, dp, dp, dp, dp, vp]
     L.frsdg_adjoint_solve.argtypes = [vp, d, d, dp, dp, dp, ctypes.POINTER(NewtonOpts),
@@ -361,6 +363,21 @@ class PhysicsEvaluator:
         out = self.empty() if out is None else out
         _check(self.lib.frsdg_ilu0_apply(self.h, _ptr(P), _ptr(Dinv), _ptr(x), _ptr(out), _stream()))
         return out
+
+    def linear_solve(self, u, b, sigma, t=0.0, out=None, **opts):
+        """(sigma I + J(u)) x = b by the device GMRES; returns x, the stats and
+        the residual history (||b||, then the estimate after each iteration)."""
+        out = self.empty() if out is None else out
+        o = dict(NEWTON_DEFAULTS)
+        o.update(opts)
+        no = NewtonOpts(**o)
+        st = NewtonStats()
+        cap = int(no.ksp_max_it) + 1
+        hist = (ctypes.c_double * cap)()
+        n = ctypes.c_int(0)
+        _check(self.lib.frsdg_linear_solve(self.h, t, sigma, _ptr(u), _ptr(b), _ptr(out), ctypes.byref(no),
+                                           ctypes.byref(st), hist, cap, ctypes.byref(n), _stream()))
+        return out, st.as_dict(), list(hist[:n.value])
 
     def adjoint_solve(self, u, rhs, sigma, t=0.0, out=None, **opts):
         """(sigma I + J(u))^T lambda = rhs by GMRES on the adjoint kernel."""

@@ -477,6 +477,7 @@ namespace
         const double* u; // linearisation point
         frsdg_newton_stats* stats;
         int transpose = 0; // solve with J^T (adjoint kernel) instead of J
+        std::vector<double>* hist = nullptr; // GMRES residual norms: ||b||, then the estimate after each iteration
 
         unsigned grid(long long n) const { return static_cast<unsigned>((n + 255) / 256); }
 
@@ -647,7 +648,11 @@ namespace
                 }
                 first = false;
                 const double beta = norm(r);
-                if (beta0 < 0) beta0 = beta;
+                if (beta0 < 0)
+                {
+                    beta0 = beta;
+                    if (hist) hist->push_back(beta);
+                }
                 if (!(beta > o->ksp_rtol * beta0) || beta == 0.0 || its >= o->ksp_max_it) break;
                 hipLaunchKernelGGL(k_axpby, dim3(grid(N)), dim3(256), 0, st, N, 0.0, r, 1.0 / beta, r);
                 std::fill(g.begin(), g.end(), 0.0);
@@ -695,6 +700,7 @@ namespace
                     H[(j + 1) * m + j] = 0.0;
                     g[j + 1] = -sn[j] * g[j];
                     g[j]     = cs[j] * g[j];
+                    if (hist) hist->push_back(std::fabs(g[j + 1]));
                     if (std::fabs(g[j + 1]) <= o->ksp_rtol * beta0 || hn == 0.0)
                     {
                         ++j;
@@ -1042,4 +1048,75 @@ extern "C" int frsdg_ilu0_apply(frsdg_ctx* c, const double* P, const double* Din
     int rc = ensure_ws(c, c->ws ? c->ws->m : 1);
     if (rc || (rc = color_plan_ensure(c, c->ws))) return rc;
     return ilu_apply(c->ws, static_cast<hipStream_t>(stream), P, Dinv, x, z);
+}
+
+// (sigma I + dR/du(u)) x = b by the device GMRES, with the residual history.
+extern "C" int frsdg_linear_solve(frsdg_ctx* c, double t, double sigma, const double* u, const double* b, double* x,
+                                  const frsdg_newton_opts* o, frsdg_newton_stats* stats, double* hist, int hist_cap,
+                                  int* hist_len, void* stream)
+{
+    if (!c || !u || !b || !x || !o || hist_cap < 0 || (hist_cap > 0 && (!hist || !hist_len))) return FRSDG_ERR_INVALID;
+    if (!aligned(u) || !aligned(b) || !aligned(x)) return FRSDG_ERR_ALIGN;
+    if (o->precond < 0 || o->precond > 4 || o->jac_type < 0 || o->jac_type > 1 || o->ksp_restart < 1)
+        return FRSDG_ERR_INVALID;
+    if (o->precond >= 1 && o->precond <= 2 && c->d.dims != 2) return FRSDG_ERR_UNSUPPORTED;
+    frsdg_newton_stats local{};
+    frsdg_newton_stats* S = stats ? stats : &local;
+    *S = frsdg_newton_stats{};
+    int rc = ensure_ws(c, o->ksp_restart);
+    if (rc) return rc;
+    SolverWs* s = c->ws;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    FR_TRY(hipStreamSynchronize(st));
+    const auto t0 = std::chrono::steady_clock::now();
+    const long long N = s->N;
+    Ctx K{c, s, st, o, t, sigma, u, S};
+    std::vector<double> h;
+    K.hist = &h;
+    if (o->precond >= 3)
+    {
+        // the preconditioning matrix of F(x) = IFunction(t, x, sigma (T(x) - T(u))) at u
+        const bool cs = c->d.cond_sys != 0;
+        const unsigned gn = static_cast<unsigned>((c->nm.nodes + 255) / 256);
+        const unsigned g  = static_cast<unsigned>((N + 255) / 256);
+        if (cs)
+        {
+            if (!c->w2) FR_TRY(hipMalloc(&c->w2, sizeof(double) * c->nd));
+            hipLaunchKernelGGL(k_p2c, dim3(gn), dim3(256), 0, st, c->nm, u, s->un);
+        }
+        else
+            FR_TRY(hipMemcpyAsync(s->un, u, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+        auto eval_F = [&](const double* xx, double* F) -> int
+        {
+            S->res_evals++;
+            if (cs)
+            {
+                hipLaunchKernelGGL(k_p2c, dim3(gn), dim3(256), 0, st, c->nm, xx, c->w2);
+                hipLaunchKernelGGL(k_waxpby, dim3(g), dim3(256), 0, st, N, sigma, c->w2, -sigma, s->un, s->ud);
+            }
+            else
+                hipLaunchKernelGGL(k_waxpby, dim3(g), dim3(256), 0, st, N, sigma, xx, -sigma, s->un, s->ud);
+            return frsdg_ifunction(c, t, xx, s->ud, F, st);
+        };
+        if ((rc = K.build_ilu(o->precond == 3 ? 0 : 2, eval_F))) return rc;
+    }
+    else if (o->precond >= 1)
+        if ((rc = K.build_block_jacobi())) return rc;
+    FR_TRY(hipMemcpyAsync(s->F, b, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+    S->fnorm0 = K.norm(s->F);
+    int its = 0;
+    if ((rc = K.gmres(s->F, x, its))) return rc;
+    S->ksp_its = its;
+    if ((rc = K.jv(x, s->w))) return rc;
+    hipLaunchKernelGGL(k_waxpby, dim3(K.grid(N)), dim3(256), 0, st, N, 1.0, s->F, -1.0, s->w, s->w);
+    S->fnorm     = K.norm(s->w);
+    S->converged = S->fnorm <= o->ksp_rtol * S->fnorm0 * 10.0;
+    S->seconds   = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (hist_len)
+    {
+        const int n = static_cast<int>(std::min<size_t>(h.size(), static_cast<size_t>(hist_cap)));
+        for (int i = 0; i < n; ++i) hist[i] = h[i];
+        *hist_len = n;
+    }
+    return FRSDG_OK;
 }

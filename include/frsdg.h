@@ -264,6 +264,19 @@ int frsdg_ilu0_factor(frsdg_ctx* ctx, double* P, double* Dinv, void* stream);
 int frsdg_ilu0_apply(frsdg_ctx* ctx, const double* P, const double* Dinv, const double* x, double* z,
                      void* stream);
 
+/* The linear solve of one Newton iteration on its own: (sigma I + dR/du(u)) x = b
+ * by the device GMRES (PETSc KSPSolve with the options' KSP settings and
+ * preconditioner; precond 3/4 build the coloured matrix of
+ * F(x) = IFunction(t, x, sigma (T(x) - T(u))) at u).  hist (host, hist_cap
+ * entries, may be NULL with hist_cap 0) receives the residual history as
+ * -ksp_monitor / KSPGetResidualHistory report it for right-preconditioned
+ * GMRES: ||b||, then the GMRES estimate of ||b - A x_j|| after every
+ * iteration j; *hist_len its length.  stats: ksp_its, fnorm0 = ||b||,
+ * fnorm = the true final residual.  Synchronises the stream. */
+int frsdg_linear_solve(frsdg_ctx* ctx, double t, double sigma, const double* u, const double* b, double* x,
+                       const frsdg_newton_opts* opts, frsdg_newton_stats* stats, double* hist, int hist_cap,
+                       int* hist_len, void* stream);
+
 /* Adjoint linear solve (sigma I + dR/du(u))^T lambda = rhs by GMRES on the
  * adjoint kernel (opts: ksp_* fields, precond 0..2, jac_type for the block
  * probes).  No reference counterpart.  Synchronises the stream. */
