@@ -290,11 +290,14 @@ namespace frsdg
                 const int r = tx + 16 * i, c = ty + 16 * j;
                 x[i][j] = (r < nd && c < nd) ? D[r + static_cast<long long>(c) * nd] : 0.0;
             }
-        double* rowk = bufA; // pivot row k
-        double* colk = bufB; // column k below the pivot, then the multipliers
+        // pivot row and column of step k in buffer k & 1: one barrier per step (a
+        // thread passing step k+1's barrier has finished reading step k's buffers,
+        // so step k+2 may overwrite them)
         for (int k = 0; k < nd; ++k)
         {
             const int kt = k & 15, kb = k >> 4;
+            double* rowk = bufA + (k & 1) * GT;
+            double* colk = bufB + (k & 1) * GT;
             if (tx == kt)
 #pragma unroll
                 for (int i = 0; i < 9; ++i)
@@ -310,14 +313,13 @@ namespace frsdg
             __syncthreads();
             const double piv = rowk[k];
             if (tid == 0 && ::fabs(piv) <= 1e-12) info[0] = 1; // PETSc's default zero-pivot tolerance
-            if (tid > k && tid < nd) colk[tid] /= piv;
-            __syncthreads();
+            const double rpiv = 1.0 / piv;
 #pragma unroll
             for (int i = 0; i < 9; ++i)
             {
                 const int r = tx + 16 * i;
                 if (r <= k || r >= nd) continue;
-                const double l = colk[r];
+                const double l = colk[r] * rpiv;
 #pragma unroll
                 for (int j = 0; j < 9; ++j)
                 {
@@ -328,7 +330,6 @@ namespace frsdg
                         x[i][j] = ::fma(-l, rowk[c], x[i][j]);
                 }
             }
-            __syncthreads();
         }
 #pragma unroll
         for (int i = 0; i < 9; ++i)
@@ -348,9 +349,8 @@ namespace frsdg
     {
         const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
         double x[9][9];
-        double* rowk = bufA;
-        double* colk = bufB;
-        // L^-1: X = I; for k ascending, rows r > k: X_r -= L_rk X_k
+        // L^-1: X = I; for k ascending, rows r > k: X_r -= L_rk X_k (buffers
+        // alternate by step: one barrier per step)
 #pragma unroll
         for (int i = 0; i < 9; ++i)
 #pragma unroll
@@ -359,6 +359,8 @@ namespace frsdg
         for (int k = 0; k < nd - 1; ++k)
         {
             const int kt = k & 15, kb = k >> 4;
+            double* rowk = bufA + (k & 1) * GT;
+            double* colk = bufB + (k & 1) * GT;
             if (tx == kt)
 #pragma unroll
                 for (int i = 0; i < 9; ++i)
@@ -378,7 +380,6 @@ namespace frsdg
                 for (int j = 0; j < 9; ++j)
                     if (ty + 16 * j <= k) x[i][j] = ::fma(-l, rowk[ty + 16 * j], x[i][j]);
             }
-            __syncthreads();
         }
 #pragma unroll
         for (int i = 0; i < 9; ++i)
@@ -397,22 +398,27 @@ namespace frsdg
     {
         const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
         double x[9][9];
-        double* rowk = bufA;
-        double* colk = bufB;
-        double nxt;
         // U^-1: X = I; for k descending, X_k /= U_kk, rows r < k: X_r -= U_rk X_k
+        // (buffers alternate by step; the diagonal entry and the column of U
+        // are loaded one step ahead)
 #pragma unroll
         for (int i = 0; i < 9; ++i)
 #pragma unroll
             for (int j = 0; j < 9; ++j) x[i][j] = (tx + 16 * i == ty + 16 * j) ? 1.0 : 0.0;
-        nxt = (tid < nd) ? D[tid + static_cast<long long>(nd - 1) * nd] : 0.0;
+        double nxt = (tid < nd) ? D[tid + static_cast<long long>(nd - 1) * nd] : 0.0;
+        double ukk = D[(nd - 1) + static_cast<long long>(nd - 1) * nd];
         for (int k = nd - 1; k >= 0; --k)
         {
             const int kt = k & 15, kb = k >> 4;
-            if (tid < nd) colk[tid] = nxt; // column k of U (the diagonal at colk[k])
-            if (k > 0 && tid < nd) nxt = D[tid + static_cast<long long>(k - 1) * nd];
-            __syncthreads();
-            const double ukk = colk[k];
+            double* rowk = bufA + (k & 1) * GT;
+            double* colk = bufB + (k & 1) * GT;
+            if (tid < k) colk[tid] = nxt; // column k of U above the diagonal
+            const double rukk = 1.0 / ukk;
+            if (k > 0)
+            {
+                if (tid < nd) nxt = D[tid + static_cast<long long>(k - 1) * nd];
+                ukk = D[(k - 1) + static_cast<long long>(k - 1) * nd];
+            }
             if (tx == kt)
 #pragma unroll
                 for (int i = 0; i < 9; ++i)
@@ -420,7 +426,7 @@ namespace frsdg
 #pragma unroll
                         for (int j = 0; j < 9; ++j)
                         {
-                            x[i][j] /= ukk;
+                            x[i][j] *= rukk;
                             rowk[ty + 16 * j] = x[i][j];
                         }
             __syncthreads();
@@ -434,7 +440,6 @@ namespace frsdg
                 for (int j = 0; j < 9; ++j)
                     if (ty + 16 * j >= k) x[i][j] = ::fma(-u, rowk[ty + 16 * j], x[i][j]);
             }
-            __syncthreads();
         }
 #pragma unroll
         for (int i = 0; i < 9; ++i)
@@ -444,7 +449,6 @@ namespace frsdg
                 const int r = tx + 16 * i, c = ty + 16 * j;
                 if (r < nd && c < nd) Ui[r + static_cast<long long>(c) * nd] = x[i][j];
             }
-        __syncthreads();
     }
 
     // The same for blocks larger than 144, in global memory: the LU, then one
@@ -625,7 +629,7 @@ namespace frsdg
     __global__ __launch_bounds__(256) void k_ilu_diag(int S, int nd, const int* __restrict__ lvl, int off, double* P,
                                                       int* __restrict__ info)
     {
-        __shared__ double bufA[GT], bufB[GT];
+        __shared__ double bufA[2 * GT], bufB[2 * GT];
         const long long I = lvl[off + blockIdx.x];
         double* D = P + (I * S + S / 2) * static_cast<long long>(nd) * nd;
         if constexpr (REG)
@@ -640,7 +644,7 @@ namespace frsdg
     __global__ __launch_bounds__(256) void k_ilu_inv(int S, int nd, const int* __restrict__ lvl, int off,
                                                      const double* __restrict__ P, double* __restrict__ Dinv)
     {
-        __shared__ double bufA[GT], bufB[GT];
+        __shared__ double bufA[2 * GT], bufB[2 * GT];
         const long long I   = lvl[off + blockIdx.x];
         const long long bsz = static_cast<long long>(nd) * nd;
         const double*   D   = P + (I * S + S / 2) * bsz;
