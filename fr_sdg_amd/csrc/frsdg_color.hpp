@@ -66,8 +66,8 @@ namespace frsdg
         int* lpos = nullptr; // [E][2][CMAXN] per level position: lower (slot, element) pairs, slot -1 ends
         int* upos = nullptr; // [E][2][CMAXN] per level position: upper (slot, element) pairs
         int* info = nullptr; // zero-pivot flag
-        double* P    = nullptr; // E*S*nd*nd, factored in place by the preconditioner
-        double* Dinv = nullptr; // E*2*nd*nd: L_II^-1, U_II^-1
+        double* P    = nullptr; // E*S*nd*nd for the Newton preconditioner (allocated on first use)
+        double* Dinv = nullptr; // E*2*nd*nd: L_II^-1, U_II^-1 (with P)
         double* y    = nullptr; // forward-sweep vector
         double* v    = nullptr; // perturbed state / unit colour vector
         double* f0   = nullptr; // F at the base state

@@ -366,8 +366,6 @@ namespace
             return rc;
         FR_TRY(hipMalloc(&cp->info, sizeof(int)));
         const long long bsz = static_cast<long long>(nd) * nd;
-        FR_TRY(hipMalloc(&cp->P, sizeof(double) * cp->E * cp->S * bsz));
-        FR_TRY(hipMalloc(&cp->Dinv, sizeof(double) * cp->E * 2 * bsz));
         int width = 1;
         for (size_t l = 0; l + 1 < cp->lvl_ptr.size(); ++l) width = std::max(width, cp->lvl_ptr[l + 1] - cp->lvl_ptr[l]);
         // scratch blocks: 1 + CMAXN per row of the widest level, at least 256 for the scaling pass
@@ -590,6 +588,13 @@ namespace
         {
             int rc = color_plan_ensure(c, s);
             if (rc) return rc;
+            frsdg::ColorPlan* cp = s->cp;
+            if (!cp->P) // the preconditioner's own matrix, allocated on first use
+            {
+                const long long bsz = static_cast<long long>(cp->nd) * cp->nd;
+                FR_TRY(hipMalloc(&cp->P, sizeof(double) * cp->E * cp->S * bsz));
+                FR_TRY(hipMalloc(&cp->Dinv, sizeof(double) * cp->E * 2 * bsz));
+            }
             if ((rc = color_assemble(method, evalF, s->cp->P))) return rc;
             return ilu_factor(s, st, s->cp->P, s->cp->Dinv);
         }
