@@ -103,7 +103,7 @@ def dense_beuler_logvars(o, w_n, dt, tol=1e-12, maxit=30):
     return w
 
 
-@pytest.mark.parametrize("jac_type,precond", [(0, 0), (1, 1), (0, 1), (0, 2)])
+@pytest.mark.parametrize("jac_type,precond", [(0, 0), (1, 1), (0, 1), (0, 2), (0, 3), (1, 4)])
 def test_beuler_with_transient_variables(jac_type, precond):
     """One implicit step on the tiny bump in log variables: JFNK == dense Newton,
     and C(W_{n+1}) == the conservative-variable step (the same discrete system)."""
