@@ -85,6 +85,7 @@ EXPORTS = [
     "frsdg_batch_ssprk_step", "frsdg_batch_ssprk_run", "frsdg_batch_ssprk_step_each",
     "frsdg_filter_transpose", "frsdg_mass", "frsdg_boundary_weights", "frsdg_adjoint_solve", "frsdg_pseudo_run",
     "frsdg_color_jacobian", "frsdg_ilu0_factor", "frsdg_ilu0_apply", "frsdg_linear_solve",
+    "frsdg_ilu0_apply_transpose",
 ]
 
 _lib = None
@@ -142,6 +143,7 @@ def load_library(path=LIB_PATH):
                                      ctypes.POINTER(ctypes.c_double), i, ctypes.POINTER(ctypes.c_int), vp]
     L.frsdg_ilu0_factor.argtypes = [vp, dp, dp, vp]
     L.frsdg_ilu0_apply.argtypes = [vp, dp, dp, dp, dp, vp]
+    L.frsdg_ilu0_apply_transpose.argtypes = [vp, dp, dp, dp, dp, vp]
     L.frsdg_adjoint_solve.argtypes = [vp, d, d, dp, dp, dp, ctypes.POINTER(NewtonOpts),
                                       ctypes.POINTER(NewtonStats), vp]
     L.frsdg_strerror.argtypes = [i]
@@ -362,6 +364,11 @@ class PhysicsEvaluator:
     def ilu0_apply(self, P, Dinv, x, out=None):
         out = self.empty() if out is None else out
         _check(self.lib.frsdg_ilu0_apply(self.h, _ptr(P), _ptr(Dinv), _ptr(x), _ptr(out), _stream()))
+        return out
+
+    def ilu0_apply_transpose(self, P, Dinv, x, out=None):
+        out = self.empty() if out is None else out
+        _check(self.lib.frsdg_ilu0_apply_transpose(self.h, _ptr(P), _ptr(Dinv), _ptr(x), _ptr(out), _stream()))
         return out
 
     def linear_solve(self, u, b, sigma, t=0.0, out=None, **opts):

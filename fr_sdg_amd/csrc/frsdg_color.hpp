@@ -72,6 +72,7 @@ namespace frsdg
         double* v    = nullptr; // perturbed state / unit colour vector
         double* f0   = nullptr; // F at the base state
         double* f1   = nullptr; // F at the perturbed state
+        double* t    = nullptr; // transposed sweeps: U_JJ^T y_J, then L_KK^T z_K (allocated on first use)
         double* T    = nullptr; // factorisation scratch: tcap blocks
         long long tcap = 0;
     };
@@ -788,6 +789,112 @@ namespace frsdg
 #pragma unroll
             for (int k = 0; k < 16; ++k) sum += red[k][rl];
             out[I * nd + r] = sum;
+        }
+    }
+
+    // Transposed apply, z = (L U)^-T x, on the same stored factors (the
+    // adjoint's preconditioner).  With t_J = U_JJ^T y_J and s_K = L_KK^T z_K:
+    //   forward levels:  t_I = x_I - sum_{J below} N_JI^T t_J,  y_I = U_II^-T t_I
+    //   backward levels: s_I = y_I - sum_{K above} M_KI^T s_K,  z_I = L_II^-T s_I
+    // where N_JI / M_KI are the neighbour row's scaled blocks at the mirrored
+    // slot S - 1 - s (the box stencil's offsets are symmetric).  One element per
+    // workgroup of 16 waves: wave w takes rows w + 16 i, lanes take columns
+    // (the transposed blocks are read along their contiguous columns), and a
+    // row's 64 lane sums are reduced by cross-lane shuffles.
+    template <int ND>
+    __device__ inline void sweepT_block(int nd, int lane, int w, const double* __restrict__ B,
+                                        const double* __restrict__ v, double (&acc)[21])
+    {
+        constexpr int RW = ND ? (ND + 15) / 16 : 21;
+        if constexpr (ND > 0)
+        {
+#pragma unroll
+            for (int c0 = 0; c0 < ND; c0 += 64)
+            {
+                const int  c  = c0 + lane;
+                const bool ok = c < ND;
+                const double vc = ok ? v[c] : 0.0;
+#pragma unroll
+                for (int i = 0; i < RW; ++i)
+                {
+                    const int r = w + 16 * i;
+                    if (ok && r < ND) acc[i] = ::fma(B[c + r * ND], vc, acc[i]);
+                }
+            }
+        }
+        else
+        {
+            for (int c = lane; c < nd; c += 64)
+            {
+                const double vc = v[c];
+#pragma unroll
+                for (int i = 0; i < RW; ++i)
+                {
+                    const int r = w + 16 * i;
+                    if (r < nd) acc[i] = ::fma(B[c + static_cast<long long>(r) * nd], vc, acc[i]);
+                }
+            }
+        }
+    }
+
+    __device__ inline double wave_sum(double v)
+    {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        return v;
+    }
+
+    template <bool FWD, int ND>
+    __global__ __launch_bounds__(1024) void k_ilu_sweepT(int S, int nd, const int* __restrict__ pos_tab,
+                                                         const int* __restrict__ lvl, int off,
+                                                         const double* __restrict__ P,
+                                                         const double* __restrict__ Dinv,
+                                                         const double* __restrict__ in, double* aux, double* out)
+    {
+        __shared__ double tl[CMAXND];
+        if constexpr (ND > 0) nd = ND;
+        constexpr int RW = ND ? (ND + 15) / 16 : 21;
+        const long long pos  = off + blockIdx.x;
+        const long long I    = lvl[pos];
+        const long long bsz  = static_cast<long long>(nd) * nd;
+        const int       lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        int sl[CMAXN], el[CMAXN];
+#pragma unroll
+        for (int q = 0; q < CMAXN; ++q)
+        {
+            sl[q] = pos_tab[pos * 2 * CMAXN + q];
+            el[q] = pos_tab[pos * 2 * CMAXN + CMAXN + q];
+        }
+        double acc[21];
+#pragma unroll
+        for (int i = 0; i < 21; ++i) acc[i] = 0.0;
+#pragma unroll
+        for (int q = 0; q < CMAXN; ++q)
+            if (sl[q] >= 0)
+                sweepT_block<ND>(nd, lane, w, P + (static_cast<long long>(el[q]) * S + (S - 1 - sl[q])) * bsz,
+                                 aux + static_cast<long long>(el[q]) * nd, acc);
+#pragma unroll
+        for (int i = 0; i < RW; ++i)
+        {
+            const double a = wave_sum(acc[i]);
+            const int    r = w + 16 * i;
+            if (lane == 0 && r < nd)
+            {
+                const double tv = in[I * nd + r] - a;
+                tl[r]           = tv;
+                aux[I * nd + r] = tv;
+            }
+            acc[i] = 0.0;
+        }
+        __syncthreads();
+        // out_I = (U_II^-1)^T t_I (forward) or (L_II^-1)^T s_I (backward)
+        sweepT_block<ND>(nd, lane, w, Dinv + I * 2 * bsz + (FWD ? bsz : 0), tl, acc);
+#pragma unroll
+        for (int i = 0; i < RW; ++i)
+        {
+            const double a = wave_sum(acc[i]);
+            const int    r = w + 16 * i;
+            if (lane == 0 && r < nd) out[I * nd + r] = a;
         }
     }
 } // namespace frsdg

@@ -60,7 +60,7 @@ namespace frsdg
         {
             for (int* p : {cp->nbr, cp->cls, cp->low, cp->upd, cp->upp, cp->lvl, cp->lpos, cp->upos, cp->info})
                 if (p) (void) hipFree(p);
-            for (double* p : {cp->P, cp->Dinv, cp->y, cp->v, cp->f0, cp->f1, cp->T})
+            for (double* p : {cp->P, cp->Dinv, cp->y, cp->v, cp->f0, cp->f1, cp->T, cp->t})
                 if (p) (void) hipFree(p);
             delete cp;
         }
@@ -446,6 +446,38 @@ namespace
                                dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l]), rch), dim3(256), 0, st, cp.S,
                                cp.nd, cp.upos, cp.lvl, cp.lvl_ptr[l], P, Dinv, cp.y, z);
     }
+    // z = (L U)^-T x (the adjoint's preconditioner; z may alias x).
+    template <int ND>
+    void ilu_sweeps_T(const frsdg::ColorPlan& cp, hipStream_t st, const double* P, const double* Dinv, const double* x,
+                      double* z)
+    {
+        const int nlev = static_cast<int>(cp.lvl_ptr.size()) - 1;
+        for (int l = 0; l < nlev; ++l)
+            hipLaunchKernelGGL((frsdg::k_ilu_sweepT<true, ND>), dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l])),
+                               dim3(1024), 0, st, cp.S, cp.nd, cp.lpos, cp.lvl, cp.lvl_ptr[l], P, Dinv, x, cp.t, cp.y);
+        for (int l = nlev - 1; l >= 0; --l)
+            hipLaunchKernelGGL((frsdg::k_ilu_sweepT<false, ND>), dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l])),
+                               dim3(1024), 0, st, cp.S, cp.nd, cp.upos, cp.lvl, cp.lvl_ptr[l], P, Dinv, cp.y, cp.t, z);
+    }
+    int ilu_apply_T(SolverWs* s, hipStream_t st, const double* P, const double* Dinv, const double* x, double* z)
+    {
+        frsdg::ColorPlan& cp = *s->cp;
+        if (!cp.t) FR_TRY(hipMalloc(&cp.t, sizeof(double) * s->N));
+        switch (cp.nd)
+        {
+            case 16: ilu_sweeps_T<16>(cp, st, P, Dinv, x, z); break;
+            case 36: ilu_sweeps_T<36>(cp, st, P, Dinv, x, z); break;
+            case 64: ilu_sweeps_T<64>(cp, st, P, Dinv, x, z); break;
+            case 100: ilu_sweeps_T<100>(cp, st, P, Dinv, x, z); break;
+            case 144: ilu_sweeps_T<144>(cp, st, P, Dinv, x, z); break;
+            case 196: ilu_sweeps_T<196>(cp, st, P, Dinv, x, z); break;
+            case 256: ilu_sweeps_T<256>(cp, st, P, Dinv, x, z); break;
+            case 324: ilu_sweeps_T<324>(cp, st, P, Dinv, x, z); break;
+            default: ilu_sweeps_T<0>(cp, st, P, Dinv, x, z); break;
+        }
+        return hip_rc(hipGetLastError());
+    }
+
     int ilu_apply(SolverWs* s, hipStream_t st, const double* P, const double* Dinv, const double* x, double* z)
     {
         const frsdg::ColorPlan& cp = *s->cp;
@@ -488,7 +520,9 @@ namespace
         }
         int prec(const double* x, double* z)
         {
-            if (o->precond >= 3) return ilu_apply(s, st, s->cp->P, s->cp->Dinv, x, z);
+            if (o->precond >= 3)
+                return transpose ? ilu_apply_T(s, st, s->cp->P, s->cp->Dinv, x, z)
+                                 : ilu_apply(s, st, s->cp->P, s->cp->Dinv, x, z);
             if (o->precond == 2)
             {
                 // red-black element block Gauss-Seidel, one forward sweep, matrix-free:
@@ -964,8 +998,7 @@ extern "C" int frsdg_adjoint_solve(frsdg_ctx* c, double t, double sigma, const d
     if (!c || !u || !rhs || !lambda || !o) return FRSDG_ERR_INVALID;
     if (!aligned(u) || !aligned(rhs) || !aligned(lambda)) return FRSDG_ERR_ALIGN;
     if (o->precond < 0 || o->precond > 4) return FRSDG_ERR_INVALID;
-    if (o->precond >= 3) return FRSDG_ERR_UNSUPPORTED; // no transposed ILU sweeps
-    if (o->precond >= 1 && c->d.dims != 2) return FRSDG_ERR_UNSUPPORTED;
+    if (o->precond >= 1 && o->precond <= 2 && c->d.dims != 2) return FRSDG_ERR_UNSUPPORTED;
     frsdg_newton_stats local{};
     frsdg_newton_stats* S = stats ? stats : &local;
     *S = frsdg_newton_stats{};
@@ -976,7 +1009,36 @@ extern "C" int frsdg_adjoint_solve(frsdg_ctx* c, double t, double sigma, const d
     FR_TRY(hipStreamSynchronize(st));
     const auto t0 = std::chrono::steady_clock::now();
     Ctx K{c, s, st, o, t, sigma, u, S};
-    if (o->precond >= 1)
+    if (o->precond >= 3)
+    {
+        // the coloured matrix of F(x) = IFunction(t, x, sigma (T(x) - T(u))) at u;
+        // its ILU(0) applied transposed
+        const long long N = s->N;
+        const bool cs = c->d.cond_sys != 0;
+        const unsigned gn = static_cast<unsigned>((c->nm.nodes + 255) / 256);
+        const unsigned g  = static_cast<unsigned>((N + 255) / 256);
+        if (cs)
+        {
+            if (!c->w2) FR_TRY(hipMalloc(&c->w2, sizeof(double) * c->nd));
+            hipLaunchKernelGGL(k_p2c, dim3(gn), dim3(256), 0, st, c->nm, u, s->un);
+        }
+        else
+            FR_TRY(hipMemcpyAsync(s->un, u, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+        auto eval_F = [&](const double* xx, double* F) -> int
+        {
+            S->res_evals++;
+            if (cs)
+            {
+                hipLaunchKernelGGL(k_p2c, dim3(gn), dim3(256), 0, st, c->nm, xx, c->w2);
+                hipLaunchKernelGGL(k_waxpby, dim3(g), dim3(256), 0, st, N, sigma, c->w2, -sigma, s->un, s->ud);
+            }
+            else
+                hipLaunchKernelGGL(k_waxpby, dim3(g), dim3(256), 0, st, N, sigma, xx, -sigma, s->un, s->ud);
+            return frsdg_ifunction(c, t, xx, s->ud, F, st);
+        };
+        if ((rc = K.build_ilu(o->precond == 3 ? 0 : 2, eval_F))) return rc;
+    }
+    else if (o->precond >= 1)
         if ((rc = K.build_block_jacobi())) return rc;
     K.transpose = 1;
     FR_TRY(hipMemcpyAsync(s->F, rhs, sizeof(double) * s->N, hipMemcpyDeviceToDevice, st));
@@ -1124,4 +1186,14 @@ extern "C" int frsdg_linear_solve(frsdg_ctx* c, double t, double sigma, const do
         *hist_len = n;
     }
     return FRSDG_OK;
+}
+
+extern "C" int frsdg_ilu0_apply_transpose(frsdg_ctx* c, const double* P, const double* Dinv, const double* x, double* z,
+                                          void* stream)
+{
+    if (!c || !P || !Dinv || !x || !z) return FRSDG_ERR_INVALID;
+    if (!aligned(P) || !aligned(Dinv) || !aligned(x) || !aligned(z)) return FRSDG_ERR_ALIGN;
+    int rc = ensure_ws(c, c->ws ? c->ws->m : 1);
+    if (rc || (rc = color_plan_ensure(c, c->ws))) return rc;
+    return ilu_apply_T(c->ws, static_cast<hipStream_t>(stream), P, Dinv, x, z);
 }

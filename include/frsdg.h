@@ -264,6 +264,11 @@ int frsdg_ilu0_factor(frsdg_ctx* ctx, double* P, double* Dinv, void* stream);
 int frsdg_ilu0_apply(frsdg_ctx* ctx, const double* P, const double* Dinv, const double* x, double* z,
                      void* stream);
 
+/* z = (L U)^-T x with the same factors (PCApplyTranspose of ILU; the
+ * preconditioner of frsdg_adjoint_solve with precond 3/4); z may alias x. */
+int frsdg_ilu0_apply_transpose(frsdg_ctx* ctx, const double* P, const double* Dinv, const double* x, double* z,
+                               void* stream);
+
 /* The linear solve of one Newton iteration on its own: (sigma I + dR/du(u)) x = b
  * by the device GMRES (PETSc KSPSolve with the options' KSP settings and
  * preconditioner; precond 3/4 build the coloured matrix of
@@ -278,7 +283,8 @@ int frsdg_linear_solve(frsdg_ctx* ctx, double t, double sigma, const double* u, 
                        int* hist_len, void* stream);
 
 /* Adjoint linear solve (sigma I + dR/du(u))^T lambda = rhs by GMRES on the
- * adjoint kernel (opts: ksp_* fields, precond 0..2, jac_type for the block
+ * adjoint kernel (opts: ksp_* fields; precond 0..2 on probed blocks, 3/4 the
+ * transposed ILU(0) of the coloured matrix of F at u; jac_type for the block
  * probes).  No reference counterpart.  Synchronises the stream. */
 int frsdg_adjoint_solve(frsdg_ctx* ctx, double t, double sigma, const double* u, const double* rhs,
                         double* lambda, const frsdg_newton_opts* opts, frsdg_newton_stats* stats,

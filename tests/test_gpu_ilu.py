@@ -157,6 +157,14 @@ def test_ilu0_factor_and_apply(name):
     xd = _dev(torch, x)
     ev.ilu0_apply(P, Dinv, xd, out=xd)  # z may alias x
     assert np.array_equal(_host(xd), z)
+    # the transposed apply (the adjoint's preconditioner): (L U)^-T x
+    Lo = np.tril(LU, -1) + np.eye(A.shape[0])
+    Uo = np.triu(LU)
+    zt = _host(ev.ilu0_apply_transpose(P, Dinv, _dev(torch, x)))
+    assert C.rel_l2(zt, np.linalg.solve((Lo @ Uo).T, x)) <= 1e-10
+    xd = _dev(torch, x)
+    ev.ilu0_apply_transpose(P, Dinv, xd, out=xd)
+    assert np.array_equal(_host(xd), zt)
 
 
 def test_zero_pivot_and_colouring_limits():

@@ -74,7 +74,7 @@ def wall_pressure(torch, ev, u, W):
     return Jv, grad
 
 
-@pytest.mark.parametrize("precond", [0, 1, 2])
+@pytest.mark.parametrize("precond", [0, 1, 2, 3, 4])
 def test_wall_pressure_adjoint_vs_fd(precond):
     """Bump channel, one backward-Euler step u1(u0): dJ(u1)/du0 = sigma lambda with
     (sigma I + J(u1))^T lambda = dJ/du1 (adjoint kernel + GMRES), against
