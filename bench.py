@@ -381,11 +381,13 @@ def bench_ilu(torch, fr):
     x = torch.randn(ev.ndofs, dtype=torch.float64, device="cuda")
     ev.ilu0_apply(P, Dinv, x)
     t_a, _ = wall(lambda: ev.ilu0_apply(P, Dinv, x), 5)
+    t_at, _ = wall(lambda: ev.ilu0_apply_transpose(P, Dinv, x), 3)
     fl = ilu_flops(cfg["nx"], cfg["ny"], nd)
     by = (E * S + 2 * E) * nd * nd * 8.0 + 4.0 * ev.ndofs * 8.0
     out.update({"assemble_fd_s": t_fd, "assemble_exact_s": t_ex, "factor_s": t_f, "factor_gflop": fl / 1e9,
                 "factor_tflops": fl / t_f / 1e12, "factor_frac_fp64_peak": fl / t_f / 78.6e12,
                 "apply_ms": t_a * 1e3, "apply_GBps": by / t_a / 1e9, "apply_frac_hbm": by / t_a / 8.0e12,
+                "apply_transpose_ms": t_at * 1e3,
                 "levels": cfg["nx"] + 2 * cfg["ny"] - 2})
     del P, Dinv
     torch.cuda.empty_cache()
