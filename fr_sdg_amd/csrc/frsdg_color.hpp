@@ -56,6 +56,7 @@ namespace frsdg
         std::vector<int> lvl_ptr;   // level offsets into lvl (host)
         std::vector<int> lvl_nlow;  // most lower neighbours of a row, per level (host)
         std::vector<int> lvl_nupp;  // most upper neighbours of a row, per level (host)
+        std::vector<int> lvl_npair; // [level][q]: most update pairs of a row's q-th lower neighbour (host)
         std::vector<int> cls_count; // elements per colour class (host)
         int* nbr  = nullptr; // [E][S] neighbour element per slot, -1 none
         int* cls  = nullptr; // [E] colour class
@@ -169,6 +170,7 @@ namespace frsdg
         for (long long e = 0; e < E; ++e) h_lvl[fill[level[e]]++] = static_cast<int>(e);
         cp.lvl_nlow.assign(nlev, 0);
         cp.lvl_nupp.assign(nlev, 0);
+        cp.lvl_npair.assign(static_cast<size_t>(nlev) * CMAXN, 0);
         for (long long e = 0; e < E; ++e)
         {
             int nl = 0, nu = 0;
@@ -176,6 +178,13 @@ namespace frsdg
             while (nu < CMAXN && h_upp[e * CMAXN + nu] >= 0) ++nu;
             cp.lvl_nlow[level[e]] = std::max(cp.lvl_nlow[level[e]], nl);
             cp.lvl_nupp[level[e]] = std::max(cp.lvl_nupp[level[e]], nu);
+            for (int q = 0; q < nl; ++q)
+            {
+                int np = 0;
+                while (np < CMAXP && h_upd[((e * CMAXN + q) * CMAXP + np) * 2] >= 0) ++np;
+                int& m = cp.lvl_npair[static_cast<size_t>(level[e]) * CMAXN + q];
+                m      = std::max(m, np);
+            }
         }
         // the sweeps read a row's neighbours by level position, in one round trip
         h_lpos.assign(E * 2 * CMAXN, -1);
@@ -598,9 +607,10 @@ namespace frsdg
                   T + static_cast<long long>(blockIdx.x) * (1 + CMAXN) * bsz, false, As, Bs);
     }
 
-    // blockIdx.y = task * nt^2 + tile; tasks 0..CMAXP-1 the update pairs, task
-    // CMAXP the store of T1 as L_IK
-    static __global__ __launch_bounds__(256) void k_ilu_pstep(int S, int nd, int q, const int* __restrict__ lpos,
+    // blockIdx.y = task * nt^2 + tile; tasks 0..npair-1 the update pairs (npair:
+    // the most any row of the level has for this neighbour), task npair the
+    // store of T1 as L_IK
+    static __global__ __launch_bounds__(256) void k_ilu_pstep(int S, int nd, int q, int npair, const int* __restrict__ lpos,
                                                               const int* __restrict__ upd,
                                                               const int* __restrict__ lvl, int off, double* P,
                                                               const double* __restrict__ T)
@@ -614,7 +624,7 @@ namespace frsdg
         const int nt = (nd + TT - 1) / TT, nt2 = nt * nt;
         const int task = blockIdx.y / nt2, tile = blockIdx.y - task * nt2, tr = tile % nt, tc = tile / nt;
         const double* L = T + static_cast<long long>(blockIdx.x) * (1 + CMAXN) * bsz;
-        if (task == CMAXP)
+        if (task == npair)
         {
             tile_copy(nd, tr, tc, L, P + (I * S + sK) * bsz);
             return;

@@ -390,8 +390,9 @@ namespace
             {
                 hipLaunchKernelGGL(frsdg::k_ilu_lstep, dim3(rows, nt2), dim3(256), 0, st, cp.S, cp.nd, q, cp.lpos, cp.lvl,
                                    off, P, Dinv, cp.T);
-                hipLaunchKernelGGL(frsdg::k_ilu_pstep, dim3(rows, (frsdg::CMAXP + 1) * nt2), dim3(256), 0, st, cp.S, cp.nd,
-                                   q, cp.lpos, cp.upd, cp.lvl, off, P, cp.T);
+                const int np = cp.lvl_npair[static_cast<size_t>(l) * frsdg::CMAXN + q];
+                hipLaunchKernelGGL(frsdg::k_ilu_pstep, dim3(rows, static_cast<unsigned>(np + 1) * nt2), dim3(256), 0, st,
+                                   cp.S, cp.nd, q, np, cp.lpos, cp.upd, cp.lvl, off, P, cp.T);
             }
             if (cp.nd <= frsdg::GT)
             {
