@@ -208,7 +208,7 @@ def main():
             progress("ilu done")
         if args.newton_pseudo:
             try:
-                line["newton_pseudo"] = bench_newton_pseudo(torch, fr)
+                line["newton_pseudo"] = bench_newton_pseudo(torch, fr, prev_dt_run=True)
             except Exception as exc:
                 line["newton_pseudo"] = {"error": repr(exc)}
             progress("newton pseudo done")
@@ -404,7 +404,7 @@ def bench_ilu(torch, fr):
     return out
 
 
-def bench_newton_pseudo(torch, fr, steps=12):
+def bench_newton_pseudo(torch, fr, steps=12, prev_dt_run=False):
     """The bump 256x64 p=5 driven to steady state by pseudo-transient
     continuation (frsdg_pseudo_run: SER step rule from CFL 1, x1.5 per step,
     one backward-Euler JFNK solve per step: GMRES(60) to 1e-3 preconditioned
@@ -414,7 +414,10 @@ def bench_newton_pseudo(torch, fr, steps=12):
     step sees them frozen, as the reference's lambda BCs are.  Reported: the
     steady-residual history of both and the time each takes to reach the
     reduction the complex step ends with (equal nonlinear-residual
-    reduction)."""
+    reduction).  With prev_dt_run, the tangent also runs with PETSc's default
+    rule (dt_{n+1} = 1.5 dt_n f_{n-1} / f_n, no
+    -ts_pseudo_increment_dt_from_initial_dt; GMRES capped at 300 iterations),
+    in chunks of 4 steps with a progress line each."""
     from fr_sdg_amd import configs as CF
     cfg = CF.bump(tangent_bc_mode=0)
     ev = fr.PhysicsEvaluator(cfg)
@@ -430,6 +433,23 @@ def bench_newton_pseudo(torch, fr, steps=12):
         red = [f / r["fnorm"][0] for f in r["fnorm"]]
         runs[name] = {"fnorm": r["fnorm"], "reduction": red, "seconds": r["seconds"],
                       "cfl": [dt / (dt0 / 1.0) for dt in r["dt"]], "finite": bool(torch.isfinite(u).all())}
+    if prev_dt_run:
+        u = u0.clone()
+        o2 = dict(opts, ksp_max_it=300)
+        dt, fn, sec, f_all, t_off = dt0, [], [], [], 0.0
+        for chunk in range((steps + 3) // 4):
+            r = ev.pseudo_run(u, dt, 1.5, min(4, steps - 4 * chunk), from_initial_dt=False, jac_type=0, **o2)
+            f_all += r["fnorm"]
+            fn += r["dt"]
+            sec += [t_off + x for x in r["seconds"]]
+            t_off = sec[-1]
+            # the rule's next step, as the uninterrupted run would take it
+            prev = f_all[-2] if len(f_all) > 1 else f_all[-1]
+            dt = 1.5 * r["dt"][-1] * prev / f_all[-1]
+            progress(f"pseudo-transient tangent (previous-dt rule): {len(f_all)} steps, reduction "
+                     f"{f_all[-1] / f_all[0]:.2e}")
+        runs["tangent_prev_dt"] = {"fnorm": f_all, "reduction": [f / f_all[0] for f in f_all], "seconds": sec,
+                                   "cfl": [d / dt0 for d in fn], "finite": bool(torch.isfinite(u).all())}
     target = min(runs["complex_step"]["reduction"])
 
     def t_to(run):
