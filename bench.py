@@ -212,6 +212,11 @@ def main():
             except Exception as exc:
                 line["newton_pseudo"] = {"error": repr(exc)}
             progress("newton pseudo done")
+    try:
+        line["widened_rows"] = bench_rows(torch, fr)
+    except Exception as exc:
+        line["widened_rows"] = {"error": repr(exc)}
+    progress("widened rows done")
     if not args.no_batch:
         try:
             line["batched"] = bench_batch(torch, fr)
@@ -467,6 +472,61 @@ def bench_newton_pseudo(torch, fr, steps=12, prev_dt_run=False):
             **runs, "target_reduction": target,
             "tangent_s_to_target": tt, "complex_step_s_to_target": tc,
             "speedup_at_equal_reduction": (tc / tt) if (tt and tc) else None}
+
+
+def bench_rows(torch, fr, reps=50):
+    """Cost of the widened rows on their own workloads (stream time per launch
+    from HIP events over `reps` launches):
+    * positivity-preserving log variables (-cond_sys): the IFunction and the
+      tangent product on the bump 256x64 p=5 with and without the change of
+      variables; the difference is the map (k_p2c / k_p2c_tan), 16 B/DOF;
+    * radial coordinates: the residual kernel on 128x128 p=4 quads (y >= 0)
+      with and without the source (1/r per node read in the last phase)."""
+    from fr_sdg_amd import configs as CF
+    s = torch.cuda.current_stream()
+
+    def t_of(fn):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        a, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        for _ in range(reps):
+            fn()
+        e.record(s)
+        e.synchronize()
+        return a.elapsed_time(e) / reps / 1e3
+
+    out = {}
+    res = {}
+    for cs in (0, 1):
+        cfg = dict(CF.bump(tangent_bc_mode=0), cond_sys=cs)
+        ev = fr.PhysicsEvaluator(cfg)
+        u = ev.impose_ic(fr.IC_UNIFORM, CF.bump_freestream())
+        u.mul_(1.0 + 1e-3 * torch.sin(torch.arange(ev.ndofs, device="cuda", dtype=torch.float64)))
+        ud = torch.zeros_like(u)
+        v = torch.cos(torch.arange(ev.ndofs, device="cuda", dtype=torch.float64))
+        o = ev.empty()
+        res[cs] = (t_of(lambda: ev.fun_implicit(u, ud, out=o)), t_of(lambda: ev.jvp(u, v, 1.0, out=o)), ev.ndofs)
+        del ev, u, ud, v, o
+        torch.cuda.empty_cache()
+    nd = res[0][2]
+    dif = res[1][0] - res[0][0]
+    out["log_variables"] = {"workload": "bump 256x64 p=5 (2.36 M DOFs)", "ifunction_s": res[0][0],
+                            "ifunction_cond_sys_s": res[1][0], "jvp_s": res[0][1], "jvp_cond_sys_s": res[1][1],
+                            "map_overhead_s": dif, "map_GBps": (16.0 * nd / dif / 1e9) if dif > 0 else None}
+    rad = {}
+    for cs in (0, 1):
+        cfg = dict(dims=2, nx=128, ny=128, p=4, lx=10.0, ly=10.0, riemann=1, bc_type=[1, 1, 1, 1], coord_sys=cs)
+        ev = fr.PhysicsEvaluator(cfg)
+        u = ev.impose_ic(fr.IC_VORTEX, [1.0, 1.0, 5.0, 1.0, 10.0])
+        o = ev.empty()
+        rad[cs] = t_of(lambda: ev.residual_kernel(u, out=o))
+        del ev, u, o
+        torch.cuda.empty_cache()
+    out["radial"] = {"workload": "128x128 p=4 quads on [0,10]^2, HLLC (1.64 M DOFs)", "residual_s": rad[0],
+                     "residual_radial_s": rad[1], "overhead": rad[1] / rad[0] - 1.0}
+    return out
 
 
 def bench_newton(torch, fr, steps=6):
