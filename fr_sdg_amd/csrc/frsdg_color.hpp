@@ -324,22 +324,34 @@ namespace frsdg
             const double piv = rowk[k];
             if (tid == 0 && ::fabs(piv) <= 1e-12) info[0] = 1; // PETSc's default zero-pivot tolerance
             const double rpiv = 1.0 / piv;
+            // every thread runs the same 81 FMAs: rows at or above the pivot get
+            // a zero multiplier and columns at or left of it a zero pivot-row
+            // entry (selects, not per-element branches); all 144 entries of both
+            // buffers were written this step
+            double w[9], l[9];
+#pragma unroll
+            for (int j = 0; j < 9; ++j)
+            {
+                const double v = rowk[ty + 16 * j];
+                w[j] = (ty + 16 * j > k) ? v : 0.0;
+            }
 #pragma unroll
             for (int i = 0; i < 9; ++i)
             {
-                const int r = tx + 16 * i;
-                if (r <= k || r >= nd) continue;
-                const double l = colk[r] * rpiv;
+                const double v = colk[tx + 16 * i];
+                l[i] = (tx + 16 * i > k && tx + 16 * i < nd) ? v * rpiv : 0.0;
+            }
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+#pragma unroll
+                for (int j = 0; j < 9; ++j) x[i][j] = ::fma(-l[i], w[j], x[i][j]);
+            // the pivot column below the diagonal becomes the multipliers
+            if (ty == kt)
 #pragma unroll
                 for (int j = 0; j < 9; ++j)
-                {
-                    const int c = ty + 16 * j;
-                    if (c == k)
-                        x[i][j] = l;
-                    else if (c > k)
-                        x[i][j] = ::fma(-l, rowk[c], x[i][j]);
-                }
-            }
+                    if (j == kb)
+#pragma unroll
+                        for (int i = 0; i < 9; ++i) x[i][j] = (tx + 16 * i > k) ? l[i] : x[i][j];
         }
 #pragma unroll
         for (int i = 0; i < 9; ++i)
@@ -380,16 +392,22 @@ namespace frsdg
             if (tid > k && tid < nd) colk[tid] = nxt;
             if (tid < nd) nxt = D[tid + static_cast<long long>(k + 1) * nd];
             __syncthreads();
+            // row k of X is zero right of column k (X is unit lower triangular),
+            // so every thread runs the same 81 FMAs and rows at or above the
+            // pivot get a zero multiplier (selects, not per-element branches)
+            double w[9], l[9];
+#pragma unroll
+            for (int j = 0; j < 9; ++j) w[j] = rowk[ty + 16 * j];
 #pragma unroll
             for (int i = 0; i < 9; ++i)
             {
-                const int r = tx + 16 * i;
-                if (r <= k || r >= nd) continue;
-                const double l = colk[r];
-#pragma unroll
-                for (int j = 0; j < 9; ++j)
-                    if (ty + 16 * j <= k) x[i][j] = ::fma(-l, rowk[ty + 16 * j], x[i][j]);
+                const double v = colk[tx + 16 * i];
+                l[i] = (tx + 16 * i > k && tx + 16 * i < nd) ? v : 0.0;
             }
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+#pragma unroll
+                for (int j = 0; j < 9; ++j) x[i][j] = ::fma(-l[i], w[j], x[i][j]);
         }
 #pragma unroll
         for (int i = 0; i < 9; ++i)
@@ -440,16 +458,22 @@ namespace frsdg
                             rowk[ty + 16 * j] = x[i][j];
                         }
             __syncthreads();
+            // row k of X is zero left of column k (X is upper triangular), so
+            // every thread runs the same 81 FMAs and rows at or below the pivot
+            // get a zero multiplier (selects, not per-element branches)
+            double w[9], u[9];
+#pragma unroll
+            for (int j = 0; j < 9; ++j) w[j] = rowk[ty + 16 * j];
 #pragma unroll
             for (int i = 0; i < 9; ++i)
             {
-                const int r = tx + 16 * i;
-                if (r >= k) continue;
-                const double u = colk[r];
-#pragma unroll
-                for (int j = 0; j < 9; ++j)
-                    if (ty + 16 * j >= k) x[i][j] = ::fma(-u, rowk[ty + 16 * j], x[i][j]);
+                const double v = colk[tx + 16 * i];
+                u[i] = (tx + 16 * i < k) ? v : 0.0;
             }
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+#pragma unroll
+                for (int j = 0; j < 9; ++j) x[i][j] = ::fma(-u[i], w[j], x[i][j]);
         }
 #pragma unroll
         for (int i = 0; i < 9; ++i)
