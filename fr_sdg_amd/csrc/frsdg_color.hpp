@@ -549,59 +549,90 @@ namespace frsdg
     // T2_J = L_II^-1 A_IJ for the upper blocks (k_ilu_ustep) and their store
     // (k_ilu_ucopy).  Scratch per row position b: T1 at T + b (1 + CMAXN)
     // blocks, T2_q after it.
+    //
+    // A tile is nine 16 x 16 sub-tiles on the FP64 matrix cores
+    // (v_mfma_f64_16x16x4f64: lane l supplies A[l % 16][l / 16] and
+    // B[l / 16][l % 16] of a 16x4 by 4x16 product and holds D[l / 16 + 4 v][l % 16]
+    // in accumulator v); wave w takes sub-tiles w, w + 4, w + 8.  The cores
+    // compute the transposed product B^T A^T, so that a lane's results run
+    // down a column of the column-major block and the stores stay coalesced.
+    // The k dimension is staged through LDS in 16-deep slabs, the next slab
+    // loaded into registers while the cores work on the current one.
     constexpr int TT = 48, TKB = 16;
+    typedef double dbl4 __attribute__((ext_vector_type(4)));
     __device__ inline void tile_gemm(int nd, int tr, int tc, const double* __restrict__ A,
                                      const double* __restrict__ B, double* __restrict__ C, bool sub,
                                      double* __restrict__ As, double* __restrict__ Bs)
     {
-        const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+        const int tid = threadIdx.x, w = tid >> 6, li = tid & 15, lk = (tid >> 4) & 3;
         const int r0 = tr * TT, c0 = tc * TT;
-        double acc[3][3];
+        dbl4 acc[3];
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+        for (int m = 0; m < 3; ++m) acc[m] = dbl4{0.0, 0.0, 0.0, 0.0};
+        // 48 x 16 values of A (rows r0.., k) and 16 x 48 of B (k, columns c0..) per slab, 3 each per thread
+        double pa[3], pb[3];
+        auto fetch = [&](int k0) {
 #pragma unroll
-            for (int j = 0; j < 3; ++j) acc[i][j] = 0.0;
+            for (int m = 0; m < 3; ++m)
+            {
+                const int o = tid + 256 * m;
+                const int q = o / TT, s = o - q * TT;    // A: column k0 + q of the tile rows, s down the column
+                const int r = r0 + s, ka = k0 + q;
+                pa[m] = (r < nd && ka < nd) ? A[r + static_cast<long long>(ka) * nd] : 0.0;
+                const int cq = o / TKB, cs = o - cq * TKB; // B: column c0 + cq, cs along k
+                const int c = c0 + cq, kb = k0 + cs;
+                pb[m] = (c < nd && kb < nd) ? B[kb + static_cast<long long>(c) * nd] : 0.0;
+            }
+        };
+        fetch(0);
         for (int k0 = 0; k0 < nd; k0 += TKB)
         {
 #pragma unroll
-            for (int m = 0; m < 3; ++m) // 48 x 16 values of A and 16 x 48 of B, 3 each per thread
+            for (int m = 0; m < 3; ++m)
             {
-                const int o  = tid + 256 * m;
-                const int kk = o / TT, rr = o - kk * TT;
-                const int r = r0 + rr, k = k0 + kk;
-                As[kk * TT + rr] = (r < nd && k < nd) ? A[r + static_cast<long long>(k) * nd] : 0.0;
-                const int cc = o / TKB, kb = o - cc * TKB;
-                const int c = c0 + cc, k2 = k0 + kb;
-                Bs[kb * TT + cc] = (c < nd && k2 < nd) ? B[k2 + static_cast<long long>(c) * nd] : 0.0;
+                const int o = tid + 256 * m;
+                const int q = o / TT, s = o - q * TT, cq = o / TKB, cs = o - cq * TKB;
+                As[q * TT + s]   = pa[m]; // As[k][row]
+                Bs[cs * TT + cq] = pb[m]; // Bs[k][column]
             }
             __syncthreads();
+            if (k0 + TKB < nd) fetch(k0 + TKB);
 #pragma unroll
-            for (int kk = 0; kk < TKB; ++kk)
+            for (int ks = 0; ks < TKB; ks += 4)
             {
-                double a[3], b[3];
 #pragma unroll
-                for (int i = 0; i < 3; ++i) a[i] = As[kk * TT + tx + 16 * i];
-#pragma unroll
-                for (int j = 0; j < 3; ++j) b[j] = Bs[kk * TT + ty + 16 * j];
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) acc[i][j] = ::fma(a[i], b[j], acc[i][j]);
+                for (int m = 0; m < 3; ++m)
+                {
+                    const int t = w + 4 * m;
+                    if (t < 9) // uniform per wave
+                    {
+                        const int ti = t % 3, tj = t / 3;
+                        const double bt = Bs[(ks + lk) * TT + 16 * tj + li]; // (B^T)[l % 16][l / 16]
+                        const double at = As[(ks + lk) * TT + 16 * ti + li]; // (A^T)[l / 16][l % 16]
+                        acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(bt, at, acc[m], 0, 0, 0);
+                    }
+                }
             }
             __syncthreads();
         }
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
+        for (int m = 0; m < 3; ++m)
+        {
+            const int t = w + 4 * m;
+            if (t >= 9) continue;
+            const int ti = t % 3, tj = t / 3;
+            const int r = r0 + 16 * ti + li;
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+            for (int v = 0; v < 4; ++v)
             {
-                const int r = r0 + tx + 16 * i, c = c0 + ty + 16 * j;
+                const int c = c0 + 16 * tj + lk + 4 * v; // (B^T A^T)[l / 16 + 4 v][l % 16] = (A B)[r][c]
                 if (r < nd && c < nd)
                 {
                     const long long at = r + static_cast<long long>(c) * nd;
-                    C[at] = sub ? C[at] - acc[i][j] : acc[i][j];
+                    C[at] = sub ? C[at] - acc[m][v] : acc[m][v];
                 }
             }
+        }
     }
 
     __device__ inline void tile_copy(int nd, int tr, int tc, const double* __restrict__ src, double* __restrict__ dst)
