@@ -838,7 +838,65 @@ namespace frsdg
             el[q] = pos_tab[pos * 2 * CMAXN + CMAXN + q];
         }
         double acc = 0.0;
-        if (r < nd)
+        if constexpr (ND > 0)
+        {
+            // The row's blocks (the diagonal inverse, then the neighbours'
+            // scaled blocks, packed in pos_tab) in batches of SWB: every load of
+            // a batch is issued before its first FMA, so a row pays one memory
+            // latency per batch rather than one per block.  Addresses are
+            // clamped into the block (rows and columns past ND are discarded or
+            // multiplied by zero), so no load is masked.  Per thread the sums
+            // run in the same order as sweep_dot's.
+            constexpr int NC = (ND + 15) / 16, SWB = 5;
+            int nb = 1;
+#pragma unroll
+            for (int q = 0; q < CMAXN; ++q) nb += sl[q] >= 0 ? 1 : 0;
+            const int rr = r < ND ? r : ND - 1;
+#pragma unroll
+            for (int b0 = 0; b0 <= CMAXN; b0 += SWB)
+            {
+                if (b0 >= nb) break; // uniform per workgroup
+                double bv[SWB][NC], vv[SWB][NC];
+#pragma unroll
+                for (int t = 0; t < SWB; ++t)
+                {
+                    const int qq = b0 + t;
+                    if (qq > CMAXN) break;
+                    if (qq < nb)
+                    {
+                        const double* B = qq == 0 ? Dinv + I * 2 * bsz + (FWD ? 0 : bsz) : P + (I * S + sl[qq - 1]) * bsz;
+                        const double* V = qq == 0 ? in + I * nd : out + static_cast<long long>(el[qq - 1]) * nd;
+#pragma unroll
+                        for (int i = 0; i < NC; ++i)
+                        {
+                            const int c = g + 16 * i, cc = c < ND ? c : ND - 1;
+                            bv[t][i] = B[rr + cc * ND];
+                            const double x = V[cc];
+                            vv[t][i] = c < ND ? x : 0.0;
+                        }
+                    }
+                    else
+                    {
+#pragma unroll
+                        for (int i = 0; i < NC; ++i) bv[t][i] = vv[t][i] = 0.0;
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < SWB; ++t)
+                {
+                    const int qq = b0 + t;
+                    if (qq > CMAXN) break;
+                    double part = 0.0;
+#pragma unroll
+                    for (int i = 0; i < NC; ++i) part = ::fma(bv[t][i], vv[t][i], part);
+                    if (qq == 0)
+                        acc = part;
+                    else if (qq < nb)
+                        acc -= part;
+                }
+            }
+        }
+        else if (r < nd)
         {
             acc = sweep_dot<ND>(nd, r, g, Dinv + I * 2 * bsz + (FWD ? 0 : bsz), in + I * nd);
 #pragma unroll
