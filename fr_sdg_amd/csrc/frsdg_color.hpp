@@ -1020,4 +1020,117 @@ namespace frsdg
             if (lane == 0 && r < nd) out[I * nd + r] = a;
         }
     }
+
+    // The level recurrences of the transposed sweeps need only the neighbours'
+    // t (forward) or s (backward), not their diagonal products, so those
+    // products leave the level chain: each level runs k_ilu_sweepT_lvl,
+    //   aux_I = in_I - sum_q B_q^T aux_{m_q}   (B_q: row m_q's block at the mirrored slot),
+    // with a block row split over workgroups (16 entries each, 4 per wave,
+    // lanes along the blocks' contiguous columns) and the loads of SWB blocks
+    // issued before their FMAs; then one launch over all rows forms
+    // out_I = (U_II^-1)^T aux_I or (L_II^-1)^T aux_I (k_ilu_diagT).  Each
+    // entry's lane partial sums and their shuffle reduction follow
+    // k_ilu_sweepT's order, so the results are the same.
+    template <int ND>
+    __global__ __launch_bounds__(256) void k_ilu_sweepT_lvl(int S, const int* __restrict__ pos_tab,
+                                                            const int* __restrict__ lvl, int off,
+                                                            const double* __restrict__ P,
+                                                            const double* __restrict__ in, double* aux)
+    {
+        constexpr int NCH = (ND + 63) / 64, SWB = ND <= 144 ? 4 : (ND <= 256 ? 2 : 1);
+        constexpr long long bsz = static_cast<long long>(ND) * ND;
+        const long long pos = off + blockIdx.x;
+        const long long I   = lvl[pos];
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, rb = blockIdx.y * 16 + w * 4;
+        int sl[CMAXN], el[CMAXN];
+#pragma unroll
+        for (int q = 0; q < CMAXN; ++q)
+        {
+            sl[q] = pos_tab[pos * 2 * CMAXN + q];
+            el[q] = pos_tab[pos * 2 * CMAXN + CMAXN + q];
+        }
+        int nb = 0; // slots are packed: the row's neighbours first, then -1
+#pragma unroll
+        for (int q = 0; q < CMAXN; ++q) nb += sl[q] >= 0 ? 1 : 0;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int b0 = 0; b0 < CMAXN; b0 += SWB)
+        {
+            if (b0 >= nb) break; // uniform per workgroup
+            double bv[SWB][4][NCH], vv[SWB][NCH];
+#pragma unroll
+            for (int t = 0; t < SWB; ++t)
+            {
+                const int q = b0 + t;
+                if (q < nb)
+                {
+                    const double* B = P + (static_cast<long long>(el[q]) * S + (S - 1 - sl[q])) * bsz;
+                    const double* V = aux + static_cast<long long>(el[q]) * ND;
+#pragma unroll
+                    for (int ch = 0; ch < NCH; ++ch)
+                    {
+                        const int c = ch * 64 + lane, cc = c < ND ? c : ND - 1;
+                        const double x = V[cc];
+                        vv[t][ch] = c < ND ? x : 0.0;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                        {
+                            const int r = rb + j, rr = r < ND ? r : ND - 1;
+                            bv[t][j][ch] = B[cc + rr * ND];
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < SWB; ++t)
+                if (b0 + t < nb)
+#pragma unroll
+                    for (int ch = 0; ch < NCH; ++ch)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) acc[j] = ::fma(bv[t][j][ch], vv[t][ch], acc[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            const double a = wave_sum(acc[j]);
+            const int    r = rb + j;
+            if (lane == 0 && r < ND) aux[I * ND + r] = in[I * ND + r] - a;
+        }
+    }
+
+    // blockIdx.x: block row, blockIdx.y: 16 entries
+    template <bool FWD, int ND>
+    __global__ __launch_bounds__(256) void k_ilu_diagT(const double* __restrict__ Dinv, const double* __restrict__ aux,
+                                                       double* out)
+    {
+        constexpr int NCH = (ND + 63) / 64;
+        constexpr long long bsz = static_cast<long long>(ND) * ND;
+        const long long I = blockIdx.x;
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, rb = blockIdx.y * 16 + w * 4;
+        const double* B = Dinv + I * 2 * bsz + (FWD ? bsz : 0);
+        double bv[4][NCH], vv[NCH];
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch)
+        {
+            const int c = ch * 64 + lane, cc = c < ND ? c : ND - 1;
+            const double x = aux[I * ND + cc];
+            vv[ch] = c < ND ? x : 0.0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                const int r = rb + j, rr = r < ND ? r : ND - 1;
+                bv[j][ch] = B[cc + rr * ND];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            double acc = 0.0;
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) acc = ::fma(bv[j][ch], vv[ch], acc);
+            const double a = wave_sum(acc);
+            const int    r = rb + j;
+            if (lane == 0 && r < ND) out[I * ND + r] = a;
+        }
+    }
 } // namespace frsdg

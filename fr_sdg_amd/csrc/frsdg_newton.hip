@@ -453,12 +453,30 @@ namespace
                       double* z)
     {
         const int nlev = static_cast<int>(cp.lvl_ptr.size()) - 1;
-        for (int l = 0; l < nlev; ++l)
-            hipLaunchKernelGGL((frsdg::k_ilu_sweepT<true, ND>), dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l])),
-                               dim3(1024), 0, st, cp.S, cp.nd, cp.lpos, cp.lvl, cp.lvl_ptr[l], P, Dinv, x, cp.t, cp.y);
-        for (int l = nlev - 1; l >= 0; --l)
-            hipLaunchKernelGGL((frsdg::k_ilu_sweepT<false, ND>), dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l])),
-                               dim3(1024), 0, st, cp.S, cp.nd, cp.upos, cp.lvl, cp.lvl_ptr[l], P, Dinv, cp.y, cp.t, z);
+        if constexpr (ND > 0)
+        {
+            // level chains of t (then s) in cp.t; the diagonal products for all
+            // rows after each chain (y into cp.y, then z: x is no longer read)
+            const unsigned rch = static_cast<unsigned>((ND + 15) / 16);
+            const dim3     all(static_cast<unsigned>(cp.E), rch);
+            for (int l = 0; l < nlev; ++l)
+                hipLaunchKernelGGL((frsdg::k_ilu_sweepT_lvl<ND>), dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l]), rch),
+                                   dim3(256), 0, st, cp.S, cp.lpos, cp.lvl, cp.lvl_ptr[l], P, x, cp.t);
+            hipLaunchKernelGGL((frsdg::k_ilu_diagT<true, ND>), all, dim3(256), 0, st, Dinv, cp.t, cp.y);
+            for (int l = nlev - 1; l >= 0; --l)
+                hipLaunchKernelGGL((frsdg::k_ilu_sweepT_lvl<ND>), dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l]), rch),
+                                   dim3(256), 0, st, cp.S, cp.upos, cp.lvl, cp.lvl_ptr[l], P, cp.y, cp.t);
+            hipLaunchKernelGGL((frsdg::k_ilu_diagT<false, ND>), all, dim3(256), 0, st, Dinv, cp.t, z);
+        }
+        else
+        {
+            for (int l = 0; l < nlev; ++l)
+                hipLaunchKernelGGL((frsdg::k_ilu_sweepT<true, ND>), dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l])),
+                                   dim3(1024), 0, st, cp.S, cp.nd, cp.lpos, cp.lvl, cp.lvl_ptr[l], P, Dinv, x, cp.t, cp.y);
+            for (int l = nlev - 1; l >= 0; --l)
+                hipLaunchKernelGGL((frsdg::k_ilu_sweepT<false, ND>), dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l])),
+                                   dim3(1024), 0, st, cp.S, cp.nd, cp.upos, cp.lvl, cp.lvl_ptr[l], P, Dinv, cp.y, cp.t, z);
+        }
     }
     int ilu_apply_T(SolverWs* s, hipStream_t st, const double* P, const double* Dinv, const double* x, double* z)
     {
