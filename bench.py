@@ -1,20 +1,25 @@
 """Benchmark: FP64 DOF-RHS evals/s of the FR residual on one MI355X.
 
-Workload (one "step" = one SSP-RK3 time step = 3 fused residual+stage
-launches): 2D isentropic vortex, periodic [0,10]^2 (the reference's box with
-the vortex at its centre), 128x128 affine quads, p=4 (25 nodes/element, 4
-conserved variables, 1,638,400 DOFs), HLLC, ideal gas gamma=1.4, beta=5,
-stream (1,1,1,1), CFL 0.4/(2p+1) = 0.044 on the element size.  Steps are
-replayed from a captured hipGraph.
+Headline workload (one "step" = one SSP-RK3 time step = 3 launches of the
+fused residual + stage kernel k_res2d_stage<8, curved>, the post-step
+spectral filter fused into the third): the large curved radial shock tube,
+[-1,1]^2 warped by x += 0.05 sin(pi x) sin(pi y) (same in y), 1024^2 quads,
+p=7 (64 nodes/element, 268,435,456 DOFs, 2.15 GB per state), van der Waals
+gas (a=0.1, b=0.02), HLLC, exponential filter (order 8, alpha 36), CFL
+0.4/(2p+1).  The stepping graph (16 steps) is captured, instantiated and
+replayed during the warm-up; the timed region replays it (plus plain
+launches for steps % 16), inputs resident in HBM.
 
-    python bench.py [--gpus 1] [--steps 1000] [--warmup 50]
+    python bench.py [--gpus 1] [--steps 200] [--warmup 32]
 
-Prints ONE JSON line.  Extra fields: tangent vs complex-step J*v kernel time
-(same GPU), and the residual at the large curved configuration (1024^2 warped
-quads, p=7, van der Waals, HLLC) where the state (2.15 GB) is far larger than
-the caches.
+Prints ONE JSON line.  Extra fields: the 2D isentropic vortex (128^2, p=4,
+HLLC, periodic) stepped the same way; the residual-only and J*v kernels;
+the JFNK comparison on the bump; the batched sweep; measured peaks (HBM copy
+of a 2.15 GB state, FP64 FMA loop); the CPU baseline (the oracle on the same
+state, all host threads and one thread).
 """
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -32,30 +37,34 @@ sys.path.insert(0, ROOT)
 METRIC = "FP64 DOF-RHS evals/s + Newton step time (tangent vs complex-step), 1 GPU"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
+GRAPH_CHUNK = 16  # steps per captured graph in frsdg_ssprk_run
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-large", action="store_true", help="skip the large curved residual")
+    ap.add_argument("--no-large", action="store_true", help="skip the residual-only / limiter figures")
     ap.add_argument("--no-newton", action="store_true", help="skip the implicit (JFNK) comparison")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep comparison")
+    ap.add_argument("--headline-only", action="store_true", help="only the headline step (profiling runs)")
     ap.add_argument("--newton-pseudo", action="store_true",
                     help="add the pseudo-transient tangent vs complex-step comparison with ILU(0) of the "
                          "coloured matrix (about 2 minutes)")
     ap.add_argument("--ilu", action="store_true",
                     help="add the coloured-matrix / ILU(0) preconditioner timings on the bump (about 30 s)")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the all-core CPU baseline (0: OMP_NUM_THREADS, else min(16, cores))")
     ap.add_argument("--adjoint-large", action="store_true",
                     help="run only the unsteady adjoint of int(rho^2) on the 1024^2 p=7 warped mesh")
     ap.add_argument("--adj-steps", type=int, default=2000)
     ap.add_argument("--adj-snapshots", type=int, default=32)
     ap.add_argument("--adj-n", type=int, default=1024)
-    ap.add_argument("--kernel-only", choices=["stage", "residual", "tangent", "cstep", "adjoint", "large", "limiter",
-                                                "copy", "batch", "batch_each", "ilu"], default=None,
+    ap.add_argument("--kernel-only", choices=["step", "stage", "residual", "tangent", "cstep", "adjoint", "large",
+                                                "limiter", "copy", "vcopy", "batch", "batch_each", "ilu"],
+                    default=None,
                     help="profiling mode: launch only this kernel `--steps` times, print nothing else")
     return ap.parse_args()
 
@@ -85,16 +94,116 @@ def time_kernel(torch, fn, reps):
     return a.elapsed_time(b) / 1e3 / reps
 
 
+def stepping_setup(torch, fr, which):
+    """(evaluator, initial state, dt, cfg, workload text) of a stepped workload:
+    'large' = the curved radial shock tube (headline), 'vortex' = the periodic
+    isentropic vortex.  dt = CFL * h / max(|v| + c) at t = 0, CFL 0.4/(2p+1)."""
+    from fr_sdg_amd import configs as CF
+    if which == "large":
+        cfg = CF.warped(1024)
+        ev = fr.PhysicsEvaluator(cfg)
+        u0 = ev.impose_ic(fr.IC_EXPLOSION, CF.WARPED_PRM)
+        h = cfg["lx"] / cfg["nx"]
+        text = ("curved radial shock tube: [-1,1]^2 warped by 0.05 sin(pi x) sin(pi y), 1024x1024 quads, p=7, "
+                "van der Waals a=0.1 b=0.02, HLLC, exponential filter order 8 alpha 36 (post-step, fused), "
+                "SSP-RK3 at CFL 0.4/(2p+1); jump r=0.4 (1,1)/(0.125,0.1) spread over a tanh layer of width 0.1")
+    else:
+        cfg = vortex_cfg()
+        ev = fr.PhysicsEvaluator(cfg)
+        u0 = ev.impose_ic(fr.IC_VORTEX, VORTEX_PRM)
+        h = cfg["lx"] / cfg["nx"]
+        text = "2D isentropic vortex 128x128 p=4 HLLC ideal, periodic, SSP-RK3 at CFL 0.4/(2p+1)"
+    dt = CF.explicit_cfl(cfg["p"]) * h / CF.sound_speed_max(u0, cfg)
+    return ev, u0, dt, cfg, text
+
+
+def timed_steps(torch, ev, u, dt, steps, warmup):
+    """Warm up with whole graph chunks (the graph of these exact u/dt is
+    captured and instantiated there), then time `steps` steps with HIP events
+    on the launch stream.  Returns (seconds, wall seconds)."""
+    w = max(GRAPH_CHUNK, GRAPH_CHUNK * ((warmup + GRAPH_CHUNK - 1) // GRAPH_CHUNK))
+    ev.ssprk_run(u, dt, w)
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_wall0 = time.perf_counter()
+    e0.record(s)
+    ev.ssprk_run(u, dt, steps)
+    e1.record(s)
+    e1.synchronize()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3, time.perf_counter() - t_wall0
+
+
+def step_bytes_per_dof(n, curved):
+    """Algorithmic HBM bytes of one SSP-RK3 step per DOF: stage 1 reads u and
+    writes w1 (16), stages 2 and 3 also read u^n (24 each); curved meshes add
+    per launch the node metrics (8) and the face metrics (nx, ny, scale per
+    face node: 12/n)."""
+    return 64.0 + (3.0 * (8.0 + 12.0 / n) if curved else 0.0)
+
+
+def load_peaks_lib():
+    from fr_sdg_amd import build as B
+    if not os.path.exists(B.PEAKS_LIB):
+        raise RuntimeError(f"{B.PEAKS_LIB} not built; run __graft_entry__.build()")
+    L = ctypes.CDLL(B.PEAKS_LIB)
+    L.frsdg_peak_fma.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    L.frsdg_peak_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int,
+                                  ctypes.c_void_p]
+    return L
+
+
+def measured_peaks(torch):
+    """HBM copy bandwidth on a 2.15 GB state (the headline's state size; read +
+    write counted) and the FP64 vector FMA-loop rate, beside the datasheet
+    figures."""
+    L = load_peaks_lib()
+    s = torch.cuda.current_stream()
+    st = ctypes.c_void_p(s.cuda_stream)
+    n = 1024 * 1024 * 64 * 4  # doubles of one headline state
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    blocks = 256 * 8 * 4
+    run = lambda: L.frsdg_peak_copy(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()), n, blocks, st)
+    if run():
+        raise RuntimeError("copy probe failed")
+    t_copy = time_kernel(torch, run, 20)
+    t_torch = time_kernel(torch, lambda: b.copy_(a), 20)
+    del a, b
+    fblocks, iters = 256 * 4 * 8, 4096
+    o = torch.empty(fblocks * 256, dtype=torch.float64, device="cuda")
+    runf = lambda: L.frsdg_peak_fma(ctypes.c_void_p(o.data_ptr()), fblocks, iters, st)
+    if runf():
+        raise RuntimeError("fma probe failed")
+    t_fma = time_kernel(torch, runf, 10)
+    fl = 16.0 * iters * fblocks * 256
+    del o
+    torch.cuda.empty_cache()
+    return {"copy_GBps": 16.0 * n / t_copy / 1e9, "copy_torch_GBps": 16.0 * n / t_torch / 1e9,
+            "copy_bytes": 16.0 * n, "fp64_fma_TFLOPs": fl / t_fma / 1e12,
+            "datasheet": {"hbm_GBps": HBM_PEAK_GBS, "fp64_vector_TFLOPs": FP64_PEAK_TFS}}
+
+
+def read_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary (JSON)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch"), name
+
+
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; use --gpus 1",
+        print(json.dumps({"error": "this build runs on one GPU only (no multi-GPU path); use --gpus 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
 
     import fr_sdg_amd as fr
-    from fr_sdg_amd import configs as CF
 
     torch.cuda.set_device(0)
     if args.kernel_only:
@@ -102,63 +211,26 @@ def main():
     if args.adjoint_large:
         print(json.dumps(adjoint_large(torch, fr, args)))
         return
-    cfg = vortex_cfg()
-    ev = fr.PhysicsEvaluator(cfg)
+
+    # ---------------- headline: curved radial shock tube, graph-replayed SSP-RK3 ----------------
+    ev, u0, dt, cfg, text = stepping_setup(torch, fr, "large")
     n = cfg["p"] + 1
     E = cfg["nx"] * cfg["ny"]
     ndof = ev.ndofs
-    u0 = ev.impose_ic(fr.IC_VORTEX, VORTEX_PRM)
-    # dt = CFL * h / max(|u| + c) at t = 0
-    st = u0.view(-1, 4)
-    rho = st[:, 0]
-    uu = st[:, 1] / rho
-    vv = st[:, 2] / rho
-    pres = 0.4 * (st[:, 3] - 0.5 * rho * (uu * uu + vv * vv))
-    c = torch.sqrt(1.4 * pres / rho)
-    smax = float(torch.max(torch.sqrt(uu * uu + vv * vv) + c))
-    h = cfg["lx"] / cfg["nx"]
-    dt = CF.explicit_cfl(cfg["p"]) * h / smax  # CFL 0.044 at p = 4 (0.1 diverges within ~30 steps)
-
     u = u0.clone()
-    ev.ssprk_run(u, dt, args.warmup)
-    torch.cuda.synchronize()
-    s = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t_wall0 = time.perf_counter()
-    e0.record(s)
-    ev.ssprk_run(u, dt, args.steps)
-    e1.record(s)
-    e1.synchronize()
-    torch.cuda.synchronize()
-    t_wall = time.perf_counter() - t_wall0
-    elapsed = e0.elapsed_time(e1) / 1e3
+    elapsed, t_wall = timed_steps(torch, ev, u, dt, args.steps, args.warmup)
     if not bool(torch.isfinite(u).all()):
         raise RuntimeError("non-finite state after the timed steps")
     n_res = 3 * args.steps
     value = ndof * n_res / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
-    # dominant kernel: fused residual + SSP-RK stage, 3 per step.
-    # algorithmic HBM bytes per step: stage 1 reads u, writes w1 (16 B/DOF);
-    # stages 2,3 read u_s and u^n, write (24 B/DOF each) -> 64 B/DOF per step.
-    bytes_launch = 64.0 * ndof / 3.0
+    bytes_launch = step_bytes_per_dof(n, True) * ndof / 3.0
     t_launch = elapsed / n_res
     achieved = bytes_launch / t_launch / 1e9
-    flops_launch = flops_per_element(n) * E
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_vortex_p4_128.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-
-    # tangent vs complex-step J*v kernels (same GPU, same state)
-    v = torch.randn(ndof, dtype=torch.float64, device="cuda")
-    out = ev.empty()
-    t_tan = time_kernel(torch, lambda: ev.jvp(u, v, 1.0, out=out), 50)
-    t_cs = time_kernel(torch, lambda: ev.jvp_cstep(u, v, 1.0, 1e-7, out=out), 50)
-    R = ev.empty()
-    t_res = time_kernel(torch, lambda: ev.residual_kernel(u, out=R), 100)
-
+    # FP64 work per launch: the residual (HLLC, ideal-gas count) + the filter's
+    # two line sweeps (16 n^3 per element) once per step
+    flops_launch = (flops_per_element(n) + 16 * n ** 3 / 3.0) * E
+    traffic, tsrc = read_traffic("traffic_step_warped_p7_1024.json")
     line = {
         "metric": METRIC,
         "value": value,
@@ -172,72 +244,93 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "2D isentropic vortex 128x128 p=4 HLLC ideal, SSP-RK3, CFL 0.044",
-                   "ndofs": ndof, "elements": E, "p": cfg["p"], "dt": dt,
-                   "residual_evals_per_step": 3, "graph_captured": True},
+        "config": {"workload": text, "ndofs": ndof, "elements": E, "p": cfg["p"], "dt": dt,
+                   "residual_evals_per_step": 3, "graph_captured": True,
+                   "graph_steps": GRAPH_CHUNK, "warmup_steps_run": max(GRAPH_CHUNK, GRAPH_CHUNK * (
+                       (args.warmup + GRAPH_CHUNK - 1) // GRAPH_CHUNK))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_res2d<5,IOStage> (fused residual + SSP-RK stage)",
-                     "bytes_per_launch_alg": bytes_launch, "avg_launch_s": t_launch,
+                     "kernel": "k_res2d_stage<8,curved> (residual + SSP-RK stage; the filter fused into every "
+                               "third launch), 3 launches per step",
+                     "bytes_per_launch_alg": bytes_launch, "bytes_per_dof_step": step_bytes_per_dof(n, True),
+                     "avg_launch_s": t_launch, "traffic_source": tsrc,
                      "fp64_tflops": flops_launch / t_launch / 1e12,
                      "fp64_frac": flops_launch / t_launch / 1e12 / FP64_PEAK_TFS},
-        "residual_only": {"avg_s": t_res, "dof_rhs_per_s": ndof / t_res,
-                          "gbs_alg": 16.0 * ndof / t_res / 1e9},
-        "jvp": {"tangent_s": t_tan, "complex_step_s": t_cs, "speedup": t_cs / t_tan},
         "wall_s_timed": t_wall,
     }
-
-    progress("headline done")
-    if not args.no_large:
-        try:
-            line["large_curved"] = bench_large(torch, fr)
-        except Exception as exc:  # report, keep the headline line valid
-            line["large_curved"] = {"error": repr(exc)}
-        progress("large curved done")
-    if not args.no_newton:
-        try:
-            line["newton"] = bench_newton(torch, fr)
-        except Exception as exc:
-            line["newton"] = {"error": repr(exc)}
-        progress("newton done")
-        if args.ilu:
-            try:
-                line["ilu"] = bench_ilu(torch, fr)
-            except Exception as exc:
-                line["ilu"] = {"error": repr(exc)}
-            progress("ilu done")
-        if args.newton_pseudo:
-            try:
-                line["newton_pseudo"] = bench_newton_pseudo(torch, fr, prev_dt_run=True)
-            except Exception as exc:
-                line["newton_pseudo"] = {"error": repr(exc)}
-            progress("newton pseudo done")
-    try:
-        line["widened_rows"] = bench_rows(torch, fr)
-    except Exception as exc:
-        line["widened_rows"] = {"error": repr(exc)}
-    progress("widened rows done")
-    if not args.no_batch:
-        try:
-            line["batched"] = bench_batch(torch, fr)
-        except Exception as exc:
-            line["batched"] = {"error": repr(exc)}
-        progress("batched done")
-        try:
-            # the same sweep on meshes 1/4 the size (N in 8..48): launch-bound instances
-            # (20 steps: within 100 steps shocks form and the filtered p=5,6
-            # instances lose positivity on these coarse meshes)
-            line["batched_small"] = bench_batch(torch, fr, steps=20, scale=0.25)
-        except Exception as exc:
-            line["batched_small"] = {"error": repr(exc)}
-
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    del u
+    progress(f"headline done: {ms_per_step:.3f} ms/step, {achieved / HBM_PEAK_GBS:.3f} of HBM peak")
+    if not args.headline_only:
+        extra_legs(torch, fr, args, line, ev, u0)
+    del ev, u0
+    torch.cuda.empty_cache()
+    if not args.no_cpu and not args.headline_only:
+        line["cpu_baseline"] = cpu_baseline(args, dt)
+        progress("cpu baseline done")
     print(json.dumps(line))
 
 
+def extra_legs(torch, fr, args, line, ev, u0):
+    """Extra keys of the bench line (each leg reports its error instead of failing the line)."""
+    def leg(key, fn):
+        try:
+            line[key] = fn()
+        except Exception as exc:  # report, keep the headline line valid
+            line[key] = {"error": repr(exc)}
+        progress(f"{key} done")
+
+    leg("measured_peaks", lambda: measured_peaks(torch))
+    if not args.no_large:
+        leg("large_curved", lambda: bench_large(torch, fr, ev, u0))
+    leg("vortex", lambda: bench_vortex(torch, fr, args))
+    if not args.no_newton:
+        leg("newton", lambda: bench_newton(torch, fr))
+        if args.ilu:
+            leg("ilu", lambda: bench_ilu(torch, fr))
+        if args.newton_pseudo:
+            leg("newton_pseudo", lambda: bench_newton_pseudo(torch, fr, prev_dt_run=True))
+    leg("widened_rows", lambda: bench_rows(torch, fr))
+    if not args.no_batch:
+        leg("batched", lambda: bench_batch(torch, fr))
+        # the same sweep on meshes 1/4 the size (N in 8..48): launch-bound instances
+        # (20 steps: within 100 steps shocks form and the filtered p=5,6
+        # instances lose positivity on these coarse meshes)
+        leg("batched_small", lambda: bench_batch(torch, fr, steps=20, scale=0.25))
+
+
+def bench_vortex(torch, fr, args):
+    """The periodic isentropic vortex (128^2, p=4, affine, 13 MB per state:
+    cache-resident) stepped like the headline, plus its residual-only and
+    tangent / complex-step J*v kernels."""
+    ev, u0, dt, cfg, text = stepping_setup(torch, fr, "vortex")
+    ndof = ev.ndofs
+    u = u0.clone()
+    steps = max(args.steps, 160)
+    elapsed, _ = timed_steps(torch, ev, u, dt, steps, args.warmup)
+    t_launch = elapsed / (3 * steps)
+    v = torch.randn(ndof, dtype=torch.float64, device="cuda")
+    out = ev.empty()
+    t_tan = time_kernel(torch, lambda: ev.jvp(u, v, 1.0, out=out), 50)
+    t_cs = time_kernel(torch, lambda: ev.jvp_cstep(u, v, 1.0, 1e-7, out=out), 50)
+    t_vjp = time_kernel(torch, lambda: ev.vjp(u, v, 1.0, out=out), 50)
+    t_res = time_kernel(torch, lambda: ev.residual_kernel(u, out=out), 100)
+    bytes_launch = step_bytes_per_dof(5, False) * ndof / 3.0
+    res = {"workload": text, "ndofs": ndof, "steps": steps, "dt": dt, "ms_per_step": 1e3 * elapsed / steps,
+           "dof_rhs_per_s": 3 * ndof * steps / elapsed, "avg_launch_s": t_launch,
+           "hbm_frac_alg": bytes_launch / t_launch / 1e9 / HBM_PEAK_GBS,
+           "residual_only": {"avg_s": t_res, "dof_rhs_per_s": ndof / t_res, "gbs_alg": 16.0 * ndof / t_res / 1e9},
+           "jvp": {"tangent_s": t_tan, "complex_step_s": t_cs, "speedup": t_cs / t_tan, "adjoint_s": t_vjp},
+           "finite": bool(torch.isfinite(u).all())}
+    del ev, u, u0, v, out
+    torch.cuda.empty_cache()
+    return res
+
+
 def kernel_only(torch, fr, args):
-    """Launch one kernel kind back to back (for rocprofv3 counter / trace runs)."""
+    """Launch one kernel kind back to back (for rocprofv3 counter / trace runs).
+    'step' = SSP-RK3 steps of the headline (plain launches, no graph), 'copy' =
+    a plain copy of the headline's 2.15 GB state, 'vcopy' = of the vortex's
+    13 MB state (counter scaling references)."""
     from fr_sdg_amd import configs as CF
     if args.kernel_only in ("batch", "batch_each"):
         b, u = batch_setup(fr)
@@ -263,20 +356,18 @@ def kernel_only(torch, fr, args):
         torch.cuda.synchronize()
         print(json.dumps({"kernel_only": "ilu", "applies": args.steps, "ndofs": ev.ndofs}))
         return
-    if args.kernel_only in ("large", "limiter"):
-        cfg = dict(dims=2, nx=1024, ny=1024, p=7, lx=2.0, ly=2.0, riemann=1, geometry=fr.GEOM_WARPED,
-                   geom_param=[0.05, -1.0, -1.0, 0.0], bc_type=[1, 1, 1, 1], gas_kind=1, vdw_a=0.1,
-                   vdw_b=0.02, apply_weno=int(args.kernel_only == "limiter"))
-        prm, kind = CF.WARPED_PRM, fr.IC_EXPLOSION
+    k = args.kernel_only
+    if k in ("step", "large", "limiter", "copy"):
+        ev, u, dt, cfg, _ = stepping_setup(torch, fr, "large")
+        if k == "limiter":
+            ev = fr.PhysicsEvaluator(dict(cfg, apply_filter=0, apply_weno=1))
     else:
-        cfg, prm, kind = vortex_cfg(), VORTEX_PRM, fr.IC_VORTEX
-    ev = fr.PhysicsEvaluator(cfg)
-    u = ev.impose_ic(kind, prm)
+        ev, u, dt, cfg, _ = stepping_setup(torch, fr, "vortex")
     out = ev.empty()
     v = torch.randn(ev.ndofs, dtype=torch.float64, device="cuda")
-    k = args.kernel_only
     fn = {
-        "stage": lambda: ev.ssprk_step(u, 1e-4),
+        "step": lambda: ev.ssprk_step(u, dt),
+        "stage": lambda: ev.ssprk_step(u, dt),
         "residual": lambda: ev.residual_kernel(u, out=out),
         "large": lambda: ev.residual_kernel(u, out=out),
         "limiter": lambda: ev.limit_solution(u, out=out),
@@ -284,40 +375,36 @@ def kernel_only(torch, fr, args):
         "cstep": lambda: ev.jvp_cstep(u, v, 1.0, 1e-7, out=out),
         "adjoint": lambda: ev.vjp(u, v, 1.0, out=out),
         "copy": lambda: out.copy_(u),
+        "vcopy": lambda: out.copy_(u),
     }[k]
     for _ in range(args.steps):
         fn()
     torch.cuda.synchronize()
     print(json.dumps({"kernel_only": k, "launches": args.steps, "ndofs": ev.ndofs,
-                      "bytes_per_copy": 16 * ev.ndofs if k == "copy" else None}))
+                      "bytes_per_copy": 16 * ev.ndofs if k in ("copy", "vcopy") else None}))
 
 
-def bench_large(torch, fr):
-    """Residual at 1024^2 warped quads, p=7, VdW, HLLC (2.15 GB per state)."""
-    from fr_sdg_amd import configs as CF
-    cfg = dict(dims=2, nx=1024, ny=1024, p=7, lx=2.0, ly=2.0, riemann=1, geometry=fr.GEOM_WARPED,
-               geom_param=[0.05, -1.0, -1.0, 0.0], bc_type=[1, 1, 1, 1], gas_kind=1, vdw_a=0.1,
-               vdw_b=0.02)
-    ev = fr.PhysicsEvaluator(cfg)
-    u = ev.impose_ic(fr.IC_EXPLOSION, CF.WARPED_PRM)
+def bench_large(torch, fr, ev, u):
+    """Residual only (no stage, no filter) and the WENO limiter on the
+    headline's initial state (1024^2 warped p=7 VdW HLLC, 2.15 GB per state)."""
     R = ev.empty()
     t = time_kernel(torch, lambda: ev.residual_kernel(u, out=R), 20)
     n = 8
     ndof = ev.ndofs
-    E = cfg["nx"] * cfg["ny"]
+    E = ev.cfg["nx"] * ev.cfg["ny"]
     bytes_alg = (16.0 + 8.0 + 12.0 / n) * ndof  # u, R, volume metrics, face (nx,ny,scale)
     ok = bool(torch.isfinite(R).all())
     res = {"workload": "warped 1024x1024 p=7 VdW HLLC residual", "ndofs": ndof, "avg_s": t,
            "dof_rhs_per_s": ndof / t, "gbs_alg": bytes_alg / t / 1e9,
            "hbm_frac": bytes_alg / t / 1e9 / HBM_PEAK_GBS,
-           "fp64_tflops": (32 * n ** 3 + 89 * n ** 2 + 424 * n) * E / t / 1e12, "finite": ok}
-    del ev, R
+           "fp64_tflops": flops_per_element(n) * E / t / 1e12, "finite": ok}
+    del R
     torch.cuda.empty_cache()
     # WENO limiter with shock sensor on the same state (limit_solution with
     # -apply_weno, no filter): smoothness of every element line, then limiting.
     # Algorithmic bytes: read u, write the output (16 B/DOF); arithmetic:
     # 2 (n-1)(2n+3) unfused FP64 operations per DOF for the smoothness sums.
-    evl = fr.PhysicsEvaluator(dict(cfg, apply_weno=1))
+    evl = fr.PhysicsEvaluator(dict(ev.cfg, apply_filter=0, apply_weno=1))
     out = evl.empty()
     tl = time_kernel(torch, lambda: evl.limit_solution(u, out=out), 20)
     changed = float((out != u).double().mean())
@@ -327,7 +414,7 @@ def bench_large(torch, fr):
                       "hbm_frac": 16.0 * ndof / tl / 1e9 / HBM_PEAK_GBS,
                       "fp64_tops_unfused": 2 * (n - 1) * (2 * n + 3) * ndof / tl / 1e12,
                       "fraction_of_dofs_limited": changed}
-    del evl, u, out
+    del evl, out
     torch.cuda.empty_cache()
     return res
 
@@ -679,27 +766,42 @@ def adjoint_large(torch, fr, args):
             "rel_diff": abs(ad - fd) / abs(fd), "fd_eps": eps, "fd_seconds": t_fd}
 
 
-def cpu_baseline(seconds):
-    """The CPU oracle (a restatement of the reference loops, 1 thread) on the same workload."""
+def cpu_baseline(args, dt):
+    """The CPU oracle (C++ restatement of the reference's loops) on the
+    headline workload: SSP-RK3 steps (3 residuals + the post-step filter) of
+    the same 1024^2 p=7 state, on all host threads (OpenMP over element rows,
+    as the reference runs one MPI rank per core) and on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
-    from pyoracle import Oracle
+    import pyoracle
+    from fr_sdg_amd import configs as CF
 
-    cfg = vortex_cfg()
-    o = Oracle(cfg)
-    u = o.impose_ic(1, VORTEX_PRM)
-    o.residual(u)
-    reps = 0
+    threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    o = pyoracle.Oracle(CF.warped(1024))
+    u = o.impose_ic(2, CF.WARPED_PRM)
+
+    def run(nthreads, min_s, max_steps):
+        pyoracle.set_threads(nthreads)
+        steps, t0, x = 0, time.perf_counter(), u
+        while True:
+            x = o.ssprk_step(x, dt)
+            steps += 1
+            el = time.perf_counter() - t0
+            if el >= min_s or steps >= max_steps:
+                return steps, el
+    steps_all, el_all = run(threads, 8.0, 20)
+    # one thread: one residual evaluation of the same state (a whole step is ~45 s there)
+    pyoracle.set_threads(1)
     t0 = time.perf_counter()
-    while True:
-        o.residual(u)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    return {"value": o.ndofs * reps / el, "unit": "DOF-RHS evals/s", "cores": 1, "kind": "port",
-            "sample": f"{reps} residual evaluations of the full 128x128 p=4 vortex state "
-                      f"({o.ndofs} DOFs) by the oracle, {el:.1f} s on 1 host thread"}
+    o.residual(u)
+    el_one = time.perf_counter() - t0
+    pyoracle.set_threads(threads)
+    nd = o.ndofs
+    return {"value": 3 * nd * steps_all / el_all, "unit": "DOF-RHS evals/s", "cores": threads, "kind": "port",
+            "sample": f"{steps_all} SSP-RK3 steps (3 residuals + filter each) of the full headline state "
+                      f"({nd} DOFs) by the oracle on {threads} threads, {el_all:.1f} s",
+            "value_1thread": nd / el_one,
+            "sample_1thread": f"1 residual evaluation of the same state on 1 thread, {el_one:.1f} s",
+            "host_cpus": os.cpu_count()}
 
 
 if __name__ == "__main__":

@@ -3,7 +3,8 @@
     python -m fr_sdg_amd.build        # or fr_sdg_amd.build.build()
 
 The library is a plain C-ABI shared object (include/frsdg.h); it links the
-HIP runtime only.  Packed-FP32 instructions are disabled for all device code.
+HIP runtime only.  libfrsdg_peaks.so (copy and FP64 FMA-loop probes) is
+built beside it for the benchmark.  Packed-FP32 instructions are disabled for all device code.
 """
 import os
 import subprocess
@@ -12,6 +13,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libfrsdg.so")
+# measured machine peaks for the benchmark (copy bandwidth, FP64 FMA rate); not part of the residual ABI
+PEAKS_LIB = os.path.join(HERE, "libfrsdg_peaks.so")
+PEAKS_SRC = os.path.join(HERE, "csrc", "frsdg_peaks.hip")
 SOURCES = [os.path.join(HERE, "csrc", f) for f in ("frsdg_capi.hip", "frsdg_newton.hip", "frsdg_batch.hip")]
 HEADERS = [
     os.path.join(HERE, "csrc", f)
@@ -23,7 +27,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
-    "-std=c++20",
+    "-std=c++23",
     "-fPIC",
     "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -55,6 +59,12 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    tmp = PEAKS_LIB + ".tmp"
+    cmd = [HIPCC, *FLAGS, PEAKS_SRC, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, PEAKS_LIB)
     return LIB
 
 

@@ -462,19 +462,25 @@ namespace
         }
     };
 
+    // one SSP-RK stage (k_res2d_stage in 2D, the stage output policy of the
+    // residual kernel in 1D); filt: apply the post-step spectral filter (2D)
     template <int N>
-    struct StageFilt
+    struct Stage
     {
-        static int run(const frsdg_ctx* c, const IOStage& io, hipStream_t s)
+        static int run(const frsdg_ctx* c, const IOStage& io, int filt, hipStream_t s)
         {
+            if (c->d.dims != 2) return Launch<IOStage>::template R1<N>::run(c, io, s);
             using T            = Tile2D<N, double>;
             const long long nb = tile_count(c->p2.nx, c->p2.ny, T::TX, T::TY);
             if (c->p2.curved)
-                hipLaunchKernelGGL((k_res2d_stage_filter<N, true>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, s,
-                                   c->p2, io);
+                hipLaunchKernelGGL((k_res2d_stage<N, true, true>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, s,
+                                   c->p2, io, filt);
+            else if (filt)
+                hipLaunchKernelGGL((k_res2d_stage<N, false, true>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, s,
+                                   c->p2, io, filt);
             else
-                hipLaunchKernelGGL((k_res2d_stage_filter<N, false>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, s,
-                                   c->p2, io);
+                hipLaunchKernelGGL((k_res2d_stage<N, false, false>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, s,
+                                   c->p2, io, filt);
             return hip_rc(hipGetLastError());
         }
     };
@@ -543,7 +549,7 @@ namespace
         auto stage = [&](const double* us, double a0, double b, double cdt, double* out)
         {
             IOStage io{us, u, out, a0, b, cdt};
-            return launch_res(c, io, s);
+            return by_n<Stage>(c->n, c, io, 0, s);
         };
         int rc;
         if (stages == 3)
@@ -554,7 +560,7 @@ namespace
             {
                 // last stage fused with the post-step filter
                 IOStage io{c->w2, u, u, 1.0 / 3.0, 2.0 / 3.0, dt};
-                return by_n<StageFilt>(c->n, c, io, s);
+                return by_n<Stage>(c->n, c, io, 1, s);
             }
             if (weno_only(c))
             {
@@ -573,7 +579,7 @@ namespace
             // last stage reads w1 (neighbour traces) and writes u (w2 when the
             // WENO limiter follows)
             IOStage io{c->w1, nullptr, weno_only(c) ? c->w2 : u, 0.0, 1.0, h};
-            if ((rc = launch_res(c, io, s))) return rc;
+            if ((rc = by_n<Stage>(c->n, c, io, 0, s))) return rc;
             if (weno_only(c)) return by_n<Weno>(c->n, c, c->w2, u, s);
         }
         else

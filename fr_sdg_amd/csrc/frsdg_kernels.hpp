@@ -818,13 +818,34 @@ namespace frsdg
         __shared__ TileSmem<N, typename IO::S, CURVED> sm;
         res2d_tile<N, IO, CURVED, false>(P, io, xcd_tile(blockIdx.x, gridDim.x), T::TX, T::TY, false, sm);
     }
-    // last SSP-RK stage fused with the post-step spectral filter
+    // Waves per SIMD that the LDS of one curved stage tile allows (160 KB per
+    // CU, four waves per workgroup), at most 4, for n <= 8: the register
+    // budget of those stage kernels is held to that occupancy, so the
+    // filter's line tasks (compiled into every stage launch) cannot cost a
+    // wave (n = 8: 130 -> 126 VGPRs, 3 -> 4 waves).  Affine tiles and n = 9
+    // stay unconstrained (1): a 128-register cap spills there.
     template <int N, bool CURVED>
-    __global__ __launch_bounds__(256) void k_res2d_stage_filter(const Res2DP P, const IOStage io)
+    constexpr int stage_waves()
+    {
+        constexpr int wg = static_cast<int>((160 * 1024) / sizeof(TileSmem<N, double, CURVED>));
+        if (!CURVED || N > 8) return 1;
+        return wg < 1 ? 1 : (wg > 4 ? 4 : wg);
+    }
+    // SSP-RK stage fused with the residual; with FILT and `filt` (the last
+    // stage of a step when the spectral filter is on) the post-step filter is
+    // applied in LDS before the store.  Curved meshes run every stage of a
+    // step through the FILT instantiation (one kernel, a uniform branch);
+    // affine meshes compile the filter only into the stage that uses it (the
+    // combined affine kernel spills 36 bytes at n = 6, 8).
+    template <int N, bool CURVED, bool FILT>
+    __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_waves<N, CURVED>())))
+    void k_res2d_stage(const Res2DP P, const IOStage io, int filt)
     {
         using T = Tile2D<N, double>;
         __shared__ TileSmem<N, double, CURVED> sm;
-        res2d_tile<N, IOStage, CURVED, true>(P, io, xcd_tile(blockIdx.x, gridDim.x), T::TX, T::TY, true, sm);
+        // affine FILT kernels always filter (the unfiltered store is dead code there)
+        const bool f = CURVED ? filt != 0 : FILT;
+        res2d_tile<N, IOStage, CURVED, FILT>(P, io, xcd_tile(blockIdx.x, gridDim.x), T::TX, T::TY, f, sm);
     }
 
     // ------------------------------------------------------------------

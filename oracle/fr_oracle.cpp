@@ -7,6 +7,8 @@
 #include "fr_oracle.h"
 #include "fr_oracle.hpp"
 
+#include <omp.h>
+
 #include <cstring>
 #include <memory>
 #include <numbers>
@@ -541,13 +543,21 @@ namespace fro
         }
 
         // physics_impl_2D.hpp:644-902 residual_kernel (single rank: start=0, max=N)
+        //
+        // OpenMP: the volume loop writes one element per iteration, an x-edge
+        // row j only the elements of row j, a y-edge column i only those of
+        // column i; every element still receives its contributions in the
+        // serial order (volume, x-edges by I, y-edges by J), so the result is
+        // bitwise the single-thread one.
         template <class U>
         void residual(const U* x, U* f, double t) const
         {
             const long Nx = d.nx, Ny = d.ny;
+#pragma omp parallel for schedule(static)
             for (long j = 0; j < Ny; ++j)
                 for (long i = 0; i < Nx; ++i) volume_term(elem(x, i, j), elem(f, i, j), i, j);
             const bool phys_left = !per_x, phys_right = !per_x;
+#pragma omp parallel for schedule(static)
             for (long j = 0; j < Ny; ++j)
             {
                 if (phys_left)
@@ -561,6 +571,7 @@ namespace fro
                     x_edge(x, f, Nx, j, true, false, false, false, t);
             }
             const bool phys_bot = !per_y, phys_top = !per_y;
+#pragma omp parallel for schedule(static)
             for (long i = 0; i < Nx; ++i)
             {
                 if (phys_bot)
@@ -593,10 +604,11 @@ namespace fro
                 if ((j < 0 || j >= d.ny) && !per_y) return zero.data();
                 return elem(sol, i, j);
             };
-            double a[9], b[9], c[9], o[9];
+#pragma omp parallel for schedule(static)
             for (long j = 0; j < d.ny; ++j)
                 for (long i = 0; i < d.nx; ++i)
                 {
+                    double a[9], b[9], c[9], o[9];
                     const double* uL = nb(i - 1, j);
                     const double* uC = elem(sol, i, j);
                     const double* uR = nb(i + 1, j);
@@ -904,6 +916,15 @@ void* fro_create(const fro_desc* d)
     }
 }
 void fro_destroy(void* h) { delete static_cast<Handle*>(h); }
+
+// Threads of the OpenMP loops (residual, filter, SSP-RK stage combinations);
+// returns the previous setting.  Results do not depend on it.
+int fro_set_threads(int nthreads)
+{
+    const int prev = omp_get_max_threads();
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    return prev;
+}
 long fro_ndofs(void* h) { return static_cast<Handle*>(h)->ndofs(); }
 
 void fro_tables(void* hh, double* csi, double* w, double* D, double* dL, double* dR, double* filt)
@@ -1078,27 +1099,35 @@ int fro_ssprk_step(void* hh, double t, double dt, double* u, int stages)
     auto L = [&](const double* x, double tt, double* out)
     {
         h->residual(x, out, tt);
+#pragma omp parallel for schedule(static)
         for (long q = 0; q < N; ++q) out[q] = -out[q];
     };
     if (stages == 3)
     {
         L(u0.data(), t, R.data());
+#pragma omp parallel for schedule(static)
         for (long q = 0; q < N; ++q) a[q] = u0[q] + dt * R[q];
         L(a.data(), t + dt, R.data());
+#pragma omp parallel for schedule(static)
         for (long q = 0; q < N; ++q) b[q] = 0.75 * u0[q] + 0.25 * (a[q] + dt * R[q]);
         L(b.data(), t + 0.5 * dt, R.data());
+#pragma omp parallel for schedule(static)
         for (long q = 0; q < N; ++q) u[q] = (1.0 / 3.0) * u0[q] + (2.0 / 3.0) * (b[q] + dt * R[q]);
     }
     else if (stages == 4)
     {
         const double hdt = 0.5 * dt;
         L(u0.data(), t, R.data());
+#pragma omp parallel for schedule(static)
         for (long q = 0; q < N; ++q) a[q] = u0[q] + hdt * R[q];
         L(a.data(), t + hdt, R.data());
+#pragma omp parallel for schedule(static)
         for (long q = 0; q < N; ++q) b[q] = a[q] + hdt * R[q];
         L(b.data(), t + dt, R.data());
+#pragma omp parallel for schedule(static)
         for (long q = 0; q < N; ++q) a[q] = (2.0 / 3.0) * u0[q] + (1.0 / 3.0) * (b[q] + hdt * R[q]);
         L(a.data(), t + hdt, R.data());
+#pragma omp parallel for schedule(static)
         for (long q = 0; q < N; ++q) u[q] = a[q] + hdt * R[q];
     }
     else
@@ -1117,6 +1146,7 @@ static int impose_ic_cons(void* hh, int kind, const double* prm, double* u)
     if (h->s2)
     {
         const auto& s = *h->s2;
+#pragma omp parallel for schedule(static)
         for (long j = 0; j < s.d.ny; ++j)
             for (long i = 0; i < s.d.nx; ++i)
             {

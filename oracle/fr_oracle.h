@@ -40,6 +40,8 @@ typedef struct fro_desc
 } fro_desc;
 
 void* fro_create(const fro_desc* d);
+/* threads of the OpenMP element loops; returns the previous setting */
+int   fro_set_threads(int nthreads);
 void  fro_destroy(void* h);
 long  fro_ndofs(void* h);
 void  fro_tables(void* h, double* csi, double* w, double* D, double* dL, double* dR, double* filt);

@@ -49,6 +49,8 @@ def lib():
         vp = ctypes.c_void_p
         L.fro_create.restype = vp
         L.fro_create.argtypes = [ctypes.POINTER(FroDesc)]
+        L.fro_set_threads.argtypes = [ctypes.c_int]
+        L.fro_set_threads.restype = ctypes.c_int
         L.fro_destroy.argtypes = [vp]
         L.fro_ndofs.restype = ctypes.c_long
         L.fro_ndofs.argtypes = [vp]
@@ -69,6 +71,12 @@ def lib():
         L.fro_metrics.argtypes = [vp, ctypes.c_double, ctypes.c_double, dp]
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """Threads of the oracle's OpenMP element loops (results do not depend on
+    it); returns the previous setting."""
+    return lib().fro_set_threads(int(n))
 
 
 def _p(a):
