@@ -27,6 +27,7 @@
 // The same kernel body is instantiated on double (residual), Dual (forward
 // tangent J*v) and Cplx (complex step), see frsdg_scalar.hpp.
 #pragma once
+#include <type_traits>
 #include "frsdg_geometry.hpp"
 #include "frsdg_physics.hpp"
 
@@ -268,12 +269,21 @@ namespace frsdg
     // bytes (a wavefront of line tasks then spreads over all LDS banks).  V is
     // only written by columns and read per node: no padding (this keeps the
     // curved n = 8 tile at 39 KB, four workgroups per CU).
+    //
+    // Within a row the four equations of node k are stored in the order
+    // q ^ (l & 3) (row l): a half-wave of per-node accesses (one q, rows
+    // l..l+3) then spreads over twice to four times as many banks (n = 8:
+    // 32 distinct 8-byte bank pairs instead of 8, measured 3.5 against 8.9
+    // cycles per wave read), while row-line tasks (fixed k) still read
+    // base + constant offsets and column-line tasks one of four bases.
     template <int N, class S, bool CURVED>
     struct TileSmem
     {
         static constexpr int NN = N * N, EPB = Tile2D<N, S>::EPB;
         static constexpr int RB = N * 4 * static_cast<int>(sizeof(S)); // row bytes
         static constexpr int RP = N * 4 + ((RB % 64) == 0 ? 32 / static_cast<int>(sizeof(S)) : 0);
+        // position of node k, equation q in row l
+        __device__ static __forceinline__ int pos(int l, int k, int q) { return 4 * k + (q ^ (l & 3)); }
         S         U[EPB][N][RP]; // state; curved meshes: xi-line part of the volume term after the face phase
         S         F[EPB][N][RP]; // affine meshes: xi-line part of the volume term after the line phase
         S         G[EPB][N][RP]; // affine meshes: eta-line part of the volume term after the line phase
@@ -484,12 +494,12 @@ namespace frsdg
         if (f.elA >= 0)
         {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) UA[q] = sm.U[f.elA][lA][kA * 4 + q];
+            for (int q = 0; q < 4; ++q) UA[q] = sm.U[f.elA][lA][sm.pos(lA, kA, q)];
         }
         if (f.elB >= 0)
         {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) UB[q] = sm.U[f.elB][lB][kB * 4 + q];
+            for (int q = 0; q < 4; ++q) UB[q] = sm.U[f.elB][lB][sm.pos(lB, kB, q)];
         }
         if (f.elA < 0)
         {
@@ -529,7 +539,7 @@ namespace frsdg
             {
                 S fn;
                 if constexpr (CURVED)
-                    fn = sm.F[f.elA][lA][kA * 4 + q] * f.fnx + sm.G[f.elA][lA][kA * 4 + q] * f.fny;
+                    fn = sm.F[f.elA][lA][sm.pos(lA, kA, q)] * f.fnx + sm.G[f.elA][lA][sm.pos(lA, kA, q)] * f.fny;
                 else
                     fn = FA[q] * f.fnx + GA[q] * f.fny;
                 sm.Del[f.elA][fA][f.idx][q] = f.sc * (Fs[q] - fn);
@@ -542,7 +552,7 @@ namespace frsdg
             {
                 S fn;
                 if constexpr (CURVED)
-                    fn = sm.F[f.elB][lB][kB * 4 + q] * f.fnx + sm.G[f.elB][lB][kB * 4 + q] * f.fny;
+                    fn = sm.F[f.elB][lB][sm.pos(lB, kB, q)] * f.fnx + sm.G[f.elB][lB][sm.pos(lB, kB, q)] * f.fny;
                 else
                     fn = FB[q] * f.fnx + GB[q] * f.fny;
                 sm.Del[f.elB][fB][f.idx][q] = f.sc * (Fs[q] - fn);
@@ -568,8 +578,8 @@ namespace frsdg
 #pragma unroll
             for (int m = 0; m < N; ++m)
             {
-                f[m] = sm.F[le][ln][m * 4 + q];
-                if constexpr (CURVED) g[m] = sm.G[le][ln][m * 4 + q];
+                f[m] = sm.F[le][ln][sm.pos(ln, m, q)];
+                if constexpr (CURVED) g[m] = sm.G[le][ln][sm.pos(ln, m, q)];
             }
 #pragma unroll
             for (int k = 0; k < N; ++k)
@@ -584,9 +594,9 @@ namespace frsdg
                 // affine: the result replaces row ln of F (equation q), which
                 // only this task reads in this phase
                 if constexpr (CURVED)
-                    sm.U[le][ln][k * 4 + q] = ma[k] * dF + mb[k] * dG;
+                    sm.U[le][ln][sm.pos(ln, k, q)] = ma[k] * dF + mb[k] * dG;
                 else
-                    sm.F[le][ln][k * 4 + q] = P.xi_x * dF;
+                    sm.F[le][ln][sm.pos(ln, k, q)] = P.xi_x * dF;
             }
         }
         else
@@ -595,8 +605,8 @@ namespace frsdg
 #pragma unroll
             for (int m = 0; m < N; ++m)
             {
-                g[m] = sm.G[le][m][ln * 4 + q];
-                if constexpr (CURVED) f[m] = sm.F[le][m][ln * 4 + q];
+                g[m] = sm.G[le][m][sm.pos(m, ln, q)];
+                if constexpr (CURVED) f[m] = sm.F[le][m][sm.pos(m, ln, q)];
             }
 #pragma unroll
             for (int l = 0; l < N; ++l)
@@ -611,9 +621,9 @@ namespace frsdg
                 // affine: the result replaces column ln of G (equation q), which
                 // only this task reads in this phase
                 if constexpr (CURVED)
-                    sm.V[le][l][ln * 4 + q] = ma[l] * dF + mb[l] * dG;
+                    sm.V[le][l][sm.pos(l, ln, q)] = ma[l] * dF + mb[l] * dG;
                 else
-                    sm.G[le][l][ln * 4 + q] = P.eta_y * dG;
+                    sm.G[le][l][sm.pos(l, ln, q)] = P.eta_y * dG;
             }
         }
     }
@@ -628,7 +638,7 @@ namespace frsdg
         const double* Fm = P.tab + N * N + 2 * N;
         double        x[N];
 #pragma unroll
-        for (int c = 0; c < N; ++c) x[c] = eta ? sm.U[le][c][ln * 4 + q] : sm.U[le][ln][c * 4 + q];
+        for (int c = 0; c < N; ++c) x[c] = eta ? sm.U[le][c][sm.pos(c, ln, q)] : sm.U[le][ln][sm.pos(ln, c, q)];
 #pragma unroll
         for (int r = 0; r < N; ++r)
         {
@@ -636,9 +646,9 @@ namespace frsdg
 #pragma unroll
             for (int c = 0; c < N; ++c) a += Fm[r * N + c] * x[c];
             if (eta)
-                sm.G[le][r][ln * 4 + q] = a;
+                sm.G[le][r][sm.pos(r, ln, q)] = a;
             else
-                sm.F[le][ln][r * 4 + q] = a;
+                sm.F[le][ln][sm.pos(ln, r, q)] = a;
         }
     }
 
@@ -701,9 +711,9 @@ namespace frsdg
 #pragma unroll
             for (int q = 0; q < 4; ++q)
             {
-                sm.U[el][l][k * 4 + q] = own[q];
-                sm.F[el][l][k * 4 + q] = F[q];
-                sm.G[el][l][k * 4 + q] = G[q];
+                sm.U[el][l][sm.pos(l, k, q)] = own[q];
+                sm.F[el][l][sm.pos(l, k, q)] = F[q];
+                sm.G[el][l][sm.pos(l, k, q)] = G[q];
             }
         }
         __syncthreads();
@@ -755,9 +765,9 @@ namespace frsdg
             for (int q = 0; q < 4; ++q)
             {
                 if constexpr (CURVED)
-                    r[q] = sm.U[el][l][k * 4 + q] + sm.V[el][l][k * 4 + q];
+                    r[q] = sm.U[el][l][sm.pos(l, k, q)] + sm.V[el][l][sm.pos(l, k, q)];
                 else
-                    r[q] = sm.F[el][l][k * 4 + q] + sm.G[el][l][k * 4 + q];
+                    r[q] = sm.F[el][l][sm.pos(l, k, q)] + sm.G[el][l][sm.pos(l, k, q)];
                 r[q] = r[q] + sm.Del[el][0][l][q] * sm.dL[k];
                 r[q] = r[q] + sm.Del[el][1][l][q] * sm.dR[k];
                 r[q] = r[q] + sm.Del[el][2][k][q] * sm.dL[l];
@@ -783,7 +793,7 @@ namespace frsdg
                     double o[4];
                     io.val4((e * NN + node) * 4, own, r, o);
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) sm.U[el][l][k * 4 + q] = o[q];
+                    for (int q = 0; q < 4; ++q) sm.U[el][l][sm.pos(l, k, q)] = o[q];
                 }
                 __syncthreads();
                 for (int t = tid; t < 2 * nlt; t += BS)
@@ -799,7 +809,7 @@ namespace frsdg
                 {
                     double o[4];
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) o[q] = 0.5 * (sm.F[el][l][k * 4 + q] + sm.G[el][l][k * 4 + q]);
+                    for (int q = 0; q < 4; ++q) o[q] = 0.5 * (sm.F[el][l][sm.pos(l, k, q)] + sm.G[el][l][sm.pos(l, k, q)]);
                     io.put4((e * NN + node) * 4, o);
                 }
                 return;
@@ -808,21 +818,11 @@ namespace frsdg
         if (act) io.st4((e * NN + node) * 4, own, r);
     }
 
-    // ------------------------------------------------------------------
-    // 2D residual: physics_impl_2D.hpp:267-357 (volume) + :689-899 (faces)
-    // ------------------------------------------------------------------
-    template <int N, class IO, bool CURVED>
-    __global__ __launch_bounds__(BlockSize<typename IO::S>::value) void k_res2d(const Res2DP P, const IO io)
-    {
-        using T = Tile2D<N, typename IO::S>;
-        __shared__ TileSmem<N, typename IO::S, CURVED> sm;
-        res2d_tile<N, IO, CURVED, false>(P, io, xcd_tile(blockIdx.x, gridDim.x), T::TX, T::TY, false, sm);
-    }
-    // Waves per SIMD that the LDS of one curved stage tile allows (160 KB per
-    // CU, four waves per workgroup), at most 4, for n <= 8: the register
-    // budget of those stage kernels is held to that occupancy, so the
-    // filter's line tasks (compiled into every stage launch) cannot cost a
-    // wave (n = 8: 130 -> 126 VGPRs, 3 -> 4 waves).  Affine tiles and n = 9
+    // Waves per SIMD that the LDS of one curved tile allows (160 KB per CU,
+    // four waves per workgroup), at most 4, for n <= 8: the register budget
+    // of the double residual and stage kernels is held to that occupancy, so
+    // the filter's line tasks (compiled into every stage launch) cannot cost
+    // a wave (n = 8: 130 -> 128 VGPRs, 3 -> 4 waves).  Affine tiles and n = 9
     // stay unconstrained (1): a 128-register cap spills there.
     template <int N, bool CURVED>
     constexpr int stage_waves()
@@ -830,6 +830,18 @@ namespace frsdg
         constexpr int wg = static_cast<int>((160 * 1024) / sizeof(TileSmem<N, double, CURVED>));
         if (!CURVED || N > 8) return 1;
         return wg < 1 ? 1 : (wg > 4 ? 4 : wg);
+    }
+    // ------------------------------------------------------------------
+    // 2D residual: physics_impl_2D.hpp:267-357 (volume) + :689-899 (faces)
+    // ------------------------------------------------------------------
+    template <int N, class IO, bool CURVED>
+    __global__ __launch_bounds__(BlockSize<typename IO::S>::value)
+    __attribute__((amdgpu_waves_per_eu(std::is_same_v<typename IO::S, double> ? stage_waves<N, CURVED>() : 1)))
+    void k_res2d(const Res2DP P, const IO io)
+    {
+        using T = Tile2D<N, typename IO::S>;
+        __shared__ TileSmem<N, typename IO::S, CURVED> sm;
+        res2d_tile<N, IO, CURVED, false>(P, io, xcd_tile(blockIdx.x, gridDim.x), T::TX, T::TY, false, sm);
     }
     // SSP-RK stage fused with the residual; with FILT and `filt` (the last
     // stage of a step when the spectral filter is on) the post-step filter is
