@@ -85,7 +85,7 @@ EXPORTS = [
     "frsdg_batch_ssprk_step", "frsdg_batch_ssprk_run", "frsdg_batch_ssprk_step_each",
     "frsdg_filter_transpose", "frsdg_mass", "frsdg_boundary_weights", "frsdg_adjoint_solve", "frsdg_pseudo_run",
     "frsdg_color_jacobian", "frsdg_ilu0_factor", "frsdg_ilu0_apply", "frsdg_linear_solve",
-    "frsdg_ilu0_apply_transpose",
+    "frsdg_ilu0_apply_transpose", "frsdg_l2_error_noalloc",
 ]
 
 _lib = None
@@ -120,6 +120,8 @@ def load_library(path=LIB_PATH):
     L.frsdg_impose_ic.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double), dp, vp]
     L.frsdg_l2_error.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double), d, dp,
                                  ctypes.POINTER(ctypes.c_double), vp]
+    L.frsdg_l2_error_noalloc.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double), d, dp,
+                                         ctypes.POINTER(ctypes.c_double), vp]
     L.frsdg_beuler_step.argtypes = [vp, d, d, dp, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats), vp]
     L.frsdg_pseudo_run.argtypes = [vp, d, d, d, i, d, d, i, dp, ctypes.POINTER(NewtonOpts),
                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
@@ -423,6 +425,15 @@ class PhysicsEvaluator:
         prm = (ctypes.c_double * 8)(*(list(prm) + [0.0] * (8 - len(prm))))
         out = (ctypes.c_double * 4)()
         _check(self.lib.frsdg_l2_error(self.h, kind, prm, t, _ptr(u), out, _stream()))
+        return list(out)
+
+
+    def compute_error_noalloc(self, kind, prm, u, t=0.0):
+        """compute_error through the allocation-free entry point (static device
+        scratch, at most 2^24 nodes; calls that overlap on one GPU race)."""
+        prm = (ctypes.c_double * 8)(*(list(prm) + [0.0] * (8 - len(prm))))
+        out = (ctypes.c_double * 4)()
+        _check(self.lib.frsdg_l2_error_noalloc(self.h, kind, prm, t, _ptr(u), out, _stream()))
         return list(out)
 
 

@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <cstdint>
 #include <cstring>
 #include <new>
@@ -300,11 +302,13 @@ namespace
         sp[2 * n + q] = gas_rhoE(gas, rho, vel, p);
     }
     // compute_error partial sums (physics_impl_2D.hpp:203-265): one 4-vector per
-    // block, reduced in fixed order by k_sum_blocks (deterministic).
+    // block, reduced in fixed order by k_sum_blocks (deterministic).  With
+    // cond_sys the stored node state is (log rho, u, v, log e) and is converted
+    // to conservative variables first (physics_impl_2D.hpp:229-240).
     __global__ __launch_bounds__(256) void k_l2_part(GeomP g, GasP gas, ICP c, int nx, int ny, int n,
                                                      const double* __restrict__ csi,
                                                      const double* __restrict__ w, long long E,
-                                                     const double* __restrict__ u, double* part)
+                                                     const double* __restrict__ u, int cond_sys, double* part)
     {
         __shared__ double s[256][4];
         const long long NN = static_cast<long long>(n) * n;
@@ -322,9 +326,16 @@ namespace
             const VolM m = composite_metrics(g, nx, ny, cx, cy);
             ic_state(gas, c, x, y, st);
             const double cv = w[k] * w[l] * m.detJ;
+            double       un[4];
+            for (int q = 0; q < 4; ++q) un[q] = u[t * 4 + q];
+            if (cond_sys)
+            {
+                const double wv[4] = {un[0], un[1], un[2], un[3]};
+                p2c_node<double>(2, wv, un);
+            }
             for (int q = 0; q < 4; ++q)
             {
-                const double dd = u[t * 4 + q] - st[q];
+                const double dd = un[q] - st[q];
                 acc[q]          = cv * dd * dd;
             }
         }
@@ -599,6 +610,54 @@ namespace
     }
 } // namespace
 
+namespace
+{
+    // File-scope scratch of frsdg_l2_error_noalloc: 4 partial sums per block
+    // of 256 nodes plus the final 4-vector.  Each GPU holds its own copy
+    // until the process ends; host code reaches it only through the device
+    // address that hipGetSymbolAddress returns for the current device.
+    static __device__ __align__(256) double scratch[4 * (FRSDG_L2_SCRATCH_BLOCKS + 1)];
+
+    constexpr int MAX_DEVICES = 64;
+    std::mutex    g_scratch_mu;
+    double*       g_scratch_ptr[MAX_DEVICES] = {};
+
+    // device address of `scratch` on the current device, looked up once per device
+    int scratch_device_ptr(double** out)
+    {
+        int dev = 0;
+        FR_TRY(hipGetDevice(&dev));
+        if (dev < 0 || dev >= MAX_DEVICES) return FRSDG_ERR_UNSUPPORTED;
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch_ptr[dev])
+        {
+            void* p = nullptr;
+            if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return FRSDG_ERR_HIP;
+            g_scratch_ptr[dev] = static_cast<double*>(p);
+        }
+        *out = g_scratch_ptr[dev];
+        return FRSDG_OK;
+    }
+
+    // compute_error into a caller-provided reduction buffer red (4 * (nb + 1)
+    // doubles: the result, then the per-block partials); synchronises s
+    int l2_error_into(const frsdg_ctx* c, const ICP& ic, const double* u, double* red, double* out, hipStream_t s)
+    {
+        const int       n   = c->n;
+        const long long tot = c->E * n * n;
+        const long long nb  = (tot + 255) / 256;
+        hipLaunchKernelGGL(k_l2_part, dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, geom_of(c->d), c->p2.gas, ic,
+                           c->d.nx, c->d.ny, n, c->d_csi, c->d_csi + n, c->E, u, c->d.cond_sys, red + 4);
+        hipLaunchKernelGGL(k_sum_blocks, dim3(1), dim3(256), 0, s, nb, red + 4, red);
+        FR_TRY(hipGetLastError());
+        double h[4];
+        FR_TRY(hipMemcpyAsync(h, red, sizeof(h), hipMemcpyDeviceToHost, s));
+        FR_TRY(hipStreamSynchronize(s));
+        for (int q = 0; q < 4; ++q) out[q] = std::sqrt(h[q]);
+        return FRSDG_OK;
+    }
+} // namespace
+
 extern "C" {
 
 const char* frsdg_strerror(int code)
@@ -716,6 +775,12 @@ int frsdg_create(const frsdg_desc* d, frsdg_ctx** out)
     }
     c->E  = static_cast<long long>(d->nx) * d->ny;
     if (c->E >= (1LL << 31) - 1024) return fail(FRSDG_ERR_UNSUPPORTED); // 32-bit element indices in kernels
+    {
+        // set-up of frsdg_l2_error_noalloc: the device address of its static
+        // scratch on this device (once per device)
+        double* scr = nullptr;
+        if (scratch_device_ptr(&scr)) return fail(FRSDG_ERR_HIP);
+    }
     c->nd = 4LL * n * n * c->E;
     Res2DP& P = c->p2;
     P.nx      = d->nx;
@@ -1073,22 +1138,23 @@ int frsdg_l2_error(frsdg_ctx* c, int kind, const double* prm, double t, const do
     const long long tot = c->E * n * n;
     const long long nb  = (tot + 255) / 256;
     if (!c->red) FR_TRY(hipMalloc(&c->red, sizeof(double) * 4 * (nb + 1)));
-    if (c->d.cond_sys) // compute_error converts the stored primitives (physics_impl_2D.hpp:229-240)
-    {
-        int rc = ensure_work(c);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_p2c, dim3(node_grid(c)), dim3(256), 0, s, c->nm, u, c->w1);
-        u = c->w1;
-    }
-    hipLaunchKernelGGL(k_l2_part, dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, geom_of(c->d), c->p2.gas, ic,
-                       c->d.nx, c->d.ny, n, c->d_csi, c->d_csi + n, c->E, u, c->red + 4);
-    hipLaunchKernelGGL(k_sum_blocks, dim3(1), dim3(256), 0, s, nb, c->red + 4, c->red);
-    FR_TRY(hipGetLastError());
-    double h[4];
-    FR_TRY(hipMemcpyAsync(h, c->red, sizeof(h), hipMemcpyDeviceToHost, s));
-    FR_TRY(hipStreamSynchronize(s));
-    for (int q = 0; q < 4; ++q) out[q] = std::sqrt(h[q]);
-    return FRSDG_OK;
+    return l2_error_into(c, ic, u, c->red, out, s);
+}
+
+int frsdg_l2_error_noalloc(frsdg_ctx* c, int kind, const double* prm, double t, const double* u, double* out,
+                           void* stream)
+{
+    if (!c || !prm || !u || !out || c->d.dims != 2 || kind < 0 || kind > 4 || kind == 3) return FRSDG_ERR_INVALID;
+    const long long nb = (c->E * c->n * c->n + 255) / 256;
+    if (nb > FRSDG_L2_SCRATCH_BLOCKS) return FRSDG_ERR_INVALID; // beyond the static scratch (E n^2 > 2^24 nodes)
+    double* scr = nullptr;
+    int     rc  = scratch_device_ptr(&scr);
+    if (rc) return rc;
+    ICP ic{};
+    ic.kind = kind;
+    for (int q = 0; q < 8; ++q) ic.prm[q] = prm[q];
+    ic.t = t;
+    return l2_error_into(c, ic, u, scr, out, static_cast<hipStream_t>(stream));
 }
 
 } // extern "C"

@@ -167,6 +167,19 @@ int frsdg_impose_ic(frsdg_ctx* ctx, int kind, const double* prm, double* u, void
 int frsdg_l2_error(frsdg_ctx* ctx, int kind, const double* prm, double t, const double* u,
                    double* out, void* stream);
 
+/* Allocation-free frsdg_l2_error (same kernels, same result, bitwise): no
+ * counterpart in the reference; for callers that cannot allocate device memory
+ * (serving loops, harnesses that forbid hipMalloc, code that must not grow the
+ * device heap).  Its reduction buffer is a file-scope device array of
+ * 4 * (FRSDG_L2_SCRATCH_BLOCKS + 1) doubles (2 MiB) that every GPU the library
+ * runs on holds until the process ends; a context with more than
+ * 256 * FRSDG_L2_SCRATCH_BLOCKS = 2^24 nodes (E n^2) is rejected with
+ * FRSDG_ERR_INVALID.  All calls on one GPU share that array: calls that overlap
+ * on one GPU (two streams, two threads) race.  Synchronises the stream. */
+#define FRSDG_L2_SCRATCH_BLOCKS 65536
+int frsdg_l2_error_noalloc(frsdg_ctx* ctx, int kind, const double* prm, double t, const double* u, double* out,
+                           void* stream);
+
 /* ---------------- implicit steps: Jacobian-free Newton-Krylov ----------------
  * The TSSetIFunction / TSSetIJacobian path of the reference
  * (petsc_solver_impl.hpp:572-579, 626-655, 168-267) with PETSc's SNES newtonls
