@@ -59,21 +59,34 @@ class FrsdgDesc(ctypes.Structure):
 class NewtonOpts(ctypes.Structure):
     _fields_ = [("jac_type", ctypes.c_int), ("precond", ctypes.c_int), ("snes_rtol", ctypes.c_double),
                 ("snes_atol", ctypes.c_double), ("snes_max_it", ctypes.c_int), ("ksp_restart", ctypes.c_int),
-                ("ksp_max_it", ctypes.c_int), ("ksp_rtol", ctypes.c_double), ("ksp_cgs_refine", ctypes.c_int)]
+                ("ksp_max_it", ctypes.c_int), ("ksp_rtol", ctypes.c_double), ("ksp_cgs_refine", ctypes.c_int),
+                ("snes_stol", ctypes.c_double), ("ksp_pc_side", ctypes.c_int), ("pc_lag", ctypes.c_int)]
 
 
 class NewtonStats(ctypes.Structure):
     _fields_ = [("newton_its", ctypes.c_int), ("ksp_its", ctypes.c_int), ("jv_evals", ctypes.c_int),
                 ("res_evals", ctypes.c_int), ("converged", ctypes.c_int), ("fnorm0", ctypes.c_double),
-                ("fnorm", ctypes.c_double), ("seconds", ctypes.c_double)]
+                ("fnorm", ctypes.c_double), ("seconds", ctypes.c_double), ("pc_builds", ctypes.c_int),
+                ("t_pc_assemble", ctypes.c_double), ("t_pc_factor", ctypes.c_double),
+                ("t_pc_apply", ctypes.c_double), ("t_jv", ctypes.c_double), ("t_orth", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-# PETSc defaults: SNES rtol 1e-8, atol 1e-50, max_it 50; KSP GMRES restart 30, rtol 1e-5, max_it 10000
+class AdjointStats(ctypes.Structure):
+    _fields_ = [("forward_steps", ctypes.c_int), ("adjoint_steps", ctypes.c_int), ("snapshots_used", ctypes.c_int),
+                ("seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# PETSc defaults: SNES rtol 1e-8, atol 1e-50, stol 1e-8, max_it 50; KSP GMRES restart 30, rtol 1e-5,
+# max_it 10000; preconditioner rebuilt every Newton iteration.  ksp_pc_side stays right (0) here, as the
+# residual-history checks assume; PETSc's GMRES default is left (1).
 NEWTON_DEFAULTS = dict(jac_type=0, precond=0, snes_rtol=1e-8, snes_atol=1e-50, snes_max_it=50, ksp_restart=30,
-                       ksp_max_it=10000, ksp_rtol=1e-5, ksp_cgs_refine=0)
+                       ksp_max_it=10000, ksp_rtol=1e-5, ksp_cgs_refine=0, snes_stol=1e-8, ksp_pc_side=0, pc_lag=1)
 
 EXPORTS = [
     "frsdg_create", "frsdg_destroy", "frsdg_ndofs", "frsdg_get_tables", "frsdg_residual",
@@ -85,7 +98,7 @@ EXPORTS = [
     "frsdg_batch_ssprk_step", "frsdg_batch_ssprk_run", "frsdg_batch_ssprk_step_each",
     "frsdg_filter_transpose", "frsdg_mass", "frsdg_boundary_weights", "frsdg_adjoint_solve", "frsdg_pseudo_run",
     "frsdg_color_jacobian", "frsdg_ilu0_factor", "frsdg_ilu0_apply", "frsdg_linear_solve",
-    "frsdg_ilu0_apply_transpose", "frsdg_l2_error_noalloc",
+    "frsdg_ilu0_apply_transpose", "frsdg_l2_error_noalloc", "frsdg_unsteady_adjoint",
 ]
 
 _lib = None
@@ -120,12 +133,14 @@ def load_library(path=LIB_PATH):
     L.frsdg_impose_ic.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double), dp, vp]
     L.frsdg_l2_error.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double), d, dp,
                                  ctypes.POINTER(ctypes.c_double), vp]
+    L.frsdg_unsteady_adjoint.argtypes = [vp, d, d, i, i, dp, dp, dp, ctypes.POINTER(AdjointStats), vp]
     L.frsdg_l2_error_noalloc.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double), d, dp,
                                          ctypes.POINTER(ctypes.c_double), vp]
     L.frsdg_beuler_step.argtypes = [vp, d, d, dp, ctypes.POINTER(NewtonOpts), ctypes.POINTER(NewtonStats), vp]
     L.frsdg_pseudo_run.argtypes = [vp, d, d, d, i, d, d, i, dp, ctypes.POINTER(NewtonOpts),
                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
-                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), vp]
+                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(NewtonStats), vp]
     L.frsdg_batch_create.argtypes = [ctypes.POINTER(FrsdgDesc), i, ctypes.POINTER(ctypes.c_void_p)]
     L.frsdg_batch_destroy.argtypes = [vp]
     L.frsdg_batch_ndofs.argtypes = [vp]
@@ -332,10 +347,12 @@ class PhysicsEvaluator:
         fns = (ctypes.c_double * max(1, max_steps))()
         tms = (ctypes.c_double * max(1, max_steps))()
         n = ctypes.c_int(0)
+        st = NewtonStats()
         _check(self.lib.frsdg_pseudo_run(self.h, t, dt0, increment, int(from_initial_dt), dt_max, t_max, max_steps,
-                                         _ptr(u), ctypes.byref(no), dts, fns, tms, ctypes.byref(n), _stream()))
+                                         _ptr(u), ctypes.byref(no), dts, fns, tms, ctypes.byref(n), ctypes.byref(st),
+                                         _stream()))
         return {"dt": list(dts[:n.value]), "fnorm": list(fns[:n.value]), "seconds": list(tms[:n.value]),
-                "steps": n.value}
+                "steps": n.value, "fnorm0": st.fnorm0, "stats": st.as_dict()}
 
     def color_layout(self):
         """(E, S, nd): block rows, slots per row (9 in 2D, 3 in 1D), DOFs per element."""
@@ -397,6 +414,15 @@ class PhysicsEvaluator:
         st = NewtonStats()
         _check(self.lib.frsdg_adjoint_solve(self.h, t, sigma, _ptr(u), _ptr(rhs), _ptr(out), ctypes.byref(no),
                                             ctypes.byref(st), _stream()))
+        return out, st.as_dict()
+
+    def unsteady_adjoint(self, u0, lam_T, dt, nsteps, max_snapshots=32, t=0.0, out=None):
+        """lambda_0 = (d u_N / d u_0)^T lambda_T over nsteps SSP-RK3 steps (binomial
+        checkpointing in the library); returns lambda_0 and the statistics."""
+        out = self.empty() if out is None else out
+        st = AdjointStats()
+        _check(self.lib.frsdg_unsteady_adjoint(self.h, t, dt, int(nsteps), int(max_snapshots), _ptr(u0), _ptr(lam_T),
+                                               _ptr(out), ctypes.byref(st), _stream()))
         return out, st.as_dict()
 
     def filter_transpose(self, w, out=None):

@@ -1,5 +1,7 @@
 """Unsteady discrete adjoint of the explicit FR time integration with
-binomial checkpointing.
+binomial checkpointing: a thin binding over frsdg_unsteady_adjoint (the
+revolve schedule runs in the library's C++ host code, every state operation in
+HIP kernels).
 
 Forward map of one step (Shu-Osher SSPRK(3,3) with rhs = -R, then the
 post-step filter F, as frsdg_ssprk_step):
@@ -8,13 +10,6 @@ post-step filter F, as frsdg_ssprk_step):
 Its transpose, with g = (dR/du)^T from the adjoint kernel:
     l3 = F^T l,  l2 = 2/3 l3 - 2/3 dt g(u2) l3,  l1 = 1/4 l2 - 1/4 dt g(u1) l2,
     l  <- 1/3 l3 + 3/4 l2 + l1 - dt g(u) l1.
-The stage states of a step are recomputed from the state at its start.
-
-Checkpointing follows the binomial (revolve) schedule of Griewank: with s
-free snapshot slots and r forward repetitions, L(s, r) = C(s+r, s) steps can
-be reversed; a segment of T <= L(s, r) steps is split so that the later part
-(reversed first, one slot fewer) has at most L(s-1, r) steps and the earlier
-part at most L(s, r-1).  Snapshots live in device memory (one state each).
 """
 from math import comb
 
@@ -22,7 +17,7 @@ import torch
 
 
 def repetitions(T, s):
-    """Smallest r with C(s + r, s) >= T."""
+    """Smallest r with C(s + r, s) >= T (forward sweeps of the binomial schedule)."""
     r = 0
     while comb(s + r, s) < T:
         r += 1
@@ -30,67 +25,30 @@ def repetitions(T, s):
 
 
 class UnsteadyAdjoint:
+    """dJ/du_0 for J(u_T), u_T = S^T(u_0), with at most `max_snapshots` stored states."""
+
     def __init__(self, ev, dt, nsteps, max_snapshots=32):
         self.ev, self.dt, self.n = ev, dt, nsteps
         self.s = max(0, int(max_snapshots))
-        self.forward_steps = 0   # counts every forward step taken (incl. recomputation)
+        self.forward_steps = 0   # forward steps recomputed by the last gradient()
         self.adjoint_steps = 0
         self.peak_snapshots = 0
-        self._live = 0
-        self.on_adjoint_step = None  # optional callback(self) after each adjoint step
-
-    # ---- forward ----
-    def advance(self, u, k):
-        for _ in range(k):
-            self.ev.ssprk_step(u, self.dt)
-        self.forward_steps += k
-        return u
+        self.seconds = 0.0
 
     def run(self, u0):
+        """u_T (u0 is not modified)."""
         u = u0.clone()
-        return self.advance(u, self.n)
-
-    # ---- one adjoint step at state u (start of the step) ----
-    def adjoint_step(self, u, lam):
-        ev, dt = self.ev, self.dt
-        R = ev.residual_kernel(u)
-        u1 = u - dt * R
-        R1 = ev.residual_kernel(u1)
-        u2 = 0.75 * u + 0.25 * (u1 - dt * R1)
-        l3 = ev.filter_transpose(lam) if ev.cfg.get("apply_filter", 0) else lam
-        l2 = (2.0 / 3.0) * (l3 - dt * ev.vjp(u2, l3))
-        l1 = 0.25 * (l2 - dt * ev.vjp(u1, l2))
-        self.adjoint_steps += 1
-        out = (1.0 / 3.0) * l3 + 0.75 * l2 + l1 - dt * ev.vjp(u, l1)
-        if self.on_adjoint_step is not None:
-            self.on_adjoint_step(self)
-        return out
-
-    def _reverse(self, a, b, ua, lam, free):
-        T = b - a
-        if T == 1:
-            return self.adjoint_step(ua, lam)
-        if free == 0:
-            for k in range(b - 1, a - 1, -1):
-                x = ua.clone()
-                self.advance(x, k - a)
-                lam = self.adjoint_step(x, lam)
-            return lam
-        r = repetitions(T, free)
-        second = min(T - 1, comb(free - 1 + r, free - 1))
-        m = b - second
-        x = ua.clone()
-        self.advance(x, m - a)
-        self._live += 1
-        self.peak_snapshots = max(self.peak_snapshots, self._live)
-        lam = self._reverse(m, b, x, lam, free - 1)
-        del x
-        self._live -= 1
-        return self._reverse(a, m, ua, lam, free)
+        self.ev.ssprk_run(u, self.dt, self.n)
+        return u
 
     def gradient(self, u0, dJdu_T):
         """lambda_0 = (d u_T / d u_0)^T dJ/du_T; u0 is not modified."""
-        return self._reverse(0, self.n, u0, dJdu_T.clone(), self.s)
+        lam, st = self.ev.unsteady_adjoint(u0, dJdu_T, self.dt, self.n, self.s)
+        self.forward_steps = st["forward_steps"]
+        self.adjoint_steps = st["adjoint_steps"]
+        self.peak_snapshots = st["snapshots_used"]
+        self.seconds = st["seconds"]
+        return lam
 
 
 def rho_squared(ev, u, m=None):

@@ -21,6 +21,23 @@
 
 namespace frsdg
 {
+    // Output of the adjoint kernels: out = a w + b (dR/du)^T w [+ c z1] [+ d z2]
+    // (the plain VJP is a = sigma, b = 1; the unsteady adjoint fuses the
+    // SSP-RK transpose combinations into it)
+    struct AdjOut
+    {
+        double a, b, c, d;
+        const double* z1;
+        const double* z2;
+        __device__ __forceinline__ double at(long long i, double w, double jtw) const
+        {
+            double o = a * w + b * jtw;
+            if (z1) o += c * z1[i];
+            if (z2) o += d * z2[i];
+            return o;
+        }
+    };
+
     constexpr int ADJ_BS = 128;
     template <int N>
     struct AdjTile
@@ -106,7 +123,7 @@ namespace frsdg
     template <int N>
     __global__ __launch_bounds__(ADJ_BS) void k_adj2d(const Res2DP P, const double* __restrict__ u,
                                                        const double* __restrict__ w,
-                                                       const double* __restrict__ wb, double sigma,
+                                                       const double* __restrict__ wb, const AdjOut ao,
                                                        double* __restrict__ out)
     {
         constexpr int NN  = N * N;
@@ -303,7 +320,7 @@ namespace frsdg
             if (k == N - 1) s += sFc[el][1][l][q];
             if (l == 0) s += sFc[el][2][k][q];
             if (l == N - 1) s += sFc[el][3][k][q];
-            ub[q] = sigma * W[q] + s;
+            ub[q] = ao.at((e * NN + node) * 4 + q, W[q], s);
         }
         st4d(out + (e * NN + node) * 4, ub);
     }
@@ -356,7 +373,7 @@ namespace frsdg
     }
     template <int N>
     __global__ __launch_bounds__(64) void k_adj1d(const Res1DP P, const double* __restrict__ u,
-                                                   const double* __restrict__ w, double sigma,
+                                                   const double* __restrict__ w, const AdjOut ao,
                                                    double* __restrict__ out)
     {
         const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -462,6 +479,6 @@ namespace frsdg
         }
 #pragma unroll
         for (int q = 0; q < 3; ++q)
-            for (int m = 0; m < N; ++m) out[b + q * N + m] = sigma * w[b + q * N + m] + ub[q][m];
+            for (int m = 0; m < N; ++m) out[b + q * N + m] = ao.at(b + q * N + m, w[b + q * N + m], ub[q][m]);
     }
 } // namespace frsdg

@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <chrono>
 #include <mutex>
 
 #include <cstdint>
@@ -447,7 +449,7 @@ namespace
     template <int N>
     struct Adj2
     {
-        static int run(const frsdg_ctx* c, const double* u, const double* w, double sigma, double* out,
+        static int run(const frsdg_ctx* c, const double* u, const double* w, const AdjOut& ao, double* out,
                        hipStream_t s)
         {
             const long long nl = c->E * 4 * N;
@@ -457,18 +459,18 @@ namespace
             const long long nb = (c->E + EPB - 1) / EPB;
             Res2DP P = c->p2;
             hipLaunchKernelGGL((k_adj2d<N>), dim3(static_cast<unsigned>(nb)), dim3(ADJ_BS), 0, s, P, u, w,
-                               static_cast<const double*>(c->wb), sigma, out);
+                               static_cast<const double*>(c->wb), ao, out);
             return hip_rc(hipGetLastError());
         }
     };
     template <int N>
     struct Adj1
     {
-        static int run(const frsdg_ctx* c, const double* u, const double* w, double sigma, double* out,
+        static int run(const frsdg_ctx* c, const double* u, const double* w, const AdjOut& ao, double* out,
                        hipStream_t s)
         {
             const int nb = (c->p1.ne + 63) / 64;
-            hipLaunchKernelGGL((k_adj1d<N>), dim3(nb), dim3(64), 0, s, c->p1, u, w, sigma, out);
+            hipLaunchKernelGGL((k_adj1d<N>), dim3(nb), dim3(64), 0, s, c->p1, u, w, ao, out);
             return hip_rc(hipGetLastError());
         }
     };
@@ -842,7 +844,7 @@ int frsdg_destroy(frsdg_ctx* c)
     if (c->gexec) (void) hipGraphExecDestroy(c->gexec);
     if (c->graph) (void) hipGraphDestroy(c->graph);
     for (double* p : {c->d_tab, c->d_vmet, c->d_fxm, c->d_fym, c->d_csi, c->d_rinv, c->w1, c->w2, c->red, c->wb,
-                      c->wbeta})
+                      c->wbeta, c->adj_pool})
         if (p) (void) hipFree(p);
     delete c;
     return FRSDG_OK;
@@ -961,14 +963,16 @@ int frsdg_vjp(frsdg_ctx* c, double, const double* u, const double* w, double sig
         if (rc) return rc;
         hipLaunchKernelGGL(k_p2c, dim3(node_grid(c)), dim3(256), 0, s, c->nm, u, c->w1);
         FR_TRY(hipGetLastError());
-        rc = c->d.dims == 1 ? by_n<Adj1>(c->n, c, c->w1, w, sigma, c->w2, s)
-                            : by_n<Adj2>(c->n, c, c->w1, w, sigma, c->w2, s);
+        const AdjOut ao{sigma, 1.0, 0.0, 0.0, nullptr, nullptr};
+        rc = c->d.dims == 1 ? by_n<Adj1>(c->n, c, c->w1, w, ao, c->w2, s)
+                            : by_n<Adj2>(c->n, c, c->w1, w, ao, c->w2, s);
         if (rc) return rc;
         hipLaunchKernelGGL(k_p2c_vjp, dim3(node_grid(c)), dim3(256), 0, s, c->nm, u, c->w2, out);
         return hip_rc(hipGetLastError());
     }
-    if (c->d.dims == 1) return by_n<Adj1>(c->n, c, u, w, sigma, out, s);
-    return by_n<Adj2>(c->n, c, u, w, sigma, out, s);
+    const AdjOut ao{sigma, 1.0, 0.0, 0.0, nullptr, nullptr};
+    if (c->d.dims == 1) return by_n<Adj1>(c->n, c, u, w, ao, out, s);
+    return by_n<Adj2>(c->n, c, u, w, ao, out, s);
 }
 
 int frsdg_pre_jv_eval(frsdg_ctx* c, double t, const double* u, double sigma)
@@ -1155,6 +1159,167 @@ int frsdg_l2_error_noalloc(frsdg_ctx* c, int kind, const double* prm, double t, 
     for (int q = 0; q < 8; ++q) ic.prm[q] = prm[q];
     ic.t = t;
     return l2_error_into(c, ic, u, scr, out, static_cast<hipStream_t>(stream));
+}
+
+} // extern "C"
+
+// ---------------- unsteady discrete adjoint (binomial checkpointing) ----------------
+namespace
+{
+    // Smallest r with C(s + r, s) >= T (repetitions of the binomial schedule).
+    int revolve_repetitions(long long T, int s)
+    {
+        int r = 0;
+        for (;; ++r)
+        {
+            // C(s + r, s) with early exit once it reaches T
+            double c = 1.0;
+            for (int i = 1; i <= r; ++i) c = c * (s + i) / i;
+            if (c >= static_cast<double>(T) - 0.5) return r;
+        }
+    }
+    long long binom(int n, int k)
+    {
+        if (k < 0 || k > n) return 0;
+        double c = 1.0;
+        for (int i = 1; i <= k; ++i) c = c * (n - k + i) / i;
+        return static_cast<long long>(c + 0.5);
+    }
+
+    // One reverse sweep of frsdg_unsteady_adjoint: the pool holds `slots`
+    // snapshot states, the recompute work state, the stage states w1/w2 of the
+    // step being reversed and the adjoint stage vectors l3, l2, l1.
+    struct UnsteadyAdj
+    {
+        frsdg_ctx*          c;
+        hipStream_t         st;
+        double              dt;
+        double*             pool;
+        int                 slots;
+        long long           nd;
+        frsdg_adjoint_stats* S;
+        int                 live = 0;
+
+        double* slot(int i) const { return pool + static_cast<long long>(i) * nd; }
+        double* work() const { return slot(slots); }
+        double* w1() const { return slot(slots + 1); }
+        double* w2() const { return slot(slots + 2); }
+        double* l3() const { return slot(slots + 3); }
+        double* l2() const { return slot(slots + 4); }
+        double* l1() const { return slot(slots + 5); }
+
+        int copy(double* dst, const double* src)
+        {
+            return hip_rc(hipMemcpyAsync(dst, src, sizeof(double) * nd, hipMemcpyDeviceToDevice, st));
+        }
+        int advance(double* x, long long k)
+        {
+            for (long long i = 0; i < k; ++i)
+            {
+                int rc = do_step(c, dt, x, 3, st);
+                if (rc) return rc;
+            }
+            S->forward_steps += static_cast<int>(k);
+            return FRSDG_OK;
+        }
+        int adj(const double* u, const double* w, const AdjOut& ao, double* out)
+        {
+            if (c->d.dims == 2) return by_n<Adj2>(c->n, c, u, w, ao, out, st);
+            return by_n<Adj1>(c->n, c, u, w, ao, out, st);
+        }
+        // lam <- (d u_{k+1} / d u_k)^T lam for the step that starts at u (SSPRK(3,3)
+        // with rhs = -R, then the post-step filter F):
+        //   u1 = u - dt R(u), u2 = 3/4 u + 1/4 (u1 - dt R(u1)), u+ = F (1/3 u + 2/3 (u2 - dt R(u2)));
+        //   l3 = F^T lam, l2 = 2/3 l3 - 2/3 dt J(u2)^T l3, l1 = 1/4 l2 - 1/4 dt J(u1)^T l2,
+        //   lam = l1 - dt J(u)^T l1 + 1/3 l3 + 3/4 l2.
+        int step_back(const double* u, double* lam)
+        {
+            int rc;
+            {
+                IOStage a{u, u, w1(), 0.0, 1.0, dt};
+                if ((rc = by_n<Stage>(c->n, c, a, 0, st))) return rc;
+                IOStage b{w1(), u, w2(), 0.75, 0.25, dt};
+                if ((rc = by_n<Stage>(c->n, c, b, 0, st))) return rc;
+            }
+            // l3 = F^T lam (a copy of lam without the filter: lam is overwritten last)
+            if (c->d.apply_filter)
+                rc = by_n<Filt>(c->n, c, lam, l3(), st, 1);
+            else
+                rc = copy(l3(), lam);
+            if (rc) return rc;
+            if ((rc = adj(w2(), l3(), AdjOut{2.0 / 3.0, -2.0 / 3.0 * dt, 0.0, 0.0, nullptr, nullptr}, l2()))) return rc;
+            if ((rc = adj(w1(), l2(), AdjOut{0.25, -0.25 * dt, 0.0, 0.0, nullptr, nullptr}, l1()))) return rc;
+            if ((rc = adj(u, l1(), AdjOut{1.0, -dt, 1.0 / 3.0, 0.75, l3(), l2()}, lam))) return rc;
+            S->adjoint_steps++;
+            return FRSDG_OK;
+        }
+        // reverse steps a..b-1 starting from the state ua at step a (never modified)
+        int reverse(long long a, long long b, const double* ua, double* lam, int free)
+        {
+            const long long T = b - a;
+            int rc;
+            if (T == 1) return step_back(ua, lam);
+            if (free == 0)
+            {
+                for (long long k = b - 1; k >= a; --k)
+                {
+                    if ((rc = copy(work(), ua)) || (rc = advance(work(), k - a)) || (rc = step_back(work(), lam)))
+                        return rc;
+                }
+                return FRSDG_OK;
+            }
+            const int       r      = revolve_repetitions(T, free);
+            const long long second = std::min(T - 1, binom(free - 1 + r, free - 1));
+            const long long m      = b - second;
+            double*         x      = slot(free - 1);
+            if ((rc = copy(x, ua)) || (rc = advance(x, m - a))) return rc;
+            S->snapshots_used = std::max(S->snapshots_used, ++live);
+            rc = reverse(m, b, x, lam, free - 1);
+            --live;
+            if (rc) return rc;
+            return reverse(a, m, ua, lam, free);
+        }
+    };
+} // namespace
+
+extern "C" {
+
+int frsdg_unsteady_adjoint(frsdg_ctx* c, double, double dt, int nsteps, int max_snapshots, const double* u0,
+                           const double* lam_T, double* lam_0, frsdg_adjoint_stats* stats, void* stream)
+{
+    if (!c || !u0 || !lam_T || !lam_0 || nsteps < 0 || max_snapshots < 0 || !(dt > 0.0)) return FRSDG_ERR_INVALID;
+    if (!aligned(u0) || !aligned(lam_T) || !aligned(lam_0)) return FRSDG_ERR_ALIGN;
+    if (lam_0 == u0) return FRSDG_ERR_INVALID;
+    // the WENO limiter (apply_weno without the filter) has no transpose
+    if (c->d.apply_weno && !c->d.apply_filter) return FRSDG_ERR_UNSUPPORTED;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    frsdg_adjoint_stats local{};
+    frsdg_adjoint_stats* S = stats ? stats : &local;
+    *S = frsdg_adjoint_stats{};
+    FR_TRY(hipStreamSynchronize(st));
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = ensure_work(c);
+    if (rc) return rc;
+    if (c->d.dims == 2 && !c->wb) FR_TRY(hipMalloc(&c->wb, sizeof(double) * 16 * c->n * c->E));
+    const int slots = static_cast<int>(std::min<long long>(max_snapshots, std::max(nsteps - 1, 0)));
+    const long long need = static_cast<long long>(slots + 6);
+    if (c->adj_pool_states < need)
+    {
+        if (c->adj_pool) FR_TRY(hipFree(c->adj_pool));
+        c->adj_pool        = nullptr;
+        c->adj_pool_states = 0;
+        FR_TRY(hipMalloc(&c->adj_pool, sizeof(double) * need * c->nd));
+        c->adj_pool_states = need;
+    }
+    if (lam_0 != lam_T) FR_TRY(hipMemcpyAsync(lam_0, lam_T, sizeof(double) * c->nd, hipMemcpyDeviceToDevice, st));
+    if (nsteps > 0)
+    {
+        UnsteadyAdj A{c, st, dt, c->adj_pool, slots, c->nd, S};
+        if ((rc = A.reverse(0, nsteps, u0, lam_0, slots))) return rc;
+    }
+    FR_TRY(hipStreamSynchronize(st));
+    S->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return FRSDG_OK;
 }
 
 } // extern "C"

@@ -44,6 +44,12 @@ struct SolverWs
     int*    info = nullptr; // per-block singularity flag
     // coloured preconditioning matrix and its ILU(0)
     frsdg::ColorPlan* cp = nullptr;
+    // lagged preconditioner of the Newton solves: which one the buffers hold
+    // (-1 none / stale) and how many Newton iterations have used it
+    int pc_built = -1;
+    int pc_age   = 0;
+    // timing events of the per-part breakdown (frsdg_newton_stats t_*)
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace frsdg
@@ -56,6 +62,8 @@ namespace frsdg
             if (p) (void) hipFree(p);
         if (s->info) (void) hipFree(s->info);
         if (s->Bf) (void) hipFree(s->Bf);
+        for (hipEvent_t& e : s->ev)
+            if (e) (void) hipEventDestroy(e);
         if (frsdg::ColorPlan* cp = s->cp)
         {
             for (int* p : {cp->nbr, cp->cls, cp->low, cp->upd, cp->upp, cp->lvl, cp->lpos, cp->upos, cp->info})
@@ -529,6 +537,13 @@ namespace
         std::vector<double>* hist = nullptr; // GMRES residual norms: ||b||, then the estimate after each iteration
 
         unsigned grid(long long n) const { return static_cast<unsigned>((n + 255) / 256); }
+        // seconds between two recorded events of s->ev (both complete)
+        double between(int a, int b) const
+        {
+            float ms = 0.0f;
+            return hipEventElapsedTime(&ms, s->ev[a], s->ev[b]) == hipSuccess ? 1e-3 * ms : 0.0;
+        }
+        void mark(int k) const { (void) hipEventRecord(s->ev[k], st); }
 
         int jv(const double* v, double* out)
         {
@@ -648,8 +663,15 @@ namespace
                 FR_TRY(hipMalloc(&cp->P, sizeof(double) * cp->E * cp->S * bsz));
                 FR_TRY(hipMalloc(&cp->Dinv, sizeof(double) * cp->E * 2 * bsz));
             }
+            mark(0);
             if ((rc = color_assemble(method, evalF, s->cp->P))) return rc;
-            return ilu_factor(s, st, s->cp->P, s->cp->Dinv);
+            mark(1);
+            if ((rc = ilu_factor(s, st, s->cp->P, s->cp->Dinv))) return rc;
+            mark(2);
+            FR_TRY(hipEventSynchronize(s->ev[2]));
+            stats->t_pc_assemble += between(0, 1);
+            stats->t_pc_factor += between(1, 2);
+            return FRSDG_OK;
         }
         int build_block_jacobi()
         {
@@ -662,6 +684,7 @@ namespace
             if (!s->pv) FR_TRY(hipMalloc(&s->pv, sizeof(double) * s->N));
             if (!s->pjv) FR_TRY(hipMalloc(&s->pjv, sizeof(double) * s->N));
             if (!s->info) FR_TRY(hipMalloc(&s->info, sizeof(int) * c->E));
+            mark(0);
             const int keep = transpose;
             transpose      = 0; // blocks are probed with forward products
             for (int col = 0; col < 2; ++col)
@@ -674,10 +697,16 @@ namespace
                                        s->pjv, s->B);
                 }
             transpose = keep;
+            mark(1);
             hipLaunchKernelGGL(k_block_inverse, dim3(static_cast<unsigned>(c->E)), dim3(256), 0, st, nd_e, s->B, s->info);
             if (!s->Bf) FR_TRY(hipMalloc(&s->Bf, sizeof(float) * nb));
             hipLaunchKernelGGL(k_to_float, dim3(grid(nb)), dim3(256), 0, st, nb, s->B, s->Bf);
-            return hip_rc(hipGetLastError());
+            FR_TRY(hipGetLastError());
+            mark(2);
+            FR_TRY(hipEventSynchronize(s->ev[2]));
+            stats->t_pc_assemble += between(0, 1);
+            stats->t_pc_factor += between(1, 2);
+            return FRSDG_OK;
         }
 
         // GMRES(m): solve J x = b (x returned in s->du), right preconditioned
@@ -689,12 +718,29 @@ namespace
             FR_TRY(hipMemsetAsync(x, 0, sizeof(double) * N, st));
             double beta0 = -1.0;
             its = 0;
+            // PETSc's GMRES default is left preconditioning with the preconditioned
+            // residual norm (-ksp_pc_side left); right keeps the true residual norm
+            const bool left = o->ksp_pc_side == 1 && o->precond != 0;
             // r = b - J x  (x = 0 on the first cycle)
             bool first = true;
             while (true)
             {
                 double* r = s->V; // V_0 holds r then v_0
-                if (first)
+                if (left)
+                {
+                    // left preconditioning: r = M^-1 (b - A x), x is the solution itself
+                    int rc;
+                    if (first)
+                        rc = prec(b, r);
+                    else
+                    {
+                        if ((rc = jv(x, s->w))) return rc;
+                        hipLaunchKernelGGL(k_waxpby, dim3(grid(N)), dim3(256), 0, st, N, 1.0, b, -1.0, s->w, s->z);
+                        rc = prec(s->z, r);
+                    }
+                    if (rc) return rc;
+                }
+                else if (first)
                     FR_TRY(hipMemcpyAsync(r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
                 else
                 {
@@ -720,9 +766,21 @@ namespace
                 for (; j < m && its < o->ksp_max_it; ++j)
                 {
                     ++its;
-                    int rc = prec(s->V + j * N, s->z);
-                    if (rc) return rc;
-                    if ((rc = jv(s->z, s->w))) return rc;
+                    int rc;
+                    mark(0);
+                    if (left)
+                    {
+                        if ((rc = jv(s->V + j * N, s->z))) return rc;
+                        mark(1);
+                        if ((rc = prec(s->z, s->w))) return rc;
+                    }
+                    else
+                    {
+                        if ((rc = prec(s->V + j * N, s->z))) return rc;
+                        mark(1);
+                        if ((rc = jv(s->z, s->w))) return rc;
+                    }
+                    mark(2);
                     // classical Gram-Schmidt: h = V^T w; w -= V h; ||w|| (optionally a second pass)
                     std::vector<double> hh(j + 1, 0.0), h2(j + 1);
                     double nw2 = 0.0;
@@ -737,6 +795,11 @@ namespace
                         FR_TRY(hipStreamSynchronize(st));
                         for (int i = 0; i <= j; ++i) hh[i] += h2[i];
                     }
+                    mark(3);
+                    FR_TRY(hipEventSynchronize(s->ev[3]));
+                    stats->t_pc_apply += left ? between(1, 2) : between(0, 1);
+                    stats->t_jv += left ? between(0, 1) : between(1, 2);
+                    stats->t_orth += between(2, 3);
                     const double hn = std::sqrt(nw2);
                     for (int i = 0; i <= j; ++i) H[i * m + j] = hh[i];
                     H[(j + 1) * m + j] = hn;
@@ -779,6 +842,7 @@ namespace
                 FR_TRY(hipStreamSynchronize(st));
                 if (done || its >= o->ksp_max_it) break;
             }
+            if (left) return FRSDG_OK;
             // solution = M^-1 x
             int rc = prec(x, s->z);
             if (rc) return rc;
@@ -796,6 +860,7 @@ namespace
         s->N    = c->nd;
         s->m    = m;
         s->nb   = static_cast<int>(std::min<long long>(1024, (c->nd + RB - 1) / RB));
+        for (hipEvent_t& e : s->ev) FR_TRY(hipEventCreate(&e));
         FR_TRY(hipMalloc(&s->V, sizeof(double) * (m + 1) * s->N));
         for (double** p : {&s->w, &s->z, &s->un, &s->ud, &s->F, &s->du, &s->ut}) FR_TRY(hipMalloc(p, sizeof(double) * s->N));
         FR_TRY(hipMalloc(&s->part, sizeof(double) * (m + 2) * s->nb));
@@ -853,16 +918,30 @@ extern "C" int frsdg_beuler_step(frsdg_ctx* c, double t, double dt, double* u, c
     double fn = K.norm(s->F);
     S->fnorm0 = fn;
     const double target = std::max(o->snes_rtol * fn, o->snes_atol);
+    bool         step_conv = false;
     while (fn > target && S->newton_its < o->snes_max_it && std::isfinite(fn))
     {
-        if (o->precond >= 3)
+        // the reference rebuilds its preconditioning matrix at every Newton
+        // iteration (PETSc's default lag 1); pc_lag k > 1 rebuilds it every
+        // k-th Newton iteration, counted across the steps of this context
+        // (-snes_lag_preconditioner k -snes_lag_preconditioner_persists)
+        if (o->precond >= 1)
         {
-            // the reference rebuilds its preconditioning matrix at every Newton
-            // iteration (PETSc's default lag 1)
-            if ((rc = K.build_ilu(o->precond == 3 ? 0 : 2, eval_F))) return rc;
+            const bool rebuild = s->pc_built != o->precond || o->pc_lag <= 1 || s->pc_age >= o->pc_lag;
+            if (rebuild)
+            {
+                s->pc_built = -1;
+                if (o->precond >= 3)
+                    rc = K.build_ilu(o->precond == 3 ? 0 : 2, eval_F);
+                else
+                    rc = K.build_block_jacobi();
+                if (rc) return rc;
+                s->pc_built = o->precond;
+                s->pc_age   = 0;
+                S->pc_builds++;
+            }
+            s->pc_age++;
         }
-        else if (o->precond >= 1)
-            if ((rc = K.build_block_jacobi())) return rc;
         // J du = -F
         hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, st, N, 0.0, s->F, -1.0, s->F);
         int its = 0;
@@ -934,13 +1013,26 @@ extern "C" int frsdg_beuler_step(frsdg_ctx* c, double t, double dt, double* u, c
             }
         }
         if (!decreased) break; // line search failed (SNES_DIVERGED_LINE_SEARCH): keep u
+        // step-norm test of SNESConvergedDefault: ||lambda du|| <= stol ||u_new||
+        bool snorm_conv = false;
+        if (o->snes_stol > 0.0)
+        {
+            const double ynorm = lam * K.norm(s->du);
+            const double xnorm = K.norm(s->ut);
+            snorm_conv = ynorm <= o->snes_stol * xnorm;
+        }
         FR_TRY(hipMemcpyAsync(u, s->ut, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
         FR_TRY(hipMemcpyAsync(s->F, s->w, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
         fn = fnew;
         S->newton_its++;
+        if (snorm_conv && fn > target)
+        {
+            step_conv = true; // SNES_CONVERGED_SNORM_RELATIVE
+            break;
+        }
     }
     S->fnorm     = fn;
-    S->converged = fn <= target;
+    S->converged = fn <= target ? 1 : (step_conv ? 2 : 0);
     // With the filter or the WENO limiter on, the reference stabilises the
     // solution once per implicit step (TSSetPostStep, or TSSetPostEvaluate
     // under -do_post_ts false: petsc_solver_impl.hpp:582-596; both run at the
@@ -956,7 +1048,7 @@ extern "C" int frsdg_beuler_step(frsdg_ctx* c, double t, double dt, double* u, c
 extern "C" int frsdg_pseudo_run(frsdg_ctx* c, double t, double dt0, double increment, int from_initial_dt,
                                 double dt_max, double t_max, int max_steps, double* u,
                                 const frsdg_newton_opts* o, double* dt_hist, double* fnorm_hist, double* time_hist,
-                                int* steps, void* stream)
+                                int* steps, frsdg_newton_stats* stats, void* stream)
 {
     if (!c || !u || !o || !steps || !(dt0 > 0.0) || !(increment > 0.0) || max_steps < 0) return FRSDG_ERR_INVALID;
     if (!aligned(u)) return FRSDG_ERR_ALIGN;
@@ -964,47 +1056,70 @@ extern "C" int frsdg_pseudo_run(frsdg_ctx* c, double t, double dt0, double incre
     int rc = ensure_ws(c, o->ksp_restart);
     if (rc) return rc;
     SolverWs*   s  = c->ws;
+    s->pc_built    = -1; // a run starts with a fresh preconditioner (lagged ones persist within it)
     hipStream_t st = static_cast<hipStream_t>(stream);
-    double      dt = dt0, f_first = -1.0, f_prev = -1.0;
     FR_TRY(hipStreamSynchronize(st));
     const auto t_start = std::chrono::steady_clock::now();
     const double t_end_tol = 1e-12 * (t_max > 0.0 ? t_max : 1.0);
+    // f(u) = ||F(u, U_t = 0)||: the IFunction with a zero time derivative
+    auto steady_norm = [&](double tt, double& f) -> int
+    {
+        if (hipMemsetAsync(s->ud, 0, sizeof(double) * s->N, st) != hipSuccess) return FRSDG_ERR_HIP;
+        int r = frsdg_ifunction(c, tt, u, s->ud, s->w, stream);
+        if (r) return r;
+        frsdg_newton_stats ns{};
+        Ctx K{c, s, st, o, tt, 0.0, u, &ns};
+        f = K.norm(s->w);
+        return FRSDG_OK;
+    };
+    // TSStep_Pseudo computes each step's size before the step, from the current
+    // state (TSPseudoComputeTimeStep -> TSPseudoTimeStepDefault): the first
+    // call records fnorm_initial = fnorm_previous = f(u_0), so the first step
+    // already takes inc * dt0
+    double f = 0.0;
+    if ((rc = steady_norm(t, f))) return rc;
+    const double f_init = f;
+    frsdg_newton_stats tot{};
+    tot.fnorm0 = f;
+    double       f_prev = f, dt = dt0;
     for (int k = 0; k < max_steps; ++k)
     {
-        if (t_max > 0.0)
-        {
-            if (t >= t_max - t_end_tol) break;
-            if (t + dt > t_max) dt = t_max - t; // -ts_exact_final_time matchstep
-        }
-        frsdg_newton_stats ns{};
-        if ((rc = frsdg_beuler_step(c, t, dt, u, o, &ns, stream))) return rc;
-        t += dt;
-        // f = ||F(u, U_t = 0)||: the IFunction with a zero time derivative
-        FR_TRY(hipMemsetAsync(s->ud, 0, sizeof(double) * s->N, st));
-        if ((rc = frsdg_ifunction(c, t, u, s->ud, s->w, stream))) return rc;
-        Ctx K{c, s, st, o, t, 0.0, u, &ns};
-        const double f = K.norm(s->w);
-        if (dt_hist) dt_hist[k] = dt;
-        if (fnorm_hist) fnorm_hist[k] = f;
-        if (time_hist) time_hist[k] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
-        *steps = k + 1;
-        if (f_first < 0.0)
-        {
-            f_first = f;
-            f_prev  = f;
-        }
+        if (t_max > 0.0 && t >= t_max - t_end_tol) break;
         double nd;
         if (f == 0.0)
             nd = 1.e12 * increment * dt;
         else if (from_initial_dt)
-            nd = increment * dt0 * f_first / f;
+            nd = increment * dt0 * f_init / f;
         else
             nd = increment * dt * f_prev / f;
         if (dt_max > 0.0) nd = std::min(nd, dt_max);
         f_prev = f;
         dt     = nd;
+        if (t_max > 0.0 && t + dt > t_max) dt = t_max - t; // -ts_exact_final_time matchstep
+        frsdg_newton_stats ns{};
+        if ((rc = frsdg_beuler_step(c, t, dt, u, o, &ns, stream))) return rc;
+        tot.newton_its += ns.newton_its;
+        tot.ksp_its += ns.ksp_its;
+        tot.jv_evals += ns.jv_evals;
+        tot.res_evals += ns.res_evals;
+        tot.pc_builds += ns.pc_builds;
+        tot.seconds += ns.seconds;
+        tot.t_pc_assemble += ns.t_pc_assemble;
+        tot.t_pc_factor += ns.t_pc_factor;
+        tot.t_pc_apply += ns.t_pc_apply;
+        tot.t_jv += ns.t_jv;
+        tot.t_orth += ns.t_orth;
+        tot.converged = ns.converged;
+        t += dt;
+        if ((rc = steady_norm(t, f))) return rc;
+        tot.fnorm = f;
+        if (dt_hist) dt_hist[k] = dt;
+        if (fnorm_hist) fnorm_hist[k] = f;
+        if (time_hist) time_hist[k] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+        *steps = k + 1;
     }
     FR_TRY(hipStreamSynchronize(st));
+    if (stats) *stats = tot;
     return FRSDG_OK;
 }
 
@@ -1024,6 +1139,7 @@ extern "C" int frsdg_adjoint_solve(frsdg_ctx* c, double t, double sigma, const d
     int rc = ensure_ws(c, o->ksp_restart);
     if (rc) return rc;
     SolverWs* s = c->ws;
+    s->pc_built = -1; // this solve builds its own preconditioner into the shared buffers
     hipStream_t st = static_cast<hipStream_t>(stream);
     FR_TRY(hipStreamSynchronize(st));
     const auto t0 = std::chrono::steady_clock::now();
@@ -1152,6 +1268,7 @@ extern "C" int frsdg_linear_solve(frsdg_ctx* c, double t, double sigma, const do
     int rc = ensure_ws(c, o->ksp_restart);
     if (rc) return rc;
     SolverWs* s = c->ws;
+    s->pc_built = -1; // this solve builds its own preconditioner into the shared buffers
     hipStream_t st = static_cast<hipStream_t>(stream);
     FR_TRY(hipStreamSynchronize(st));
     const auto t0 = std::chrono::steady_clock::now();

@@ -205,13 +205,32 @@ typedef struct frsdg_newton_opts
     int    ksp_max_it;  /* PETSc default 10000 */
     double ksp_rtol;    /* PETSc default 1e-5 */
     int    ksp_cgs_refine; /* 0 one classical Gram-Schmidt pass (PETSc default), 1 always two */
+    double snes_stol;   /* also stop when ||lambda du|| <= snes_stol * ||u|| after a step
+                           (SNES_CONVERGED_SNORM_RELATIVE; PETSc default 1e-8, 0 off) */
+    int    ksp_pc_side; /* 0 right preconditioning, test on the true residual norm;
+                           1 left, test on the preconditioned residual norm (PETSc's
+                           GMRES default, -ksp_pc_side left) */
+    int    pc_lag;      /* rebuild the preconditioner every pc_lag-th Newton iteration,
+                           counted across consecutive frsdg_beuler_step calls and the
+                           steps of one frsdg_pseudo_run (-snes_lag_preconditioner k
+                           -snes_lag_preconditioner_persists); <= 1 every iteration
+                           (PETSc default) */
 } frsdg_newton_opts;
 
 typedef struct frsdg_newton_stats
 {
+    /* converged: 1 by the residual tests (rtol / atol), 2 by the step-norm test
+       (snes_stol), 0 not converged */
     int    newton_its, ksp_its, jv_evals, res_evals, converged;
     double fnorm0, fnorm;
     double seconds; /* wall time of the solve, stream-synchronised */
+    /* where the time went (GPU seconds between events on the stream): building
+       the preconditioner (pc_builds times: the coloured matrix or the probed
+       blocks, then their ILU(0) / inverses), applying it, the J*v products of
+       GMRES and its Gram-Schmidt orthogonalisation; the rest of `seconds` is
+       residual evaluations, the line search and host work */
+    int    pc_builds;
+    double t_pc_assemble, t_pc_factor, t_pc_apply, t_jv, t_orth;
 } frsdg_newton_stats;
 
 /* One backward-Euler step: solves F(u) = (u - u^n)/dt + R(u) = 0 for u^{n+1}.
@@ -227,19 +246,22 @@ int frsdg_beuler_step(frsdg_ctx* ctx, double t, double dt, double* u, const frsd
  * [-ts_pseudo_increment_dt_from_initial_dt] -ts_pseudo_fatol 0
  * -ts_pseudo_frtol 0, tools/boundary_conforming_claude.py:113-127):
  * backward-Euler steps (frsdg_beuler_step) whose size follows
- * TSPseudoTimeStepDefault: the first step takes dt0; after every step
- * f = ||F(u, U_t = 0)||_2 (the IFunction, i.e. the steady residual) and the
- * next step takes inc * dt0 * f_first / f (from_initial_dt) or
- * inc * dt * f_previous / f, capped at dt_max when dt_max > 0, f_first being
- * the first f computed.  Stops after max_steps steps or when t reaches t_max
- * (t_max > 0; the last step is shortened to land on it, -ts_exact_final_time
- * matchstep).  dt_hist, fnorm_hist and time_hist (max_steps entries, may be
- * NULL) get each step's dt, the f after it and the wall seconds since the
- * start (stream-synchronised); *steps the number of steps taken.
- * u is updated in place.  Synchronises the stream. */
+ * TSPseudoTimeStepDefault, evaluated before each step as TSStep_Pseudo does:
+ * with f_k = ||F(u_k, U_t = 0)||_2 (the IFunction at the current state, i.e.
+ * the steady residual) and f_0 that of the initial state, step k takes
+ * inc * dt0 * f_0 / f_k (from_initial_dt) or inc * dt_{k-1} * f_{k-1} / f_k
+ * (dt_{-1} = dt0, f_{-1} = f_0), capped at dt_max when dt_max > 0 (so the
+ * first step takes inc * dt0).  Stops after max_steps steps or when t reaches
+ * t_max (t_max > 0; the last step is shortened to land on it,
+ * -ts_exact_final_time matchstep).  dt_hist, fnorm_hist and time_hist
+ * (max_steps entries, may be NULL) get each step's dt, the f after it and the
+ * wall seconds since the start (stream-synchronised); *steps the number of
+ * steps taken; stats (may be NULL) the per-step Newton statistics summed over
+ * the run, with fnorm0 = f_0 and fnorm the last f.  u is updated in place.
+ * Synchronises the stream. */
 int frsdg_pseudo_run(frsdg_ctx* ctx, double t, double dt0, double increment, int from_initial_dt, double dt_max,
                      double t_max, int max_steps, double* u, const frsdg_newton_opts* opts, double* dt_hist,
-                     double* fnorm_hist, double* time_hist, int* steps, void* stream);
+                     double* fnorm_hist, double* time_hist, int* steps, frsdg_newton_stats* stats, void* stream);
 
 /* The preconditioning matrix of the reference's implicit path
  * (IJacobianStrategy::setup_colored_preconditioner and MatFDColoringApply
@@ -302,6 +324,30 @@ int frsdg_linear_solve(frsdg_ctx* ctx, double t, double sigma, const double* u, 
 int frsdg_adjoint_solve(frsdg_ctx* ctx, double t, double sigma, const double* u, const double* rhs,
                         double* lambda, const frsdg_newton_opts* opts, frsdg_newton_stats* stats,
                         void* stream);
+
+/* Unsteady discrete adjoint of explicit SSPRK(3,3) stepping (frsdg_ssprk_step
+ * with stages 3: rhs = -R, post-step spectral filter when active): lam_0 =
+ * (d u_N / d u_0)^T lam_T for u_N = S^N(u_0) with steps of size dt.  No
+ * reference counterpart.  The states the reverse sweep needs are recomputed
+ * from at most max_snapshots stored states (binomial / revolve schedule of
+ * Griewank: a segment of T steps with s free slots and r repetitions,
+ * C(s + r, s) >= T, reverses its last min(T - 1, C(s - 1 + r, s - 1)) steps
+ * first from a new snapshot).  Each step back recomputes the step's first two
+ * stage states and applies the transposed stage combinations through three
+ * adjoint-kernel launches (the combinations fused into their outputs) and the
+ * transposed filter.  Device memory: (min(max_snapshots, N - 1) + 6) states,
+ * kept by the context for later calls.  lam_0 may alias lam_T.  The WENO
+ * limiter has no transpose: FRSDG_ERR_UNSUPPORTED when it is the post-step
+ * stabiliser.  Synchronises the stream. */
+typedef struct frsdg_adjoint_stats
+{
+    int    forward_steps;  /* forward steps recomputed from snapshots */
+    int    adjoint_steps;  /* steps reversed (= N) */
+    int    snapshots_used; /* largest number of snapshots held at once */
+    double seconds;        /* wall time, stream-synchronised */
+} frsdg_adjoint_stats;
+int frsdg_unsteady_adjoint(frsdg_ctx* ctx, double t, double dt, int nsteps, int max_snapshots, const double* u0,
+                           const double* lam_T, double* lam_0, frsdg_adjoint_stats* stats, void* stream);
 
 /* ---------------- batched sweep of independent problems ----------------
  * No reference counterpart (the reference runs one problem per process).

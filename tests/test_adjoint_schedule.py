@@ -1,14 +1,15 @@
-"""CPU check of the binomial checkpointing schedule of fr_sdg_amd.adjoint on a
-mock evaluator whose step is a fixed nonlinear map: the gradient with few
-snapshots equals the store-everything gradient, and the forward work obeys the
-revolve bound (at most r forward sweeps of the T steps)."""
+"""CPU check of the reference binomial checkpointing schedule (tests/revolve_ref.py,
+which frsdg_unsteady_adjoint must follow step for step) on a mock evaluator
+whose step is a fixed nonlinear map: the gradient with few snapshots equals the
+store-everything gradient, and the forward work obeys the revolve bound (at
+most r forward sweeps of the T steps)."""
 from math import comb
 
 import numpy as np
 import pytest
 import torch
 
-from fr_sdg_amd import adjoint as A
+import revolve_ref as A
 
 
 class MockEv:
@@ -24,7 +25,7 @@ class MockEv:
         u.copy_(torch.tanh(self.M @ u))
 
 
-class MockAdjoint(A.UnsteadyAdjoint):
+class MockAdjoint(A.Revolve):
     def adjoint_step(self, u, lam):
         self.adjoint_steps += 1
         y = torch.tanh(self.ev.M @ u)
