@@ -98,8 +98,12 @@ EXPORTS = [
     "frsdg_batch_ssprk_step", "frsdg_batch_ssprk_run", "frsdg_batch_ssprk_step_each",
     "frsdg_filter_transpose", "frsdg_mass", "frsdg_boundary_weights", "frsdg_adjoint_solve", "frsdg_pseudo_run",
     "frsdg_color_jacobian", "frsdg_ilu0_factor", "frsdg_ilu0_apply", "frsdg_linear_solve",
-    "frsdg_ilu0_apply_transpose", "frsdg_l2_error_noalloc", "frsdg_unsteady_adjoint",
+    "frsdg_ilu0_apply_transpose", "frsdg_l2_error_noalloc", "frsdg_unsteady_adjoint", "frsdg_step_health",
+    "frsdg_check_state",
 ]
+
+# health flags of stored states (frsdg_step_health / frsdg_check_state)
+HEALTH_NONFINITE, HEALTH_DENSITY, HEALTH_ENERGY = 1, 2, 4
 
 _lib = None
 
@@ -133,6 +137,8 @@ def load_library(path=LIB_PATH):
     L.frsdg_impose_ic.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double), dp, vp]
     L.frsdg_l2_error.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double), d, dp,
                                  ctypes.POINTER(ctypes.c_double), vp]
+    L.frsdg_step_health.argtypes = [vp, ctypes.POINTER(ctypes.c_int), i, vp]
+    L.frsdg_check_state.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int), vp]
     L.frsdg_unsteady_adjoint.argtypes = [vp, d, d, i, i, dp, dp, dp, ctypes.POINTER(AdjointStats), vp]
     L.frsdg_l2_error_noalloc.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double), d, dp,
                                          ctypes.POINTER(ctypes.c_double), vp]
@@ -323,8 +329,22 @@ class PhysicsEvaluator:
         return u
 
     def ssprk_run(self, u, dt, nsteps, t=0.0, stages=3):
+        """nsteps graph-replayed steps; raises FrsdgError (numerical failure) when a
+        stage output goes non-finite or non-positive (the run stops there)."""
         _check(self.lib.frsdg_ssprk_run(self.h, t, dt, _ptr(u), stages, nsteps, _stream()))
         return u
+
+    def step_health(self, clear=True):
+        """Health flags accumulated by ssprk_step (HEALTH_* bits; 0 = healthy)."""
+        f = ctypes.c_int(0)
+        _check(self.lib.frsdg_step_health(self.h, ctypes.byref(f), int(clear), _stream()))
+        return f.value
+
+    def check_state(self, u):
+        """Health flags of a conservative state (HEALTH_* bits; 0 = healthy)."""
+        f = ctypes.c_int(0)
+        _check(self.lib.frsdg_check_state(self.h, _ptr(u), ctypes.byref(f), _stream()))
+        return f.value
 
     def beuler_step(self, u, dt, t=0.0, **opts):
         """One backward-Euler step solved by JFNK (TSSetIFunction path); u updated in place."""

@@ -529,8 +529,32 @@ namespace
 
     unsigned node_grid(const frsdg_ctx* c) { return static_cast<unsigned>((c->nm.nodes + 255) / 256); }
 
+    // Health flags of stored states (bit 1 non-finite, bit 2 rho <= 0, bit 4
+    // internal energy <= 0), one thread per node; 2D and 1D layouts
+    __global__ __launch_bounds__(256) void k_health(NodeMap M, const double* __restrict__ u, int* flag)
+    {
+        const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+        if (i >= M.nodes) return;
+        double o[4];
+        if (M.dims == 2)
+            ld4d(u + i * 4, o);
+        else
+        {
+            o[0] = u[M.at(i, 0)];
+            o[1] = u[M.at(i, 1)];
+            o[2] = 0.0;
+            o[3] = u[M.at(i, 2)];
+        }
+        health_check(o, flag);
+    }
+
     int ensure_work(frsdg_ctx* c)
     {
+        if (!c->health)
+        {
+            FR_TRY(hipMalloc(&c->health, sizeof(int)));
+            FR_TRY(hipMemset(c->health, 0, sizeof(int)));
+        }
         if (!c->w1) FR_TRY(hipMalloc(&c->w1, sizeof(double) * c->nd));
         if (!c->w2) FR_TRY(hipMalloc(&c->w2, sizeof(double) * c->nd));
         if (weno_only(c) && c->d.dims == 2 && !c->wbeta)
@@ -559,10 +583,20 @@ namespace
     // one explicit step, stages fused with the residual kernel
     int do_step(frsdg_ctx* c, double dt, double* u, int stages, hipStream_t s)
     {
+        // 2D stage outputs are checked in the stage kernel's store (except the
+        // one that feeds the WENO limiter, checked after the limiter); 1D states
+        // after the step
+        int* hf    = c->d.dims == 2 ? c->health : nullptr;
         auto stage = [&](const double* us, double a0, double b, double cdt, double* out)
         {
-            IOStage io{us, u, out, a0, b, cdt};
+            IOStage io{us, u, out, a0, b, cdt, hf};
             return by_n<Stage>(c->n, c, io, 0, s);
+        };
+        auto check_u = [&]() -> int
+        {
+            if (c->d.dims == 2 && !weno_only(c)) return FRSDG_OK;
+            hipLaunchKernelGGL(k_health, dim3(node_grid(c)), dim3(256), 0, s, c->nm, u, c->health);
+            return hip_rc(hipGetLastError());
         };
         int rc;
         if (stages == 3)
@@ -572,14 +606,16 @@ namespace
             if (c->d.apply_filter && c->d.dims == 2)
             {
                 // last stage fused with the post-step filter
-                IOStage io{c->w2, u, u, 1.0 / 3.0, 2.0 / 3.0, dt};
+                IOStage io{c->w2, u, u, 1.0 / 3.0, 2.0 / 3.0, dt, hf};
                 return by_n<Stage>(c->n, c, io, 1, s);
             }
             if (weno_only(c))
             {
                 // last stage into w1 (free after stage two), then limit into u
-                if ((rc = stage(c->w2, 1.0 / 3.0, 2.0 / 3.0, dt, c->w1))) return rc;
-                return by_n<Weno>(c->n, c, c->w1, u, s);
+                IOStage io{c->w2, u, c->w1, 1.0 / 3.0, 2.0 / 3.0, dt};
+                if ((rc = by_n<Stage>(c->n, c, io, 0, s))) return rc;
+                if ((rc = by_n<Weno>(c->n, c, c->w1, u, s))) return rc;
+                return check_u();
             }
             if ((rc = stage(c->w2, 1.0 / 3.0, 2.0 / 3.0, dt, u))) return rc;
         }
@@ -591,14 +627,18 @@ namespace
             if ((rc = stage(c->w2, 2.0 / 3.0, 1.0 / 3.0, h, c->w1))) return rc;
             // last stage reads w1 (neighbour traces) and writes u (w2 when the
             // WENO limiter follows)
-            IOStage io{c->w1, nullptr, weno_only(c) ? c->w2 : u, 0.0, 1.0, h};
+            IOStage io{c->w1, nullptr, weno_only(c) ? c->w2 : u, 0.0, 1.0, h, weno_only(c) ? nullptr : hf};
             if ((rc = by_n<Stage>(c->n, c, io, 0, s))) return rc;
-            if (weno_only(c)) return by_n<Weno>(c->n, c, c->w2, u, s);
+            if (weno_only(c))
+            {
+                if ((rc = by_n<Weno>(c->n, c, c->w2, u, s))) return rc;
+                return check_u();
+            }
         }
         else
             return FRSDG_ERR_INVALID;
-        if (c->d.apply_filter) return by_n<Filt>(c->n, c, u, u, s);
-        return FRSDG_OK;
+        if (c->d.apply_filter && (rc = by_n<Filt>(c->n, c, u, u, s))) return rc;
+        return check_u();
     }
 
     GeomP geom_of(const frsdg_desc& d)
@@ -671,7 +711,7 @@ const char* frsdg_strerror(int code)
         case FRSDG_ERR_HIP: return "HIP runtime error";
         case FRSDG_ERR_UNSUPPORTED: return "unsupported on this path";
         case FRSDG_ERR_ALIGN: return "device pointer not 16-byte aligned";
-        case FRSDG_ERR_NUMERIC: return "zero pivot in a factorisation";
+        case FRSDG_ERR_NUMERIC: return "numerical failure (zero pivot, or a non-finite / non-positive state)";
     }
     return "unknown error";
 }
@@ -845,6 +885,8 @@ int frsdg_destroy(frsdg_ctx* c)
     if (c->graph) (void) hipGraphDestroy(c->graph);
     for (double* p : {c->d_tab, c->d_vmet, c->d_fxm, c->d_fym, c->d_csi, c->d_rinv, c->w1, c->w2, c->red, c->wb,
                       c->wbeta, c->adj_pool})
+        if (p) (void) hipFree(p);
+    for (int* p : {c->health, c->health_chk})
         if (p) (void) hipFree(p);
     delete c;
     return FRSDG_OK;
@@ -1063,11 +1105,20 @@ int frsdg_ssprk_run(frsdg_ctx* c, double t, double dt, double* u, int stages, in
     if (rc) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
     constexpr int CH = 16; // steps per captured graph
+    // the health flags of this run (outside any capture, so a memset is fine)
+    FR_TRY(hipMemsetAsync(c->health, 0, sizeof(int), s));
+    auto healthy = [&]() -> int
+    {
+        int f = 0;
+        FR_TRY(hipMemcpyAsync(&f, c->health, sizeof(int), hipMemcpyDeviceToHost, s));
+        FR_TRY(hipStreamSynchronize(s));
+        return f ? FRSDG_ERR_NUMERIC : FRSDG_OK;
+    };
     if (nsteps < CH) // short runs: plain launches
     {
         for (int k = 0; k < nsteps; ++k)
             if ((rc = frsdg_ssprk_step(c, t + k * dt, dt, u, stages, stream))) return rc;
-        return FRSDG_OK;
+        return healthy();
     }
     if (!c->gexec || c->g_u != u || c->g_dt != dt || c->g_st != stages)
     {
@@ -1095,9 +1146,41 @@ int frsdg_ssprk_run(frsdg_ctx* c, double t, double dt, double* u, int stages, in
         c->g_st = stages;
     }
     const int full = nsteps / CH, rem = nsteps % CH;
-    for (int k = 0; k < full; ++k) FR_TRY(hipGraphLaunch(c->gexec, s));
+    // -ts_error_if_step_fails: stop after the first graph of steps whose stage
+    // outputs went non-finite or non-positive (checked once per graph)
+    for (int k = 0; k < full; ++k)
+    {
+        FR_TRY(hipGraphLaunch(c->gexec, s));
+        if ((rc = healthy())) return rc;
+    }
     for (int k = 0; k < rem; ++k)
         if ((rc = do_step(c, dt, u, stages, s))) return rc;
+    return healthy();
+}
+
+int frsdg_step_health(frsdg_ctx* c, int* flags, int clear, void* stream)
+{
+    if (!c || !flags) return FRSDG_ERR_INVALID;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    *flags = 0;
+    if (!c->health) return FRSDG_OK;
+    FR_TRY(hipMemcpyAsync(flags, c->health, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (clear) FR_TRY(hipMemsetAsync(c->health, 0, sizeof(int), s));
+    FR_TRY(hipStreamSynchronize(s));
+    return FRSDG_OK;
+}
+
+int frsdg_check_state(frsdg_ctx* c, const double* u, int* flags, void* stream)
+{
+    if (!c || !u || !flags) return FRSDG_ERR_INVALID;
+    if (!aligned(u)) return FRSDG_ERR_ALIGN;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!c->health_chk) FR_TRY(hipMalloc(&c->health_chk, sizeof(int)));
+    FR_TRY(hipMemsetAsync(c->health_chk, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_health, dim3(node_grid(c)), dim3(256), 0, s, c->nm, u, c->health_chk);
+    FR_TRY(hipGetLastError());
+    FR_TRY(hipMemcpyAsync(flags, c->health_chk, sizeof(int), hipMemcpyDeviceToHost, s));
+    FR_TRY(hipStreamSynchronize(s));
     return FRSDG_OK;
 }
 

@@ -36,6 +36,8 @@ struct frsdg_ctx
     double* red = nullptr; // reduction partials
     double* wb  = nullptr; // adjoint lifted face cotangents [E][4][n][4]
     double* wbeta = nullptr; // WENO smoothness of every element line [E][2][n][4]
+    int*      health          = nullptr; // stage-output health flags of explicit stepping (device)
+    int*      health_chk      = nullptr; // frsdg_check_state flags (device)
     double*   adj_pool        = nullptr; // unsteady adjoint: snapshots + work states
     long long adj_pool_states = 0;
     // WENO limiter constants (apply_weno without the spectral filter)

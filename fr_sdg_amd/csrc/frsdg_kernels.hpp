@@ -1,31 +1,35 @@
 // frsdg_kernels.hpp — HIP kernels of the FR residual path for gfx950.
 //
-// Residual kernel design (2D, one launch per residual):
-//   * a workgroup owns EPB consecutive elements (EPB = floor(BS / n^2)),
-//     one thread per solution node; the element blocks are contiguous in HBM
-//     so the 32-byte node loads/stores of a workgroup are fully coalesced;
-//   * each thread evaluates the Euler fluxes F,G of its node once and stages
-//     them in LDS; the derivative-matrix contraction then runs as line tasks
-//     (one element row or column, one equation per thread): the line is read
-//     from LDS once into registers, D comes through the scalar cache, and the
-//     metric-weighted derivatives of all N nodes of the line go back to LDS,
-//     so each flux value is read twice instead of 2N (4N curved) times;
-//   * "owner computes" faces: every element evaluates the Riemann flux of
-//     its own four faces (4n face-node tasks), gathering the neighbour trace
-//     straight from global memory.  A shared face is evaluated by both
-//     neighbours with identical arguments, so the result is deterministic
-//     and conservative without atomics;
-//   * the VCJH lift and the volume term are combined per node and written
-//     once; optional output policies fuse rhs = -R, F = udot + R and the
-//     SSP-RK stage update into the same store;
-//   * affine meshes use the four constant metric terms (only the
-//     d/dxi of F and d/deta of G survive, as in the reference where the
-//     cross terms are exact zeros); curved meshes read per-node metrics and
-//     per-face (nx, ny, scale) precomputed once in HBM;
-//   * block index -> element tile is remapped so that each XCD walks a
-//     contiguous band of elements (neighbour traces hit its own L2).
-// The same kernel body is instantiated on double (residual), Dual (forward
-// tangent J*v) and Cplx (complex step), see frsdg_scalar.hpp.
+// Residual kernel design (2D, one launch per residual or SSP-RK stage):
+//   * a workgroup owns a TX x TY block (tile) of the element grid, one thread
+//     per solution node (EPB = floor(256 / n^2) elements; n = 8: 2 x 2); the
+//     element blocks are contiguous in HBM, so the 32-byte node loads / stores
+//     of a workgroup are coalesced;
+//   * each thread evaluates the Euler fluxes F, G of its node once and stages
+//     U, F, G in LDS;
+//   * face tasks: every face of the tile is evaluated once.  Interior faces
+//     take both traces from LDS; faces on the tile rim gather the outer trace
+//     from HBM (issued at kernel start) or form it from the boundary
+//     condition.  The scaled flux jump is written for the tile elements on
+//     both sides.  Neighbouring tiles evaluate a shared rim face with
+//     identical arguments: bitwise the same flux, conservative and
+//     deterministic, no atomics;
+//   * the derivative-matrix contraction runs as line tasks (one element row
+//     or column, one equation per thread): the line is read from LDS once into
+//     registers, D comes through the scalar cache, and the metric-weighted
+//     derivatives of all n nodes of the line go back to LDS;
+//   * per node: volume term + the four VCJH lifts, written once; output
+//     policies fuse rhs = -R, F = udot + R and the SSP-RK stage update (the
+//     last stage of a step optionally applies the post-step spectral filter
+//     in LDS before the store) into the same store;
+//   * affine meshes use the four constant metric terms (only d/dxi of F and
+//     d/deta of G survive, as in the reference where the cross terms are exact
+//     zeros); curved meshes read per-node metrics and per-face (nx, ny, scale)
+//     precomputed once in HBM;
+//   * block index -> tile is remapped so that each XCD walks a contiguous band
+//     of tiles (neighbour traces hit its own L2).
+// The same tile code is instantiated on double (residual, stages), Dual
+// (forward tangent J*v) and Cplx (complex step), see frsdg_scalar.hpp.
 #pragma once
 #include <type_traits>
 #include "frsdg_geometry.hpp"
@@ -73,6 +77,18 @@ namespace frsdg
         reinterpret_cast<double2*>(p)[1] = make_double2(o[2], o[3]);
     }
 
+    // Health of a stored 2D node state (rho, rho u, rho v, rho E): bit 1 a
+    // non-finite value, bit 2 rho <= 0, bit 4 internal energy <= 0
+    // (2 rho (rho E) <= |rho v|^2).  Flags are OR-ed into *flag, only on failure.
+    __device__ __forceinline__ void health_check(const double o[4], int* flag)
+    {
+        int f = 0;
+        if (!__builtin_isfinite(o[0] + o[1] + o[2] + o[3])) f |= 1;
+        if (!(o[0] > 0.0)) f |= 2;
+        if (!(2.0 * o[0] * o[3] > o[1] * o[1] + o[2] * o[2])) f |= 4;
+        if (f) atomicOr(flag, f);
+    }
+
     // out = add + scale * R   (residual: scale 1; rhs: scale -1; ifunction: add = udot)
     struct IORes
     {
@@ -113,6 +129,7 @@ namespace frsdg
         const double* u0; // may alias out (each node read by its own thread before any write)
         double* out;
         double  a0, b, cdt;
+        int*    health = nullptr; // 2D: flags of the stored node states (health_flags), may be null
         __device__ double ld(long long i) const { return u[i]; }
         __device__ void ld4(long long i, double o[4]) const { ld4d(u + i, o); }
         __device__ void st(long long i, double own, double r) const
@@ -135,12 +152,16 @@ namespace frsdg
 #pragma unroll
                 for (int q = 0; q < 4; ++q) o[q] = b * o[q];
         }
-        __device__ void put4(long long i, const double o[4]) const { st4d(out + i, o); }
+        __device__ void put4(long long i, const double o[4]) const
+        {
+            if (health) health_check(o, health);
+            st4d(out + i, o);
+        }
         __device__ void st4(long long i, const double own[4], const double r[4]) const
         {
             double o[4];
             val4(i, own, r, o);
-            st4d(out + i, o);
+            put4(i, o);
         }
     };
     // forward tangent: out = sigma*v + dR

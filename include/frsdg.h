@@ -28,7 +28,7 @@ extern "C" {
 #define FRSDG_ERR_HIP 2         /* HIP runtime error */
 #define FRSDG_ERR_UNSUPPORTED 3 /* valid but not implemented on this path */
 #define FRSDG_ERR_ALIGN 4       /* pointer not 16-byte aligned */
-#define FRSDG_ERR_NUMERIC 5     /* zero pivot in a factorisation */
+#define FRSDG_ERR_NUMERIC 5     /* zero pivot in a factorisation; failed explicit step */
 
 /* boundary types: values of numPDE::BCType (boundary_conditions.hpp:8-16) plus
  * FRSDG_BC_SLIPWALL, the curvilinear slip wall of tests/bump.cpp:250-272 */
@@ -150,9 +150,20 @@ int frsdg_boundary_weights(frsdg_ctx* ctx, int dir, double* out, void* stream);
  * stages 4 = SSPRK(4,3) (PETSc TSSSP rks3 default), each stage fused with its
  * residual; post-step filter (TSSetPostStep, petsc_solver_impl.hpp:604-605). */
 int frsdg_ssprk_step(frsdg_ctx* ctx, double t, double dt, double* u, int stages, void* stream);
-/* nsteps steps of frsdg_ssprk_step, captured once into a hipGraph and replayed. */
+/* nsteps steps of frsdg_ssprk_step, captured once into a hipGraph (16 steps) and
+ * replayed.  Every stage output is checked where it is stored (a non-finite
+ * value, rho <= 0 or internal energy <= 0 at a node); the run stops after the
+ * first 16-step graph (or the remaining plain steps) with such a state and
+ * returns FRSDG_ERR_NUMERIC (-ts_error_if_step_fails, tools/check_radial.py:208-210),
+ * u holding the last state computed.  Synchronises the stream once per graph. */
 int frsdg_ssprk_run(frsdg_ctx* ctx, double t, double dt, double* u, int stages, int nsteps,
                     void* stream);
+/* The stage-output health flags that frsdg_ssprk_step calls accumulated (bit 1
+ * non-finite, bit 2 rho <= 0, bit 4 internal energy <= 0), cleared when clear
+ * is set; frsdg_ssprk_run clears them when it starts.  Synchronises the stream. */
+int frsdg_step_health(frsdg_ctx* ctx, int* flags, int clear, void* stream);
+/* The same flags for a conservative state u.  Synchronises the stream. */
+int frsdg_check_state(frsdg_ctx* ctx, const double* u, int* flags, void* stream);
 
 /* impose_ic (physics_impl_2D.hpp:156-201, physics_impl_1D.hpp:207-244) for the
  * presets of factory.hpp: kind 0 uniform (rho,u,v,p), 1 isentropic vortex
