@@ -50,9 +50,9 @@ def parse():
     ap.add_argument("--no-newton", action="store_true", help="skip the implicit (JFNK) comparison")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep comparison")
     ap.add_argument("--headline-only", action="store_true", help="only the headline step (profiling runs)")
-    ap.add_argument("--newton-pseudo", action="store_true",
-                    help="add the pseudo-transient tangent vs complex-step comparison with ILU(0) of the "
-                         "coloured matrix (about 2 minutes)")
+    ap.add_argument("--newton-configured", action="store_true",
+                    help="add the configured bump schedule from the free stream (CFL 10, x1.1, Newton rtol 1e-8), "
+                         "two steps per product (about a minute)")
     ap.add_argument("--ilu", action="store_true",
                     help="add the coloured-matrix / ILU(0) preconditioner timings on the bump (about 30 s)")
     ap.add_argument("--cpu-threads", type=int, default=0,
@@ -287,8 +287,8 @@ def extra_legs(torch, fr, args, line, ev, u0):
         leg("newton", lambda: bench_newton(torch, fr))
         if args.ilu:
             leg("ilu", lambda: bench_ilu(torch, fr))
-        if args.newton_pseudo:
-            leg("newton_pseudo", lambda: bench_newton_pseudo(torch, fr, prev_dt_run=True))
+        if args.newton_configured:
+            leg("newton_configured", lambda: bench_newton_configured(torch, fr))
     leg("widened_rows", lambda: bench_rows(torch, fr))
     if not args.no_batch:
         leg("batched", lambda: bench_batch(torch, fr))
@@ -496,69 +496,127 @@ def bench_ilu(torch, fr):
     return out
 
 
-def bench_newton_pseudo(torch, fr, steps=12, prev_dt_run=False):
-    """The bump 256x64 p=5 driven to steady state by pseudo-transient
-    continuation (frsdg_pseudo_run: SER step rule from CFL 1, x1.5 per step,
-    one backward-Euler JFNK solve per step: GMRES(60) to 1e-3 preconditioned
-    by ILU(0) of the coloured finite-difference matrix, as the reference's
-    PETSc setup does, <= 3 Newton iterations to 1e-3).  The tangent kernel
-    linearises the wall and Dirichlet boundary states exactly; the complex
-    step sees them frozen, as the reference's lambda BCs are.  Reported: the
-    steady-residual history of both and the time each takes to reach the
-    reduction the complex step ends with (equal nonlinear-residual
-    reduction).  With prev_dt_run, the tangent also runs with PETSc's default
-    rule (dt_{n+1} = 1.5 dt_n f_{n-1} / f_n, no
-    -ts_pseudo_increment_dt_from_initial_dt; GMRES capped at 300 iterations),
-    in chunks of 4 steps with a progress line each."""
+def newton_schedule(torch, fr, ev, u0, jt, start_cfl, inc, cap_cfl, steps, wall, opts, label):
+    """Pseudo-transient continuation on the bump driven one step at a time
+    (frsdg_pseudo_run with one step of the size the SER rule gives:
+    dt_k = inc * dt_0 * f_0 / f_k, -ts_pseudo_increment_dt_from_initial_dt,
+    capped at cap_cfl), so every step prints a progress line.  Returns the
+    steady-residual history and the summed Newton statistics."""
+    from fr_sdg_amd import configs as CF
+    cfg = ev.cfg
+    h = min(cfg["lx"] / cfg["nx"], cfg["ly"] / cfg["ny"])
+    cfl_dt = h / CF.sound_speed_max(u0, cfg)
+    u = u0.clone()
+    f0 = float(torch.linalg.norm(ev.fun_implicit(u, torch.zeros_like(u))))
+    f, dt0 = f0, start_cfl * cfl_dt
+    hist = {"reduction": [1.0], "seconds": [0.0], "cfl": [], "newton_its": [], "ksp_its": []}
+    tot = {}
+    t0 = time.perf_counter()
+    for k in range(steps):
+        dt = min(inc * dt0 * f0 / f, cap_cfl * cfl_dt)
+        r = ev.pseudo_run(u, dt, 1.0, 1, jac_type=jt, **opts)
+        f = r["fnorm"][0]
+        st = r["stats"]
+        for key in ("newton_its", "ksp_its", "pc_builds", "jv_evals", "t_pc_assemble", "t_pc_factor", "t_pc_apply",
+                    "t_jv", "t_orth", "seconds"):
+            tot[key] = tot.get(key, 0) + st[key]
+        el = time.perf_counter() - t0
+        hist["reduction"].append(f / f0)
+        hist["seconds"].append(el)
+        hist["cfl"].append(dt / cfl_dt)
+        hist["newton_its"].append(st["newton_its"])
+        hist["ksp_its"].append(st["ksp_its"])
+        progress(f"newton {label}: step {k}, CFL {dt / cfl_dt:.3g}, steady residual x{f / f0:.3e}, "
+                 f"{st['newton_its']} Newton / {st['ksp_its']} GMRES iterations, {el:.1f} s")
+        if el > wall or not math.isfinite(f):
+            break
+    # per attempted Newton iteration (one preconditioner build each: a failed line
+    # search still paid for its linear solve)
+    its = max(1, tot.get("pc_builds", 0))
+    per_it = {k: tot[k] / its for k in ("t_pc_assemble", "t_pc_factor", "t_pc_apply", "t_jv", "t_orth", "seconds")}
+    per_it["ksp_its"] = tot.get("ksp_its", 0) / its
+    per_it["other"] = per_it["seconds"] - sum(per_it[k] for k in ("t_pc_assemble", "t_pc_factor", "t_pc_apply",
+                                                                   "t_jv", "t_orth"))
+    kk = max(1, tot.get("ksp_its", 0))
+    per_it["per_gmres_iteration"] = {k: tot[k] / kk for k in ("t_pc_apply", "t_jv", "t_orth")}
+    return {**hist, "f0": f0, "totals": tot, "per_newton_iteration": per_it,
+            "finite": bool(torch.isfinite(u).all())}
+
+
+BUMP_NEWTON_OPTS = dict(precond=3, ksp_pc_side=2, ksp_restart=60, ksp_rtol=1e-3, snes_rtol=1e-3, snes_max_it=3,
+                        snes_stol=1e-8)
+
+
+def bench_newton(torch, fr, steps=8, wall=30.0):
+    """Tangent vs complex-step JFNK on the channel bump 256x64 p=5 (2.36 M
+    DOFs), HLLC, driven toward steady state by pseudo-transient continuation:
+    SER rule from CFL 1 (x1.5 from the initial dt), capped at CFL 1000 (the
+    configured ramp's end); each backward-Euler step solved by Newton (PETSc's
+    bt line search, rtol 1e-3, <= 3 iterations) with flexible GMRES(60) to 1e-3
+    preconditioned by ILU(0) of the coloured finite-difference matrix (the
+    reference's preconditioning matrix and PC).  The tangent kernel linearises
+    the wall and Dirichlet states exactly; the complex step sees them frozen,
+    as the reference's lambda BCs are.  Reported: the steady-residual history
+    of both, the time each needs to reach the reduction the complex step ends
+    with, and where the time of a Newton iteration goes (preconditioner
+    assembly, factorisation and applications, J*v products, Gram-Schmidt).
+    The configured start (CFL 10 from the free stream) is run by
+    --newton-configured: no Newton iteration makes progress there."""
     from fr_sdg_amd import configs as CF
     cfg = CF.bump(tangent_bc_mode=0)
     ev = fr.PhysicsEvaluator(cfg)
     u0 = ev.impose_ic(fr.IC_UNIFORM, CF.bump_freestream())
-    h = min(cfg["lx"] / cfg["nx"], cfg["ly"] / cfg["ny"])
-    dt0 = 1.0 * h / CF.sound_speed_max(u0, cfg)
-    opts = dict(precond=3, ksp_restart=60, ksp_max_it=1000, ksp_rtol=1e-3, snes_rtol=1e-3, snes_max_it=3)
     runs = {}
-    for name, jt in (("tangent", 0), ("complex_step", 1)):
-        u = u0.clone()
-        r = ev.pseudo_run(u, dt0, 1.5, steps, jac_type=jt, **opts)
-        progress(f"pseudo-transient {name}: {r['steps']} steps, {r['seconds'][-1]:.1f} s")
-        red = [f / r["fnorm"][0] for f in r["fnorm"]]
-        runs[name] = {"fnorm": r["fnorm"], "reduction": red, "seconds": r["seconds"],
-                      "cfl": [dt / (dt0 / 1.0) for dt in r["dt"]], "finite": bool(torch.isfinite(u).all())}
-    if prev_dt_run:
-        u = u0.clone()
-        o2 = dict(opts, ksp_max_it=300)
-        dt, fn, sec, f_all, t_off = dt0, [], [], [], 0.0
-        for chunk in range((steps + 3) // 4):
-            r = ev.pseudo_run(u, dt, 1.5, min(4, steps - 4 * chunk), from_initial_dt=False, jac_type=0, **o2)
-            f_all += r["fnorm"]
-            fn += r["dt"]
-            sec += [t_off + x for x in r["seconds"]]
-            t_off = sec[-1]
-            # the rule's next step, as the uninterrupted run would take it
-            prev = f_all[-2] if len(f_all) > 1 else f_all[-1]
-            dt = 1.5 * r["dt"][-1] * prev / f_all[-1]
-            progress(f"pseudo-transient tangent (previous-dt rule): {len(f_all)} steps, reduction "
-                     f"{f_all[-1] / f_all[0]:.2e}")
-        runs["tangent_prev_dt"] = {"fnorm": f_all, "reduction": [f / f_all[0] for f in f_all], "seconds": sec,
-                                   "cfl": [d / dt0 for d in fn], "finite": bool(torch.isfinite(u).all())}
+    for name, jt, kmax in (("tangent", 0, 1000), ("complex_step", 1, 300)):
+        runs[name] = newton_schedule(torch, fr, ev, u0, jt, 1.0, 1.5, 1000.0, steps, wall,
+                                     dict(BUMP_NEWTON_OPTS, ksp_max_it=kmax), name)
     target = min(runs["complex_step"]["reduction"])
 
-    def t_to(run):
+    def t_to(run, tgt):
         for red, t in zip(run["reduction"], run["seconds"]):
-            if red <= target:
+            if red <= tgt:
                 return t
         return None
 
-    tt, tc = t_to(runs["tangent"]), t_to(runs["complex_step"])
+    tt, tc = t_to(runs["tangent"], target), t_to(runs["complex_step"], target)
+    out = {"workload": "bump 256x64 p=5 HLLC (2.36 M DOFs), pseudo-transient continuation (TSPSEUDO SER rule from CFL 1, "
+                       "x1.5 from the initial dt, capped at CFL 1000), Newton rtol 1e-3 (<= 3 its, bt line search), "
+                       "FGMRES(60) rtol 1e-3 with ILU(0) of the coloured FD matrix; tangent with the exact BC "
+                       "linearisation, complex step with the reference's frozen lambda BCs",
+           **runs, "target_reduction": target, "tangent_s_to_target": tt, "complex_step_s_to_target": tc,
+           "speedup_at_equal_reduction": (tc / tt) if (tt and tc) else None}
+    # J*v kernel times on the free-stream state
+    v = torch.sin(torch.arange(ev.ndofs, device="cuda", dtype=torch.float64))
+    o = ev.empty()
+    out["jv_kernel_s"] = {"tangent": time_kernel(torch, lambda: ev.jvp(u0, v, 1.0, out=o), 50),
+                          "complex_step": time_kernel(torch, lambda: ev.jvp_cstep(u0, v, 1.0, 1e-7, out=o), 50)}
+    out["speedup_per_jv"] = out["jv_kernel_s"]["complex_step"] / out["jv_kernel_s"]["tangent"]
+    del ev, u0, v, o
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_newton_configured(torch, fr, steps=2):
+    """The configured schedule as the reference's harness sets it
+    (tools/boundary_conforming_claude.py:64-127: -ts_type pseudo, increment 1.1
+    from the initial dt at CFL 10, here capped at CFL 1000; snes rtol 1e-8,
+    stol 1e-8, max_it 40; ksp rtol 1e-5; ILU(0) of the coloured FD matrix,
+    flexible GMRES(30) bounded at 400 iterations per solve) from the free
+    stream, both products, `steps` steps."""
+    from fr_sdg_amd import configs as CF
+    cfg = CF.bump(tangent_bc_mode=0)
+    ev = fr.PhysicsEvaluator(cfg)
+    u0 = ev.impose_ic(fr.IC_UNIFORM, CF.bump_freestream())
+    opts = dict(precond=3, ksp_pc_side=2, ksp_restart=30, ksp_rtol=1e-5, ksp_max_it=400, snes_rtol=1e-8,
+                snes_stol=1e-8, snes_max_it=40)
+    out = {"workload": "bump 256x64 p=5, configured schedule from the free stream: CFL 10, x1.1 from the initial dt, "
+                       "cap CFL 1000, Newton rtol 1e-8 / stol 1e-8 / 40 its, FGMRES(30) rtol 1e-5 (<= 400 its), "
+                       "ILU(0) of the coloured FD matrix"}
+    for name, jt in (("tangent", 0), ("complex_step", 1)):
+        out[name] = newton_schedule(torch, fr, ev, u0, jt, 10.0, 1.1, 1000.0, steps, 600.0, opts, "configured " + name)
     del ev, u0
     torch.cuda.empty_cache()
-    return {"workload": f"bump 256x64 p=5 HLLC, pseudo-transient continuation (TSPSEUDO SER rule) from CFL 1, "
-                        f"x1.5 per step, {steps} steps, GMRES(60) with ILU(0) of the coloured FD matrix; tangent "
-                        "with the exact BC linearisation, complex step with the reference's frozen lambda BCs",
-            **runs, "target_reduction": target,
-            "tangent_s_to_target": tt, "complex_step_s_to_target": tc,
-            "speedup_at_equal_reduction": (tc / tt) if (tt and tc) else None}
+    return out
 
 
 def bench_rows(torch, fr, reps=50):
@@ -613,42 +671,6 @@ def bench_rows(torch, fr, reps=50):
         torch.cuda.empty_cache()
     out["radial"] = {"workload": "128x128 p=4 quads on [0,10]^2, HLLC (1.64 M DOFs)", "residual_s": rad[0],
                      "residual_radial_s": rad[1], "overhead": rad[1] / rad[0] - 1.0}
-    return out
-
-
-def bench_newton(torch, fr, steps=6):
-    """Tangent vs complex-step JFNK on the channel bump 256x64 p=5 (2.36 M
-    DOFs) driven toward steady state by pseudo-transient continuation from
-    CFL 1 (`steps` steps, the same run as bench_newton_pseudo: GMRES(60) with
-    ILU(0) of the coloured FD matrix, the reference's PETSc pairing), compared
-    at equal steady-residual reduction, plus the J*v kernel time of each
-    product on the free-stream state.  Fixed large steps from free stream are
-    not used: on this problem Newton makes no progress there with either
-    product (the HLLC wave-speed kinks at a uniform state)."""
-    from fr_sdg_amd import configs as CF
-    out = bench_newton_pseudo(torch, fr, steps=steps)
-    cfg = CF.bump(tangent_bc_mode=0)
-    ev = fr.PhysicsEvaluator(cfg)
-    u = ev.impose_ic(fr.IC_UNIFORM, CF.bump_freestream())
-    v = torch.sin(torch.arange(ev.ndofs, device="cuda", dtype=torch.float64))
-    o = ev.empty()
-    s = torch.cuda.current_stream()
-    t = {}
-    for name, jv in (("tangent", lambda: ev.jvp(u, v, 1.0, out=o)), ("complex_step", lambda: ev.jvp_cstep(u, v, 1.0, 1e-7, out=o))):
-        for _ in range(5):
-            jv()
-        torch.cuda.synchronize()
-        a, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(s)
-        for _ in range(50):
-            jv()
-        e.record(s)
-        e.synchronize()
-        t[name] = a.elapsed_time(e) / 50 / 1e3
-    out["jv_kernel_s"] = t
-    out["speedup_per_jv"] = t["complex_step"] / t["tangent"]
-    del ev, u, v, o
-    torch.cuda.empty_cache()
     return out
 
 

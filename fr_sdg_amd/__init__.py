@@ -60,7 +60,8 @@ class NewtonOpts(ctypes.Structure):
     _fields_ = [("jac_type", ctypes.c_int), ("precond", ctypes.c_int), ("snes_rtol", ctypes.c_double),
                 ("snes_atol", ctypes.c_double), ("snes_max_it", ctypes.c_int), ("ksp_restart", ctypes.c_int),
                 ("ksp_max_it", ctypes.c_int), ("ksp_rtol", ctypes.c_double), ("ksp_cgs_refine", ctypes.c_int),
-                ("snes_stol", ctypes.c_double), ("ksp_pc_side", ctypes.c_int), ("pc_lag", ctypes.c_int)]
+                ("snes_stol", ctypes.c_double), ("ksp_pc_side", ctypes.c_int), ("pc_lag", ctypes.c_int),
+                ("pc_fp32", ctypes.c_int)]
 
 
 class NewtonStats(ctypes.Structure):
@@ -86,7 +87,8 @@ class AdjointStats(ctypes.Structure):
 # max_it 10000; preconditioner rebuilt every Newton iteration.  ksp_pc_side stays right (0) here, as the
 # residual-history checks assume; PETSc's GMRES default is left (1).
 NEWTON_DEFAULTS = dict(jac_type=0, precond=0, snes_rtol=1e-8, snes_atol=1e-50, snes_max_it=50, ksp_restart=30,
-                       ksp_max_it=10000, ksp_rtol=1e-5, ksp_cgs_refine=0, snes_stol=1e-8, ksp_pc_side=0, pc_lag=1)
+                       ksp_max_it=10000, ksp_rtol=1e-5, ksp_cgs_refine=0, snes_stol=1e-8, ksp_pc_side=0, pc_lag=1,
+                       pc_fp32=0)
 
 EXPORTS = [
     "frsdg_create", "frsdg_destroy", "frsdg_ndofs", "frsdg_get_tables", "frsdg_residual",

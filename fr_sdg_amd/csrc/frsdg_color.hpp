@@ -69,6 +69,8 @@ namespace frsdg
         int* info = nullptr; // zero-pivot flag
         double* P    = nullptr; // E*S*nd*nd for the Newton preconditioner (allocated on first use)
         double* Dinv = nullptr; // E*2*nd*nd: L_II^-1, U_II^-1 (with P)
+        float*  P32    = nullptr; // single-precision copies of P and Dinv after the factorisation
+        float*  Dinv32 = nullptr; // (Newton option pc_fp32, allocated on first use)
         double* y    = nullptr; // forward-sweep vector
         double* v    = nullptr; // perturbed state / unit colour vector
         double* f0   = nullptr; // F at the base state
@@ -799,28 +801,31 @@ namespace frsdg
     // whole chip on it.  ND = block size as a template parameter (0: run time)
     // so that a thread's loads are all in flight together.
     constexpr int SWR = 16; // rows per workgroup
-    template <int ND>
-    __device__ inline double sweep_dot(int nd, int r, int g, const double* __restrict__ B,
+    template <int ND, class T>
+    __device__ inline double sweep_dot(int nd, int r, int g, const T* __restrict__ B,
                                        const double* __restrict__ v)
     {
         double acc = 0.0;
         if constexpr (ND > 0)
         {
 #pragma unroll
-            for (int c = g; c < ND; c += 16) acc = ::fma(B[r + c * ND], v[c], acc);
+            for (int c = g; c < ND; c += 16) acc = ::fma(static_cast<double>(B[r + c * ND]), v[c], acc);
         }
         else
         {
-            for (int c = g; c < nd; c += 16) acc = ::fma(B[r + static_cast<long long>(c) * nd], v[c], acc);
+            for (int c = g; c < nd; c += 16)
+                acc = ::fma(static_cast<double>(B[r + static_cast<long long>(c) * nd]), v[c], acc);
         }
         return acc;
     }
 
-    template <bool FWD, int ND>
+    // T: the stored factors' type (double; float for the Newton preconditioner's
+    // single-precision copy, accumulated in double)
+    template <bool FWD, int ND, class T = double>
     __global__ __launch_bounds__(256) void k_ilu_sweep(int S, int nd, const int* __restrict__ pos_tab,
                                                        const int* __restrict__ lvl, int off,
-                                                       const double* __restrict__ P,
-                                                       const double* __restrict__ Dinv,
+                                                       const T* __restrict__ P,
+                                                       const T* __restrict__ Dinv,
                                                        const double* __restrict__ in, double* out)
     {
         __shared__ double red[16][SWR + 1];
@@ -864,13 +869,13 @@ namespace frsdg
                     if (qq > CMAXN) break;
                     if (qq < nb)
                     {
-                        const double* B = qq == 0 ? Dinv + I * 2 * bsz + (FWD ? 0 : bsz) : P + (I * S + sl[qq - 1]) * bsz;
+                        const T*      B = qq == 0 ? Dinv + I * 2 * bsz + (FWD ? 0 : bsz) : P + (I * S + sl[qq - 1]) * bsz;
                         const double* V = qq == 0 ? in + I * nd : out + static_cast<long long>(el[qq - 1]) * nd;
 #pragma unroll
                         for (int i = 0; i < NC; ++i)
                         {
                             const int c = g + 16 * i, cc = c < ND ? c : ND - 1;
-                            bv[t][i] = B[rr + cc * ND];
+                            bv[t][i] = static_cast<double>(B[rr + cc * ND]);
                             const double x = V[cc];
                             vv[t][i] = c < ND ? x : 0.0;
                         }

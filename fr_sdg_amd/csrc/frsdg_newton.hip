@@ -26,6 +26,8 @@ struct SolverWs
     long long N   = 0;
     int       m   = 0;       // restart length the basis was sized for
     double*   V   = nullptr; // (m+1) x N Krylov basis
+    double*   Z   = nullptr; // m x N preconditioned basis of flexible GMRES (allocated on first use)
+    int       mZ  = 0;
     double*   w   = nullptr; // work vector
     double*   z   = nullptr; // preconditioned vector / update
     double*   un  = nullptr; // u^n
@@ -58,7 +60,7 @@ namespace frsdg
     {
         if (!c || !c->ws) return;
         SolverWs* s = c->ws;
-        for (double* p : {s->V, s->w, s->z, s->un, s->ud, s->F, s->du, s->ut, s->part, s->red, s->B, s->pv, s->pjv})
+        for (double* p : {s->V, s->Z, s->w, s->z, s->un, s->ud, s->F, s->du, s->ut, s->part, s->red, s->B, s->pv, s->pjv})
             if (p) (void) hipFree(p);
         if (s->info) (void) hipFree(s->info);
         if (s->Bf) (void) hipFree(s->Bf);
@@ -69,6 +71,8 @@ namespace frsdg
             for (int* p : {cp->nbr, cp->cls, cp->low, cp->upd, cp->upp, cp->lvl, cp->lpos, cp->upos, cp->info})
                 if (p) (void) hipFree(p);
             for (double* p : {cp->P, cp->Dinv, cp->y, cp->v, cp->f0, cp->f1, cp->T, cp->t})
+                if (p) (void) hipFree(p);
+            for (float* p : {cp->P32, cp->Dinv32})
                 if (p) (void) hipFree(p);
             delete cp;
         }
@@ -440,18 +444,18 @@ namespace
     }
 
     // z = (L U)^-1 x: forward sweep over the levels, then backward (z may alias x).
-    template <int ND>
-    void ilu_sweeps(const frsdg::ColorPlan& cp, hipStream_t st, const double* P, const double* Dinv, const double* x,
+    template <int ND, class T = double>
+    void ilu_sweeps(const frsdg::ColorPlan& cp, hipStream_t st, const T* P, const T* Dinv, const double* x,
                     double* z)
     {
         const int nlev = static_cast<int>(cp.lvl_ptr.size()) - 1;
         const unsigned rch = static_cast<unsigned>((cp.nd + frsdg::SWR - 1) / frsdg::SWR);
         for (int l = 0; l < nlev; ++l)
-            hipLaunchKernelGGL((frsdg::k_ilu_sweep<true, ND>),
+            hipLaunchKernelGGL((frsdg::k_ilu_sweep<true, ND, T>),
                                dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l]), rch), dim3(256), 0, st, cp.S,
                                cp.nd, cp.lpos, cp.lvl, cp.lvl_ptr[l], P, Dinv, x, cp.y);
         for (int l = nlev - 1; l >= 0; --l)
-            hipLaunchKernelGGL((frsdg::k_ilu_sweep<false, ND>),
+            hipLaunchKernelGGL((frsdg::k_ilu_sweep<false, ND, T>),
                                dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l]), rch), dim3(256), 0, st, cp.S,
                                cp.nd, cp.upos, cp.lvl, cp.lvl_ptr[l], P, Dinv, cp.y, z);
     }
@@ -505,21 +509,22 @@ namespace
         return hip_rc(hipGetLastError());
     }
 
-    int ilu_apply(SolverWs* s, hipStream_t st, const double* P, const double* Dinv, const double* x, double* z)
+    template <class T>
+    int ilu_apply(SolverWs* s, hipStream_t st, const T* P, const T* Dinv, const double* x, double* z)
     {
         const frsdg::ColorPlan& cp = *s->cp;
         switch (cp.nd)
         {
             // 2D element blocks 4 (p+1)^2, 1D 3 (p+1)
-            case 16: ilu_sweeps<16>(cp, st, P, Dinv, x, z); break;
-            case 36: ilu_sweeps<36>(cp, st, P, Dinv, x, z); break;
-            case 64: ilu_sweeps<64>(cp, st, P, Dinv, x, z); break;
-            case 100: ilu_sweeps<100>(cp, st, P, Dinv, x, z); break;
-            case 144: ilu_sweeps<144>(cp, st, P, Dinv, x, z); break;
-            case 196: ilu_sweeps<196>(cp, st, P, Dinv, x, z); break;
-            case 256: ilu_sweeps<256>(cp, st, P, Dinv, x, z); break;
-            case 324: ilu_sweeps<324>(cp, st, P, Dinv, x, z); break;
-            default: ilu_sweeps<0>(cp, st, P, Dinv, x, z); break;
+            case 16: ilu_sweeps<16, T>(cp, st, P, Dinv, x, z); break;
+            case 36: ilu_sweeps<36, T>(cp, st, P, Dinv, x, z); break;
+            case 64: ilu_sweeps<64, T>(cp, st, P, Dinv, x, z); break;
+            case 100: ilu_sweeps<100, T>(cp, st, P, Dinv, x, z); break;
+            case 144: ilu_sweeps<144, T>(cp, st, P, Dinv, x, z); break;
+            case 196: ilu_sweeps<196, T>(cp, st, P, Dinv, x, z); break;
+            case 256: ilu_sweeps<256, T>(cp, st, P, Dinv, x, z); break;
+            case 324: ilu_sweeps<324, T>(cp, st, P, Dinv, x, z); break;
+            default: ilu_sweeps<0, T>(cp, st, P, Dinv, x, z); break;
         }
         return hip_rc(hipGetLastError());
     }
@@ -555,8 +560,14 @@ namespace
         int prec(const double* x, double* z)
         {
             if (o->precond >= 3)
-                return transpose ? ilu_apply_T(s, st, s->cp->P, s->cp->Dinv, x, z)
-                                 : ilu_apply(s, st, s->cp->P, s->cp->Dinv, x, z);
+            {
+                if (transpose) return ilu_apply_T(s, st, s->cp->P, s->cp->Dinv, x, z);
+                if (o->pc_fp32 && s->cp->P32)
+                    return ilu_apply(s, st, static_cast<const float*>(s->cp->P32),
+                                     static_cast<const float*>(s->cp->Dinv32), x, z);
+                return ilu_apply(s, st, static_cast<const double*>(s->cp->P), static_cast<const double*>(s->cp->Dinv),
+                                 x, z);
+            }
             if (o->precond == 2)
             {
                 // red-black element block Gauss-Seidel, one forward sweep, matrix-free:
@@ -667,6 +678,17 @@ namespace
             if ((rc = color_assemble(method, evalF, s->cp->P))) return rc;
             mark(1);
             if ((rc = ilu_factor(s, st, s->cp->P, s->cp->Dinv))) return rc;
+            if (o->pc_fp32 && !transpose)
+            {
+                // single-precision copies of the factors for the applications
+                const long long nP = cp->E * cp->S * static_cast<long long>(cp->nd) * cp->nd;
+                const long long nD = cp->E * 2 * static_cast<long long>(cp->nd) * cp->nd;
+                if (!cp->P32) FR_TRY(hipMalloc(&cp->P32, sizeof(float) * nP));
+                if (!cp->Dinv32) FR_TRY(hipMalloc(&cp->Dinv32, sizeof(float) * nD));
+                hipLaunchKernelGGL(k_to_float, dim3(grid(nP)), dim3(256), 0, st, nP, cp->P, cp->P32);
+                hipLaunchKernelGGL(k_to_float, dim3(grid(nD)), dim3(256), 0, st, nD, cp->Dinv, cp->Dinv32);
+                FR_TRY(hipGetLastError());
+            }
             mark(2);
             FR_TRY(hipEventSynchronize(s->ev[2]));
             stats->t_pc_assemble += between(0, 1);
@@ -719,8 +741,20 @@ namespace
             double beta0 = -1.0;
             its = 0;
             // PETSc's GMRES default is left preconditioning with the preconditioned
-            // residual norm (-ksp_pc_side left); right keeps the true residual norm
+            // residual norm (-ksp_pc_side left); right keeps the true residual norm;
+            // flexible (FGMRES) keeps the preconditioned basis Z_j = M^-1 v_j and
+            // forms the solution from it, so the residual estimate stays the true
+            // residual even where M^-1 is applied with large rounding growth
             const bool left = o->ksp_pc_side == 1 && o->precond != 0;
+            const bool flex = o->ksp_pc_side == 2 && o->precond != 0;
+            if (flex && (!s->Z || s->mZ < m))
+            {
+                if (s->Z) FR_TRY(hipFree(s->Z));
+                s->Z  = nullptr;
+                s->mZ = 0;
+                FR_TRY(hipMalloc(&s->Z, sizeof(double) * m * N));
+                s->mZ = m;
+            }
             // r = b - J x  (x = 0 on the first cycle)
             bool first = true;
             while (true)
@@ -742,6 +776,13 @@ namespace
                 }
                 else if (first)
                     FR_TRY(hipMemcpyAsync(r, b, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
+                else if (flex)
+                {
+                    // x is the solution itself
+                    int rc = jv(x, s->w);
+                    if (rc) return rc;
+                    hipLaunchKernelGGL(k_waxpby, dim3(grid(N)), dim3(256), 0, st, N, 1.0, b, -1.0, s->w, r);
+                }
                 else
                 {
                     int rc = prec(x, s->z);
@@ -776,9 +817,10 @@ namespace
                     }
                     else
                     {
-                        if ((rc = prec(s->V + j * N, s->z))) return rc;
+                        double* zj = flex ? s->Z + j * N : s->z;
+                        if ((rc = prec(s->V + j * N, zj))) return rc;
                         mark(1);
-                        if ((rc = jv(s->z, s->w))) return rc;
+                        if ((rc = jv(zj, s->w))) return rc;
                     }
                     mark(2);
                     // classical Gram-Schmidt: h = V^T w; w -= V h; ||w|| (optionally a second pass)
@@ -837,12 +879,12 @@ namespace
                     y[i] = H[i * m + i] != 0.0 ? sum / H[i * m + i] : 0.0;
                 }
                 FR_TRY(hipMemcpyAsync(s->red, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, st));
-                hipLaunchKernelGGL(k_mcomb, dim3(grid(N)), dim3(256), 0, st, N, s->V, j, s->red, s->z);
+                hipLaunchKernelGGL(k_mcomb, dim3(grid(N)), dim3(256), 0, st, N, flex ? s->Z : s->V, j, s->red, s->z);
                 hipLaunchKernelGGL(k_axpby, dim3(grid(N)), dim3(256), 0, st, N, 1.0, s->z, 1.0, x);
                 FR_TRY(hipStreamSynchronize(st));
                 if (done || its >= o->ksp_max_it) break;
             }
-            if (left) return FRSDG_OK;
+            if (left || flex) return FRSDG_OK;
             // solution = M^-1 x
             int rc = prec(x, s->z);
             if (rc) return rc;
@@ -927,7 +969,8 @@ extern "C" int frsdg_beuler_step(frsdg_ctx* c, double t, double dt, double* u, c
         // (-snes_lag_preconditioner k -snes_lag_preconditioner_persists)
         if (o->precond >= 1)
         {
-            const bool rebuild = s->pc_built != o->precond || o->pc_lag <= 1 || s->pc_age >= o->pc_lag;
+            const int  kind    = o->precond + (o->pc_fp32 ? 16 : 0); // the single-precision copies are part of it
+            const bool rebuild = s->pc_built != kind || o->pc_lag <= 1 || s->pc_age >= o->pc_lag;
             if (rebuild)
             {
                 s->pc_built = -1;
@@ -936,7 +979,7 @@ extern "C" int frsdg_beuler_step(frsdg_ctx* c, double t, double dt, double* u, c
                 else
                     rc = K.build_block_jacobi();
                 if (rc) return rc;
-                s->pc_built = o->precond;
+                s->pc_built = kind;
                 s->pc_age   = 0;
                 S->pc_builds++;
             }

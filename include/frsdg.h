@@ -220,12 +220,19 @@ typedef struct frsdg_newton_opts
                            (SNES_CONVERGED_SNORM_RELATIVE; PETSc default 1e-8, 0 off) */
     int    ksp_pc_side; /* 0 right preconditioning, test on the true residual norm;
                            1 left, test on the preconditioned residual norm (PETSc's
-                           GMRES default, -ksp_pc_side left) */
+                           GMRES default, -ksp_pc_side left); 2 flexible right
+                           preconditioning (-ksp_type fgmres): the solution is formed
+                           from the stored preconditioned basis, so its true residual
+                           is the estimate even when the preconditioner's apply carries
+                           large rounding (ILU(0) of a badly conditioned matrix) */
     int    pc_lag;      /* rebuild the preconditioner every pc_lag-th Newton iteration,
                            counted across consecutive frsdg_beuler_step calls and the
                            steps of one frsdg_pseudo_run (-snes_lag_preconditioner k
                            -snes_lag_preconditioner_persists); <= 1 every iteration
                            (PETSc default) */
+    int    pc_fp32;     /* preconditioners 3/4: apply single-precision copies of the ILU(0)
+                           factors (accumulation in double); halves the bytes each
+                           application reads.  The transposed apply stays double. */
 } frsdg_newton_opts;
 
 typedef struct frsdg_newton_stats

@@ -217,8 +217,9 @@ def test_snes_step_norm_test_stops_first():
 @pytest.mark.parametrize("precond", [1, 3])
 def test_left_preconditioned_gmres_solves_the_same_system(precond):
     """ksp_pc_side 1 (PETSc's GMRES default: left preconditioning, test on the
-    preconditioned residual) solves the same linear system as the right-
-    preconditioned solver; its history starts at ||M^-1 b||."""
+    preconditioned residual) and 2 (flexible GMRES) solve the same linear
+    system as the right-preconditioned solver; the left history starts at
+    ||M^-1 b||, and the flexible solution's true residual is its estimate."""
     import torch
 
     import fr_sdg_amd as fr
@@ -231,14 +232,18 @@ def test_left_preconditioned_gmres_solves_the_same_system(precond):
     h = min(cfg["lx"] / cfg["nx"], cfg["ly"] / cfg["ny"])
     sigma = CF.sound_speed_max(u, cfg) / (4.0 * h)
     xs = {}
-    for side in (0, 1):
+    for side in (0, 2, 1):
         x, st, hist = ev.linear_solve(u, b, sigma, precond=precond, ksp_pc_side=side, ksp_rtol=1e-12,
                                       ksp_restart=200, ksp_max_it=2000)
-        print("[pc_side]", precond, side, st["ksp_its"], hist[0], hist[-1], flush=True)
+        print("[pc_side]", precond, side, st["ksp_its"], hist[0], hist[-1], st["fnorm"] / st["fnorm0"], flush=True)
         assert st["ksp_its"] < 2000
         xs[side] = x
-    # both solve (sigma I + J) x = b
+        if side == 2:
+            # flexible GMRES: the returned solution's true residual is the last estimate
+            assert st["fnorm"] == pytest.approx(hist[-1], rel=1e-3, abs=1e-13 * hist[0])
+    # all solve (sigma I + J) x = b
     assert float(torch.linalg.norm(xs[0] - xs[1]) / torch.linalg.norm(xs[0])) <= 1e-8
+    assert float(torch.linalg.norm(xs[0] - xs[2]) / torch.linalg.norm(xs[0])) <= 1e-8
     assert hist[0] != pytest.approx(float(torch.linalg.norm(b)), rel=1e-3)  # left: ||M^-1 b||
 
 
@@ -264,3 +269,29 @@ def test_lagged_preconditioner_rebuilds_every_k_iterations():
     assert sb["pc_builds"] == -(-sb["newton_its"] // 3)  # ceil(its / 3)
     assert float(torch.linalg.norm(a - b) / torch.linalg.norm(a)) <= 1e-7
     assert sa["t_pc_factor"] > 0.0 and sa["t_jv"] > 0.0 and sa["t_orth"] > 0.0 and sa["t_pc_apply"] > 0.0
+
+
+def test_single_precision_ilu_factors_precondition_the_same_solve():
+    """pc_fp32: the ILU(0) factors applied in single precision (double
+    accumulation) still precondition flexible GMRES to the same solution; the
+    Newton step converges to the same state."""
+    import torch
+
+    import fr_sdg_amd as fr
+    from fr_sdg_amd import configs as CF
+    cfg = CF.bump(8, 4, 3)
+    ev = fr.PhysicsEvaluator(cfg)
+    u0 = ev.impose_ic(fr.IC_UNIFORM, CF.bump_freestream())
+    u0.mul_(1.0 + 0.01 * torch.sin(0.37 * torch.arange(u0.numel(), device="cuda", dtype=torch.float64)))
+    h = min(cfg["lx"] / cfg["nx"], cfg["ly"] / cfg["ny"])
+    dt = 3.0 * h / CF.sound_speed_max(u0, cfg)
+    out, sts = {}, {}
+    for fp32 in (0, 1):
+        u = u0.clone()
+        sts[fp32] = ev.beuler_step(u, dt, jac_type=0, precond=3, ksp_pc_side=2, pc_fp32=fp32, ksp_restart=60,
+                                   ksp_rtol=1e-10, snes_rtol=1e-10, snes_max_it=10)
+        out[fp32] = u
+    print("[fp32 ilu]", sts, flush=True)
+    assert sts[0]["converged"] and sts[1]["converged"]
+    assert sts[1]["ksp_its"] <= 2 * sts[0]["ksp_its"] + 5
+    assert float(torch.linalg.norm(out[0] - out[1]) / torch.linalg.norm(out[0])) <= 1e-9
