@@ -418,6 +418,23 @@ namespace
             }
         };
     };
+    // 2D forward tangent (k_tan2d: the residual's tiles, forward-mode pairs
+    // only in the node and face fluxes)
+    template <int N>
+    struct Tan2
+    {
+        static int run(const frsdg_ctx* c, const IOTan& io, hipStream_t s)
+        {
+            using T            = Tile2D<N, double>;
+            const long long nb = tile_count(c->p2.nx, c->p2.ny, T::TX, T::TY);
+            if (c->p2.curved)
+                hipLaunchKernelGGL((k_tan2d<N, true>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, c->p2, io);
+            else
+                hipLaunchKernelGGL((k_tan2d<N, false>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, c->p2, io);
+            return hip_rc(hipGetLastError());
+        }
+    };
+
     template <class IO>
     int launch_res(const frsdg_ctx* c, const IO& io, hipStream_t s)
     {
@@ -959,7 +976,8 @@ int frsdg_jvp(frsdg_ctx* c, double, const double* u, const double* v, double sig
         v = c->w2;
     }
     IOTan io{u, v, out, sigma};
-    return launch_res(c, io, s);
+    if (c->d.dims == 2) return by_n<Tan2>(c->n, c, io, s);
+    return by_n<Launch<IOTan>::template R1>(c->n, c, io, s);
 }
 
 int frsdg_jvp_cstep(frsdg_ctx* c, double, const double* u, const double* v, double sigma, double h,

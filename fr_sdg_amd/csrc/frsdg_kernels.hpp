@@ -28,8 +28,9 @@
 //     precomputed once in HBM;
 //   * block index -> tile is remapped so that each XCD walks a contiguous band
 //     of tiles (neighbour traces hit its own L2).
-// The same tile code is instantiated on double (residual, stages), Dual
-// (forward tangent J*v) and Cplx (complex step), see frsdg_scalar.hpp.
+// The same tile code is instantiated on double (residual, stages) and Cplx
+// (complex step, frsdg_scalar.hpp).  The forward tangent J*v (k_tan2d) runs
+// the same tiles with forward-mode pairs only in the node and face fluxes.
 #pragma once
 #include <type_traits>
 #include "frsdg_geometry.hpp"
@@ -226,11 +227,6 @@ namespace frsdg
     struct BlockSize
     {
         static constexpr int value = 256;
-    };
-    template <>
-    struct BlockSize<Dual>
-    {
-        static constexpr int value = 128;
     };
     template <>
     struct BlockSize<Cplx>
@@ -588,8 +584,8 @@ namespace frsdg
     // The line is held in registers; D comes through the scalar cache, the
     // metric pair (ma, mb) of the line's nodes from registers.  Affine meshes
     // keep only d/dxi F and d/deta G (the cross terms are exact zeros).
-    template <int N, class S, bool CURVED>
-    __device__ __forceinline__ void volume_line(const Res2DP& P, TileSmem<N, S, CURVED>& sm, bool eta, int le,
+    template <int N, class S, bool CURVED, class SM>
+    __device__ __forceinline__ void volume_line(const Res2DP& P, SM& sm, bool eta, int le,
                                                 int ln, int q, const double* ma = nullptr, const double* mb = nullptr)
     {
         const double* D = P.tab;
@@ -879,6 +875,222 @@ namespace frsdg
         // affine FILT kernels always filter (the unfiltered store is dead code there)
         const bool f = CURVED ? filt != 0 : FILT;
         res2d_tile<N, IOStage, CURVED, FILT>(P, io, xcd_tile(blockIdx.x, gridDim.x), T::TX, T::TY, f, sm);
+    }
+
+    // ------------------------------------------------------------------
+    // 2D forward tangent out = sigma v + R'(u) v, the directional derivative of
+    // the residual above (the reference forms J v by the complex step,
+    // petsc_solver_impl.hpp:289-356).  The residual is linear in the fluxes
+    // except at the node fluxes and the face Riemann fluxes, so only those run
+    // in forward-mode pairs (value, derivative); everything after them, the
+    // derivative-matrix contraction, the lifts and the output, is the
+    // residual's own linear code on the derivative part alone.  LDS: U primal
+    // state, V tangent state, F / G the tangent fluxes F'(u) v, G'(u) v; the
+    // same 256-thread tiles, face tasks and line tasks as the residual.
+    // ------------------------------------------------------------------
+    template <int N, bool CURVED>
+    __device__ __forceinline__ void face_flux_tan(const Res2DP& P, FaceTask<Dual>& f, TileSmem<N, double, true>& sm)
+    {
+        const int kA = f.xdir ? N - 1 : f.idx, lA = f.xdir ? f.idx : N - 1;
+        const int kB = f.xdir ? 0 : f.idx, lB = f.xdir ? f.idx : 0;
+        Dual      UA[4], UB[4], Fs[4];
+        if (f.elA >= 0)
+        {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                UA[q] = {sm.U[f.elA][lA][sm.pos(lA, kA, q)], sm.V[f.elA][lA][sm.pos(lA, kA, q)]};
+        }
+        if (f.elB >= 0)
+        {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                UB[q] = {sm.U[f.elB][lB][sm.pos(lB, kB, q)], sm.V[f.elB][lB][sm.pos(lB, kB, q)]};
+        }
+        if (f.elA < 0)
+        {
+            if (f.bdir >= 0)
+                bc_exterior(P.bc, f.bdir, UB, f.fnx, f.fny, UA);
+            else
+            {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) UA[q] = f.Uext[q];
+            }
+        }
+        if (f.elB < 0)
+        {
+            if (f.bdir >= 0)
+                bc_exterior(P.bc, f.bdir, UA, f.fnx, f.fny, UB);
+            else
+            {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) UB[q] = f.Uext[q];
+            }
+        }
+        riemann2(P.gas, P.riemann, UA, UB, f.fnx, f.fny, Fs);
+        const int fA = f.xdir ? 1 : 3, fB = f.xdir ? 0 : 2;
+        // own-side normal flux tangent: curved tiles read F'v, G'v from LDS;
+        // affine tiles find it in their own Del slot, put there by the trace
+        // node (the line tasks overwrite F, G in place meanwhile)
+        if (f.elA >= 0)
+        {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                double fn;
+                if constexpr (CURVED)
+                    fn = sm.F[f.elA][lA][sm.pos(lA, kA, q)] * f.fnx + sm.G[f.elA][lA][sm.pos(lA, kA, q)] * f.fny;
+                else
+                    fn = sm.Del[f.elA][fA][f.idx][q];
+                sm.Del[f.elA][fA][f.idx][q] = f.sc * (Fs[q].d - fn);
+            }
+        }
+        if (f.elB >= 0)
+        {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                double fn;
+                if constexpr (CURVED)
+                    fn = sm.F[f.elB][lB][sm.pos(lB, kB, q)] * f.fnx + sm.G[f.elB][lB][sm.pos(lB, kB, q)] * f.fny;
+                else
+                    fn = sm.Del[f.elB][fB][f.idx][q];
+                sm.Del[f.elB][fB][f.idx][q] = f.sc * (Fs[q].d - fn);
+            }
+        }
+    }
+
+    template <int N, bool CURVED>
+    __device__ __forceinline__ void tan2d_tile(const Res2DP& P, const IOTan& io, long long blk, int tx, int ty,
+                                               TileSmem<N, double, true>& sm)
+    {
+        constexpr int BS  = 256;
+        constexpr int NN  = N * N;
+        const int     tid = threadIdx.x;
+        if (tid < N)
+        {
+            sm.dL[tid] = P.tab[NN + tid];
+            sm.dR[tid] = P.tab[NN + N + tid];
+        }
+        const TileGeom g    = tile_geom(P, blk, tx, ty);
+        const int      el   = tid / NN;
+        const int      node = tid - el * NN;
+        const int      k    = node % N;
+        const int      l    = node / N;
+        const int      eb   = el / tx;
+        const int      ea   = el - eb * tx;
+        const bool     act  = (el < tx * ty) && (g.i0 + ea < P.nx) && (g.j0 + eb < P.ny);
+        const long long e   = static_cast<long long>(g.j0 + eb) * P.nx + (g.i0 + ea);
+        if (node == 0 && el < tx * ty) sm.eid[el] = act ? e : -1;
+
+        FaceTask<Dual> f0;
+        face_task<N, IOTan>(P, io, g, tid, f0);
+
+        // (unlike the residual, the curved line metrics are not prefetched
+        // here: held through the face phase they cost a wave per SIMD, n = 8:
+        // 184 -> 150 VGPRs, 2 -> 3 waves, 1024^2 p = 7 tangent 3.56 -> 2.71 ms)
+        const int nlt = tx * ty * N * 4;
+        double    vt[4]; // own tangent state
+        if (act)
+        {
+            Dual own[4], F[4], G[4];
+            io.ld4((e * NN + node) * 4, own);
+            flux2(P.gas, own, F, G);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                vt[q]                        = own[q].d;
+                sm.U[el][l][sm.pos(l, k, q)] = own[q].v;
+                sm.V[el][l][sm.pos(l, k, q)] = own[q].d;
+                sm.F[el][l][sm.pos(l, k, q)] = F[q].d;
+                sm.G[el][l][sm.pos(l, k, q)] = G[q].d;
+            }
+            if constexpr (!CURVED)
+            {
+                // affine faces have normals (1, 0) and (0, 1): the trace nodes
+                // leave F'v (left, right) or G'v (bottom, top) in the Del slot
+                // that the face task replaces with the scaled jump
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                {
+                    if (k == 0) sm.Del[el][0][l][q] = F[q].d;
+                    if (k == N - 1) sm.Del[el][1][l][q] = F[q].d;
+                    if (l == 0) sm.Del[el][2][k][q] = G[q].d;
+                    if (l == N - 1) sm.Del[el][3][k][q] = G[q].d;
+                }
+            }
+        }
+        __syncthreads();
+
+        if (f0.live) face_flux_tan<N, CURVED>(P, f0, sm);
+        const int nft = ((tx + 1) * ty + tx * (ty + 1)) * N;
+        for (int t = tid + BS; t < nft; t += BS)
+        {
+            FaceTask<Dual> f;
+            face_task<N, IOTan>(P, io, g, t, f);
+            if (f.live) face_flux_tan<N, CURVED>(P, f, sm);
+        }
+        // curved line tasks write over U and V, which face tasks read
+        if constexpr (CURVED) __syncthreads();
+
+        {
+            const int nt = CURVED ? 2 * nlt : nlt;
+            for (int t = tid; t < nt; t += BS)
+            {
+                const bool eta = t >= nlt;
+                const int  w   = eta ? t - nlt : t;
+                const int  q   = w & 3;
+                const int  le  = (w >> 2) / N;
+                const int  ln  = (w >> 2) - le * N;
+                if (sm.eid[le] < 0) continue;
+                if constexpr (CURVED)
+                {
+                    double ma[N], mb[N];
+                    line_metrics<N>(P.vmet, sm.eid[le], eta, ln, ma, mb);
+                    volume_line<N, double, true>(P, sm, eta, le, ln, q, ma, mb);
+                }
+                else
+                {
+                    volume_line<N, double, false>(P, sm, false, le, ln, q);
+                    volume_line<N, double, false>(P, sm, true, le, ln, q);
+                }
+            }
+        }
+        __syncthreads();
+
+        if (!act) return;
+        double o[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+            double r;
+            if constexpr (CURVED)
+                r = sm.U[el][l][sm.pos(l, k, q)] + sm.V[el][l][sm.pos(l, k, q)];
+            else
+                r = sm.F[el][l][sm.pos(l, k, q)] + sm.G[el][l][sm.pos(l, k, q)];
+            r    = r + sm.Del[el][0][l][q] * sm.dL[k];
+            r    = r + sm.Del[el][1][l][q] * sm.dR[k];
+            r    = r + sm.Del[el][2][k][q] * sm.dL[l];
+            r    = r + sm.Del[el][3][k][q] * sm.dR[l];
+            o[q] = r;
+        }
+        if (P.rinv)
+        {
+            Dual own[4], src[4];
+            io.ld4((e * NN + node) * 4, own);
+            radial_source2(P.gas, own, P.rinv[e * NN + node], src);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) o[q] = o[q] + src[q].d;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) o[q] = io.sigma * vt[q] + o[q];
+        st4d(io.out + (e * NN + node) * 4, o);
+    }
+    template <int N, bool CURVED>
+    __global__ __launch_bounds__(256) void k_tan2d(const Res2DP P, const IOTan io)
+    {
+        using T = Tile2D<N, double>;
+        __shared__ TileSmem<N, double, true> sm;
+        tan2d_tile<N, CURVED>(P, io, xcd_tile(blockIdx.x, gridDim.x), T::TX, T::TY, sm);
     }
 
     // ------------------------------------------------------------------
