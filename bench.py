@@ -754,6 +754,7 @@ def adjoint_large(torch, fr, args):
 
     adj = A.UnsteadyAdjoint(ev, dt, T, max_snapshots=args.adj_snapshots)
     uT, t_fwd = timed(lambda: adj.run(u0))
+    health = ev.step_health()
     J0, g = A.rho_squared(ev, uT, m)
     print(f"[adjoint-large] forward {t_fwd:.1f} s J={J0:.15e}", file=sys.stderr, flush=True)
     adj.forward_steps = 0
@@ -783,7 +784,8 @@ def adjoint_large(torch, fr, args):
     return {"workload": f"warped {args.adj_n}^2 p=7 VdW HLLC+filter, SSP-RK3 x {T}, J = int rho^2",
             "ndofs": ev.ndofs, "dt": dt, "snapshots": args.adj_snapshots,
             "repetition_number": A.repetitions(T, args.adj_snapshots),
-            "forward_s": t_fwd, "adjoint_s": t_adj, "recomputed_forward_steps": adj.forward_steps,
+            "forward_s": t_fwd, "adjoint_s": t_adj, "adjoint_over_forward": t_adj / t_fwd,
+            "forward_health_flags": health, "recomputed_forward_steps": adj.forward_steps,
             "peak_snapshots": adj.peak_snapshots, "J": J0, "dJ_v_adjoint": ad, "dJ_v_central_fd": fd,
             "rel_diff": abs(ad - fd) / abs(fd), "fd_eps": eps, "fd_seconds": t_fd}
 

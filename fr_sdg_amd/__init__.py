@@ -101,7 +101,7 @@ EXPORTS = [
     "frsdg_filter_transpose", "frsdg_mass", "frsdg_boundary_weights", "frsdg_adjoint_solve", "frsdg_pseudo_run",
     "frsdg_color_jacobian", "frsdg_ilu0_factor", "frsdg_ilu0_apply", "frsdg_linear_solve",
     "frsdg_ilu0_apply_transpose", "frsdg_l2_error_noalloc", "frsdg_unsteady_adjoint", "frsdg_step_health",
-    "frsdg_check_state",
+    "frsdg_check_state", "frsdg_set_stop_mask",
 ]
 
 # health flags of stored states (frsdg_step_health / frsdg_check_state)
@@ -141,6 +141,7 @@ def load_library(path=LIB_PATH):
                                  ctypes.POINTER(ctypes.c_double), vp]
     L.frsdg_step_health.argtypes = [vp, ctypes.POINTER(ctypes.c_int), i, vp]
     L.frsdg_check_state.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int), vp]
+    L.frsdg_set_stop_mask.argtypes = [vp, i]
     L.frsdg_unsteady_adjoint.argtypes = [vp, d, d, i, i, dp, dp, dp, ctypes.POINTER(AdjointStats), vp]
     L.frsdg_l2_error_noalloc.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double), d, dp,
                                          ctypes.POINTER(ctypes.c_double), vp]
@@ -341,6 +342,11 @@ class PhysicsEvaluator:
         f = ctypes.c_int(0)
         _check(self.lib.frsdg_step_health(self.h, ctypes.byref(f), int(clear), _stream()))
         return f.value
+
+    def set_stop_mask(self, mask):
+        """HEALTH_* bits that stop ssprk_run with a numerical failure (default
+        HEALTH_NONFINITE)."""
+        _check(self.lib.frsdg_set_stop_mask(self.h, int(mask)))
 
     def check_state(self, u):
         """Health flags of a conservative state (HEALTH_* bits; 0 = healthy)."""

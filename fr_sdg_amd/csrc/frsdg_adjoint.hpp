@@ -2,14 +2,14 @@
 //
 // R of element e is   V_e(u_e) + sum over its 4 face copies of  lift_f (Delta_f)
 // with Delta_f = s_f (F*(U_L, U_R; n_f) - F(U_own).n_f).  Transposing:
-//   phase 1 (k_adj_lift): per element and face copy, the lifted cotangent
+//   k_adj_lift: per element and face copy, the lifted cotangent
 //     Wbar_{e,f}(q, idx) = sum_k W_e(q, line(idx, k)) g'(k)      (g' = dL or dR);
-//   phase 2 (k_adj2d): per element,
-//     volume: Fbar(q,m,l') = sum_k D[k][m] (xi_x W)(q,k,l') + sum_l D[l][l'] (eta_x W)(q,m,l),
+//   k_adj2d_tile: per tile of elements (the residual's tiles),
+//     volume: Fbar(q,k,l) = sum_m D[m][k] (xi_x W)(q,m,l) + sum_m D[m][l] (eta_x W)(q,k,m),
 //             Gbar likewise with xi_y, eta_y;  ubar += J_F^T Fbar + J_G^T Gbar per node;
-//     faces : ubar_own += s_f J_{F*,own}^T (Wbar_{e,f} + Wbar_{e',f'}) - s_f J_{Fn,own}^T Wbar_{e,f},
-//             where (e',f') is the neighbour's copy of the same face (its Wbar is read
-//             from the phase-1 buffer) and boundary faces chain through the BC map.
+//     faces : each face once: ubar_A, ubar_B += J_{F*}^T s (Wbar_A + Wbar_B) for both
+//             traces, minus s J_{Fn,own}^T Wbar_own per side; boundary faces chain
+//             through the BC map.
 // The small Jacobian-transpose products are hand-written reverse sweeps
 // (frsdg_reverse.hpp) of the flux / Riemann / BC code the residual uses, with
 // the same branch decisions, so the adjoint linearises exactly the discrete
@@ -38,15 +38,7 @@ namespace frsdg
         }
     };
 
-    constexpr int ADJ_BS = 128;
-    template <int N>
-    struct AdjTile
-    {
-        static constexpr int NN  = N * N;
-        static constexpr int EPB = (ADJ_BS / NN) > 0 ? (ADJ_BS / NN) : 1;
-    };
-
-    // phase 1: Wbar[e][f][idx][q]
+    // lifted cotangents of the face copies: Wbar[e][f][idx][q]
     template <int N>
     __global__ __launch_bounds__(256) void k_adj_lift(long long E, const double* __restrict__ tab,
                                                        const double* __restrict__ w,
@@ -71,229 +63,470 @@ namespace frsdg
         for (int q = 0; q < 4; ++q) wb[t * 4 + q] = acc[q];
     }
 
-    // J^T c for c attached to F(U).nx + G(U).ny (Fn) and to the Riemann flux w.r.t. the own state
-    struct FaceAdjIn
+    // ------------------------------------------------------------------
+    // Tile adjoint (2D): the residual's tiles (256 threads, TX x TY elements,
+    // one thread per node), each face of a tile evaluated once.
+    //   load     : u, w -> LDS;
+    //   phase 1  : xi-line tasks  Fbar(l, k) = sum_m D[m][k] xi_x W(l, m),
+    //              Gbar likewise with xi_y (affine: eta-line tasks Gbar from
+    //              eta_y at the same time), and face tasks: one reverse Riemann
+    //              sweep per face node with cs = s (Wbar_A + Wbar_B) gives the
+    //              cotangents of both traces (rim faces keep their own side,
+    //              the neighbouring tile evaluates the other), plus the own
+    //              normal-flux term -s Wbar through flux2_vjp; results -> Cs;
+    //   phase 2  : (curved) eta-line tasks add sum_m D[m][l] (eta_x, eta_y) W(m, k);
+    //   per node : ubar = (dF/dU)^T Fbar + (dG/dU)^T Gbar + the face slots,
+    //              out = AdjOut(w, ubar).
+    // Wbar (the lifted cotangents of every element's face copies) comes from
+    // k_adj_lift; periodic wraps whose two copies carry different metrics
+    // evaluate the outer copy's Riemann flux a second time with its metrics.
+    // ------------------------------------------------------------------
+    template <int N>
+    struct AdjSmem
     {
-        double uo[4], un[4]; // own / neighbour states (neighbour unused at boundaries)
-        double nx, ny;
-        int    own_right;    // 1 if the own state is U_R (left/bottom faces), 0 if U_L
-        int    bnd, dir;     // boundary face and its BC direction
+        static constexpr int EPB = Tile2D<N, double>::EPB;
+        static constexpr int RB  = N * 4 * 8;
+        static constexpr int RP  = N * 4 + ((RB % 64) == 0 ? 4 : 0);
+        __device__ static __forceinline__ int pos(int l, int k, int q) { return 4 * k + (q ^ (l & 3)); }
+        double    U[EPB][N][RP];    // state
+        double    W[EPB][N][RP];    // cotangent
+        double    FX[EPB][N][RP];   // Fbar
+        double    GX[EPB][N][RP];   // Gbar
+        double    Cs[EPB][4][N][4]; // face cotangents of the own trace (faces 0 left, 1 right, 2 bottom, 3 top)
+        long long eid[EPB];
     };
 
-    // ubar += J_{F*,own}^T cs + J_{Fn,own}^T cn   (cs, cn already scaled), one
-    // reverse sweep through the boundary map, the Riemann flux and the flux
-    __device__ __forceinline__ void face_adjoint(const GasP& g, int kind, const BCP& bc, const FaceAdjIn& in,
-                                                 const double cs[4], const double cn[4], double ub[4])
+    struct AdjFace
     {
-        double Un[4];
-        if (in.bnd)
-            bc_exterior(bc, in.dir, in.uo, in.nx, in.ny, Un);
-        else
-#pragma unroll
-            for (int q = 0; q < 4; ++q) Un[q] = in.un[q];
-        double ob[4] = {0, 0, 0, 0}, nb[4] = {0, 0, 0, 0};
-        if (in.own_right)
-            riemann2_vjp(g, kind, Un, in.uo, in.nx, in.ny, cs, nb, ob);
-        else
-            riemann2_vjp(g, kind, in.uo, Un, in.nx, in.ny, cs, ob, nb);
-        if (in.bnd) bc_vjp(bc, in.dir, in.uo, in.nx, in.ny, nb, ob);
-        double Fb[4], Gb[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
+        bool   live, xdir, wrap;
+        int    idx, elA, elB, bdir;
+        double fnx, fny, sc;    // metrics of the face index this tile evaluates
+        double nx1, ny1, sc1;   // wrap with differing ends: the outer element's own copy
+        double Uext[4];         // outer trace
+        double wA[4], wB[4];    // lifted cotangents of side A's / side B's copy (0 beyond a physical boundary)
+    };
+
+    __device__ __forceinline__ void face_metrics(const Res2DP& P, bool xdir, int j, int i, int idx, int N,
+                                                 double& nx, double& ny, double& sc)
+    {
+        if (!P.curved)
         {
-            Fb[q] = cn[q] * in.nx;
-            Gb[q] = cn[q] * in.ny;
+            nx = xdir ? 1.0 : 0.0;
+            ny = xdir ? 0.0 : 1.0;
+            sc = xdir ? P.sc_x : P.sc_y;
+            return;
         }
-        flux2_vjp(g, in.uo, Fb, Gb, ob);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) ub[q] += ob[q];
-    }
-    // only the Riemann part (neighbour copy of a face with its own metrics)
-    __device__ __forceinline__ void riemann_adjoint(const GasP& g, int kind, const FaceAdjIn& in,
-                                                    const double cs[4], double ub[4])
-    {
-        double nb[4] = {0, 0, 0, 0};
-        if (in.own_right)
-            riemann2_vjp(g, kind, in.un, in.uo, in.nx, in.ny, cs, nb, ub);
-        else
-            riemann2_vjp(g, kind, in.uo, in.un, in.nx, in.ny, cs, ub, nb);
+        // x faces: face index i in [0, nx] of row j; y faces: face index j in [0, ny] of column i
+        const double* fm = xdir ? P.fxm + ((static_cast<long long>(j) * (P.nx + 1) + i) * N + idx) * 4
+                                : P.fym + ((static_cast<long long>(j) * P.nx + i) * N + idx) * 4;
+        nx = fm[0];
+        ny = fm[1];
+        sc = fm[2];
     }
 
-    // phase 2 (2D)
+    // geometry + HBM gathers of face task t of a tile (same enumeration as face_task)
     template <int N>
-    __global__ __launch_bounds__(ADJ_BS) void k_adj2d(const Res2DP P, const double* __restrict__ u,
-                                                       const double* __restrict__ w,
-                                                       const double* __restrict__ wb, const AdjOut ao,
-                                                       double* __restrict__ out)
+    __device__ __forceinline__ void adj_face_setup(const Res2DP& P, const double* __restrict__ u,
+                                                   const double* __restrict__ wb, const TileGeom& g, int t,
+                                                   AdjFace& f)
     {
-        constexpr int NN  = N * N;
-        constexpr int EPB = AdjTile<N>::EPB;
-        __shared__ double sA[EPB][NN][4], sB[EPB][NN][4], sC[EPB][NN][4], sDd[EPB][NN][4];
-        __shared__ double sFc[EPB][4][N][4];
-        __shared__ double sD[NN];
-        const int tid = threadIdx.x;
-        for (int q = tid; q < NN; q += ADJ_BS) sD[q] = P.tab[q];
-        const long long blk  = xcd_tile(blockIdx.x, gridDim.x);
-        const int       el   = tid / NN;
-        const int       node = tid - el * NN;
-        const int       k    = node % N;
-        const int       l    = node / N;
-        const long long e    = blk * EPB + el;
-        const bool      act  = (el < EPB) && (e < P.E);
-        double W[4], U[4];
-        if (act)
+        constexpr int NN = N * N;
+        const int     XT = (g.tx + 1) * g.ty * N;
+        const int     YT = g.tx * (g.ty + 1) * N;
+        f.live           = false;
+        if (t >= XT + YT) return;
+        f.xdir = t < XT;
+        f.bdir = -1;
+        f.wrap = false;
+        long long eA = -1, eB = -1; // global elements of the two sides (-1: physical boundary)
+        int       fo = 0, io = 0, jo = 0; // the outer element's own face index (wrap faces)
+        bool      outerA = false, wrapped = false;
+        long long eo   = -1;
+        int       node = 0;
+        if (f.xdir)
         {
-            ld4d(w + (e * NN + node) * 4, W);
-            ld4d(u + (e * NN + node) * 4, U);
-            double mt[4];
-            if (P.curved)
-                ld4d(P.vmet + (e * NN + node) * 4, mt);
-            else
+            const int b  = t / ((g.tx + 1) * N);
+            const int r  = t - b * (g.tx + 1) * N;
+            const int a  = r / N;
+            f.idx        = r - a * N;
+            const int j  = g.j0 + b;
+            const int fi = g.i0 + a;
+            if (j >= P.ny) return;
+            const bool inA = a >= 1 && fi - 1 < P.nx;
+            const bool inB = a < g.tx && fi < P.nx;
+            if (!inA && !inB) return;
+            f.live = true;
+            f.elA  = inA ? b * g.tx + a - 1 : -1;
+            f.elB  = inB ? b * g.tx + a : -1;
+            face_metrics(P, true, j, fi, f.idx, N, f.fnx, f.fny, f.sc);
+            if (inA) eA = static_cast<long long>(j) * P.nx + fi - 1;
+            if (inB) eB = static_cast<long long>(j) * P.nx + fi;
+            if (!inA)
             {
-                mt[0] = P.xi_x;
-                mt[1] = 0.0;
-                mt[2] = 0.0;
-                mt[3] = P.eta_y;
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-            {
-                sA[el][node][q]  = mt[0] * W[q]; // xi_x  -> d/dxi  of F
-                sB[el][node][q]  = mt[2] * W[q]; // eta_x -> d/deta of F
-                sC[el][node][q]  = mt[1] * W[q]; // xi_y  -> d/dxi  of G
-                sDd[el][node][q] = mt[3] * W[q]; // eta_y -> d/deta of G
-            }
-        }
-        // ---- face tasks ----
-        for (int t = tid; t < EPB * 4 * N; t += ADJ_BS)
-        {
-            const int       fe = t / (4 * N);
-            const int       ft = t - fe * 4 * N;
-            const long long eg = blk * EPB + fe;
-            if (eg >= P.E) continue;
-            const int face = ft / N, idx = ft - (ft / N) * N;
-            const int eg32 = static_cast<int>(eg);
-            const int j = eg32 / P.nx, i = eg32 - j * P.nx;
-            int       ok, ol, nk, nl, ni = i, nj = j, dir, nface;
-            bool      bnd;
-            if (face == 0)
-            {
-                ok = 0; ol = idx; nk = N - 1; nl = idx; ni = i - 1; dir = 0; nface = 1;
-                bnd = (i == 0) && !P.per_x;
-                if (ni < 0) ni += P.nx;
-            }
-            else if (face == 1)
-            {
-                ok = N - 1; ol = idx; nk = 0; nl = idx; ni = i + 1; dir = 1; nface = 0;
-                bnd = (i == P.nx - 1) && !P.per_x;
-                if (ni >= P.nx) ni -= P.nx;
-            }
-            else if (face == 2)
-            {
-                ok = idx; ol = 0; nk = idx; nl = N - 1; nj = j - 1; dir = 3; nface = 3;
-                bnd = (j == 0) && !P.per_y;
-                if (nj < 0) nj += P.ny;
-            }
-            else
-            {
-                ok = idx; ol = N - 1; nk = idx; nl = 0; nj = j + 1; dir = 2; nface = 2;
-                bnd = (j == P.ny - 1) && !P.per_y;
-                if (nj >= P.ny) nj -= P.ny;
-            }
-            // face geometry of the own copy and of the neighbour's copy
-            double m0[3], m1[3];
-            auto face_m = [&](int fi, int fj, int f, double* m)
-            {
-                if (f < 2)
+                outerA = true;
+                int ni = fi - 1;
+                if (ni < 0)
                 {
-                    if (P.curved)
+                    if (P.per_x)
                     {
-                        const double* fm =
-                            P.fxm + ((static_cast<long long>(fj) * (P.nx + 1) + fi + f) * N + idx) * 4;
-                        m[0] = fm[0]; m[1] = fm[1]; m[2] = fm[2];
+                        ni      = P.nx - 1;
+                        wrapped = true;
+                        fo      = P.nx;
                     }
                     else
-                    {
-                        m[0] = 1.0; m[1] = 0.0; m[2] = P.sc_x;
-                    }
+                        f.bdir = 0;
                 }
-                else
+                if (f.bdir < 0)
                 {
-                    if (P.curved)
+                    eo   = static_cast<long long>(j) * P.nx + ni;
+                    eA   = eo;
+                    node = f.idx * N + (N - 1);
+                }
+                jo = j;
+            }
+            else if (!inB)
+            {
+                int ni = fi;
+                if (ni >= P.nx)
+                {
+                    if (P.per_x)
                     {
-                        const double* fm =
-                            P.fym + ((static_cast<long long>(fj + (f - 2)) * P.nx + fi) * N + idx) * 4;
-                        m[0] = fm[0]; m[1] = fm[1]; m[2] = fm[2];
+                        ni      = 0;
+                        wrapped = true;
+                        fo      = 0;
                     }
                     else
-                    {
-                        m[0] = 0.0; m[1] = 1.0; m[2] = P.sc_y;
-                    }
+                        f.bdir = 1;
                 }
-            };
-            face_m(i, j, face, m0);
-            FaceAdjIn in;
-            const long long own = (eg * NN + ol * N + ok) * 4;
-            ld4d(u + own, in.uo);
-            in.nx        = m0[0];
-            in.ny        = m0[1];
-            in.own_right = (face == 0 || face == 2);
-            in.bnd       = bnd;
-            in.dir       = dir;
-            const double* wbo = wb + ((eg * 4 + face) * N + idx) * 4;
-            double        cs[4], cn[4], ub[4] = {0, 0, 0, 0};
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-            {
-                cs[q] = m0[2] * wbo[q];
-                cn[q] = -m0[2] * wbo[q];
-            }
-            if (!bnd)
-            {
-                const long long en = static_cast<long long>(nj) * P.nx + ni;
-                ld4d(u + (en * NN + nl * N + nk) * 4, in.un);
-                face_m(ni, nj, nface, m1);
-                const double* wbn = wb + ((en * 4 + nface) * N + idx) * 4;
-                if (m1[0] == m0[0] && m1[1] == m0[1] && m1[2] == m0[2])
+                if (f.bdir < 0)
                 {
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) cs[q] += m0[2] * wbn[q];
+                    eo   = static_cast<long long>(j) * P.nx + ni;
+                    eB   = eo;
+                    node = f.idx * N;
                 }
-                else
-                {
-                    // periodic wrap on a map whose two ends differ: the neighbour's
-                    // copy has its own normal and scale
-                    FaceAdjIn in2 = in;
-                    in2.nx        = m1[0];
-                    in2.ny        = m1[1];
-                    double c2[4];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) c2[q] = m1[2] * wbn[q];
-                    riemann_adjoint(P.gas, P.riemann, in2, c2, ub);
-                }
+                jo = j;
             }
-            face_adjoint(P.gas, P.riemann, P.bc, in, cs, cn, ub);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) sFc[fe][face][idx][q] = ub[q];
+            io = fo;
         }
-        __syncthreads();
-        if (!act) return;
-        // ---- volume: Fbar, Gbar at this node ----
-        double Fb[4], Gb[4];
+        else
+        {
+            const int uu = t - XT;
+            const int bf = uu / (g.tx * N);
+            const int r  = uu - bf * g.tx * N;
+            const int a  = r / N;
+            f.idx        = r - a * N;
+            const int i  = g.i0 + a;
+            const int fj = g.j0 + bf;
+            if (i >= P.nx) return;
+            const bool inA = bf >= 1 && fj - 1 < P.ny;
+            const bool inB = bf < g.ty && fj < P.ny;
+            if (!inA && !inB) return;
+            f.live = true;
+            f.elA  = inA ? (bf - 1) * g.tx + a : -1;
+            f.elB  = inB ? bf * g.tx + a : -1;
+            face_metrics(P, false, fj, i, f.idx, N, f.fnx, f.fny, f.sc);
+            if (inA) eA = static_cast<long long>(fj - 1) * P.nx + i;
+            if (inB) eB = static_cast<long long>(fj) * P.nx + i;
+            if (!inA)
+            {
+                outerA = true;
+                int nj = fj - 1;
+                if (nj < 0)
+                {
+                    if (P.per_y)
+                    {
+                        nj      = P.ny - 1;
+                        wrapped = true;
+                        fo      = P.ny;
+                    }
+                    else
+                        f.bdir = 3;
+                }
+                if (f.bdir < 0)
+                {
+                    eo   = static_cast<long long>(nj) * P.nx + i;
+                    eA   = eo;
+                    node = (N - 1) * N + f.idx;
+                }
+            }
+            else if (!inB)
+            {
+                int nj = fj;
+                if (nj >= P.ny)
+                {
+                    if (P.per_y)
+                    {
+                        nj      = 0;
+                        wrapped = true;
+                        fo      = 0;
+                    }
+                    else
+                        f.bdir = 2;
+                }
+                if (f.bdir < 0)
+                {
+                    eo   = static_cast<long long>(nj) * P.nx + i;
+                    eB   = eo;
+                    node = f.idx;
+                }
+            }
+            io = i;
+            jo = fo;
+        }
+        if (eo >= 0) ld4d(u + (eo * NN + node) * 4, f.Uext);
+        const int fa = f.xdir ? 1 : 3, fb = f.xdir ? 0 : 2;
+        if (eA >= 0)
+            ld4d(wb + ((eA * 4 + fa) * N + f.idx) * 4, f.wA);
+        else
+#pragma unroll
+            for (int q = 0; q < 4; ++q) f.wA[q] = 0.0;
+        if (eB >= 0)
+            ld4d(wb + ((eB * 4 + fb) * N + f.idx) * 4, f.wB);
+        else
+#pragma unroll
+            for (int q = 0; q < 4; ++q) f.wB[q] = 0.0;
+        if (wrapped && P.curved)
+        {
+            face_metrics(P, f.xdir, jo, io, f.idx, N, f.nx1, f.ny1, f.sc1);
+            f.wrap = !(f.nx1 == f.fnx && f.ny1 == f.fny && f.sc1 == f.sc);
+        }
+        (void)outerA;
+    }
+
+    template <int N>
+    __device__ __forceinline__ void adj_face_eval(const Res2DP& P, const AdjFace& f, AdjSmem<N>& sm)
+    {
+        const int kA = f.xdir ? N - 1 : f.idx, lA = f.xdir ? f.idx : N - 1;
+        const int kB = f.xdir ? 0 : f.idx, lB = f.xdir ? f.idx : 0;
+        double    UA[4], UB[4];
+        if (f.elA >= 0)
+        {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) UA[q] = sm.U[f.elA][lA][sm.pos(lA, kA, q)];
+        }
+        if (f.elB >= 0)
+        {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) UB[q] = sm.U[f.elB][lB][sm.pos(lB, kB, q)];
+        }
+        if (f.elA < 0)
+        {
+            if (f.bdir >= 0)
+                bc_exterior(P.bc, f.bdir, UB, f.fnx, f.fny, UA);
+            else
+#pragma unroll
+                for (int q = 0; q < 4; ++q) UA[q] = f.Uext[q];
+        }
+        if (f.elB < 0)
+        {
+            if (f.bdir >= 0)
+                bc_exterior(P.bc, f.bdir, UA, f.fnx, f.fny, UB);
+            else
+#pragma unroll
+                for (int q = 0; q < 4; ++q) UB[q] = f.Uext[q];
+        }
+        // Riemann flux cotangent: both copies of the face (the outer copy of a
+        // wrap with differing ends separately)
+        double cs[4], bA[4] = {0, 0, 0, 0}, bB[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int q = 0; q < 4; ++q)
         {
-            double f = 0.0, g = 0.0;
+            if (!f.wrap)
+                cs[q] = f.sc * f.wA[q] + f.sc * f.wB[q];
+            else
+                cs[q] = f.sc * (f.elA >= 0 ? f.wA[q] : f.wB[q]);
+        }
+        riemann2_vjp(P.gas, P.riemann, UA, UB, f.fnx, f.fny, cs, bA, bB);
+        if (f.wrap)
+        {
+            double c2[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) c2[q] = f.sc1 * (f.elA >= 0 ? f.wB[q] : f.wA[q]);
+            riemann2_vjp(P.gas, P.riemann, UA, UB, f.nx1, f.ny1, c2, bA, bB);
+        }
+        // physical boundary: the exterior state is a function of the own trace
+        if (f.bdir >= 0)
+        {
+            if (f.elA < 0)
+                bc_vjp(P.bc, f.bdir, UB, f.fnx, f.fny, bA, bB);
+            else
+                bc_vjp(P.bc, f.bdir, UA, f.fnx, f.fny, bB, bA);
+        }
+        // own normal flux -s Wbar . n of each tile side
+        if (f.elA >= 0)
+        {
+            double Fb[4], Gb[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                const double cn = -f.sc * f.wA[q];
+                Fb[q]           = cn * f.fnx;
+                Gb[q]           = cn * f.fny;
+            }
+            flux2_vjp(P.gas, UA, Fb, Gb, bA);
+            const int fa = f.xdir ? 1 : 3;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sm.Cs[f.elA][fa][f.idx][q] = bA[q];
+        }
+        if (f.elB >= 0)
+        {
+            double Fb[4], Gb[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                const double cn = -f.sc * f.wB[q];
+                Fb[q]           = cn * f.fnx;
+                Gb[q]           = cn * f.fny;
+            }
+            flux2_vjp(P.gas, UB, Fb, Gb, bB);
+            const int fb = f.xdir ? 0 : 2;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sm.Cs[f.elB][fb][f.idx][q] = bB[q];
+        }
+    }
+
+    // transposed D contraction along one line of element le, equation q:
+    //   xi-line  (row ln):    FX[ln][k] (+)= sum_m D[m][k] ma[m] W[ln][m], GX likewise with mb
+    //   eta-line (column ln): FX[l][ln] (+)= sum_m D[m][l] ma[m] W[m][ln], GX likewise with mb
+    // (ma, mb) = (xi_x, xi_y) on xi-lines, (eta_x, eta_y) on eta-lines; affine:
+    // xi-lines write FX from xi_x only, eta-lines GX from eta_y only.
+    template <int N, bool CURVED, bool ACC>
+    __device__ __forceinline__ void adj_line(const Res2DP& P, AdjSmem<N>& sm, bool eta, int le, int ln, int q)
+    {
+        const double* D = P.tab;
+        double        a[N], b[N];
+        if constexpr (CURVED)
+        {
+            double ma[N], mb[N];
+            line_metrics<N>(P.vmet, sm.eid[le], eta, ln, ma, mb);
+#pragma unroll
             for (int m = 0; m < N; ++m)
             {
-                // d/dxi terms: nodes (m, l) with D[m][k]; d/deta terms: nodes (k, m) with D[m][l]
-                f += sD[m * N + k] * sA[el][l * N + m][q] + sD[m * N + l] * sB[el][m * N + k][q];
-                g += sD[m * N + k] * sC[el][l * N + m][q] + sD[m * N + l] * sDd[el][m * N + k][q];
+                const double w = eta ? sm.W[le][m][sm.pos(m, ln, q)] : sm.W[le][ln][sm.pos(ln, m, q)];
+                a[m]           = ma[m] * w;
+                b[m]           = mb[m] * w;
             }
-            Fb[q] = f;
-            Gb[q] = g;
         }
-        double ub[4] = {0, 0, 0, 0};
+        else
+        {
+            const double mt = eta ? P.eta_y : P.xi_x;
+#pragma unroll
+            for (int m = 0; m < N; ++m)
+                a[m] = mt * (eta ? sm.W[le][m][sm.pos(m, ln, q)] : sm.W[le][ln][sm.pos(ln, m, q)]);
+        }
+#pragma unroll
+        for (int r = 0; r < N; ++r)
+        {
+            double f = 0.0, g = 0.0;
+#pragma unroll
+            for (int m = 0; m < N; ++m)
+            {
+                f += D[m * N + r] * a[m];
+                if constexpr (CURVED) g += D[m * N + r] * b[m];
+            }
+            const int l = eta ? r : ln, k = eta ? ln : r;
+            if constexpr (CURVED)
+            {
+                if constexpr (ACC)
+                {
+                    sm.FX[le][l][sm.pos(l, k, q)] += f;
+                    sm.GX[le][l][sm.pos(l, k, q)] += g;
+                }
+                else
+                {
+                    sm.FX[le][l][sm.pos(l, k, q)] = f;
+                    sm.GX[le][l][sm.pos(l, k, q)] = g;
+                }
+            }
+            else if (eta)
+                sm.GX[le][l][sm.pos(l, k, q)] = f;
+            else
+                sm.FX[le][l][sm.pos(l, k, q)] = f;
+        }
+    }
+
+    template <int N, bool CURVED>
+    __global__ __launch_bounds__(256) void k_adj2d_tile(const Res2DP P, const double* __restrict__ u,
+                                                        const double* __restrict__ w,
+                                                        const double* __restrict__ wb, const AdjOut ao,
+                                                        double* __restrict__ out)
+    {
+        using T           = Tile2D<N, double>;
+        constexpr int BS  = 256;
+        constexpr int NN  = N * N;
+        __shared__ AdjSmem<N> sm;
+        const int      tid  = threadIdx.x;
+        const TileGeom g    = tile_geom(P, xcd_tile(blockIdx.x, gridDim.x), T::TX, T::TY);
+        const int      tx = T::TX, ty = T::TY;
+        const int      el   = tid / NN;
+        const int      node = tid - el * NN;
+        const int      k    = node % N;
+        const int      l    = node / N;
+        const int      eb   = el / tx;
+        const int      ea   = el - eb * tx;
+        const bool     act  = (el < tx * ty) && (g.i0 + ea < P.nx) && (g.j0 + eb < P.ny);
+        const long long e   = static_cast<long long>(g.j0 + eb) * P.nx + (g.i0 + ea);
+        if (node == 0 && el < tx * ty) sm.eid[el] = act ? e : -1;
+
+        // phase-1 tasks: line tasks first (curved: xi-lines; affine: xi- and
+        // eta-lines), then face tasks
+        const int nlt = tx * ty * N * 4;
+        const int nl  = CURVED ? nlt : 2 * nlt;
+        const int nft = ((tx + 1) * ty + tx * (ty + 1)) * N;
+
+        if (act)
+        {
+            double U[4], Wv[4];
+            ld4d(u + (e * NN + node) * 4, U);
+            ld4d(w + (e * NN + node) * 4, Wv);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                sm.U[el][l][sm.pos(l, k, q)] = U[q];
+                sm.W[el][l][sm.pos(l, k, q)] = Wv[q];
+            }
+        }
+        __syncthreads();
+
+        for (int t = tid; t < nl + nft; t += BS)
+        {
+            if (t < nl)
+            {
+                const bool eta = t >= nlt;
+                const int  wq  = eta ? t - nlt : t;
+                const int  le  = (wq >> 2) / N;
+                if (sm.eid[le] < 0) continue;
+                adj_line<N, CURVED, false>(P, sm, eta, le, (wq >> 2) - le * N, wq & 3);
+            }
+            else
+            {
+                AdjFace f;
+                adj_face_setup<N>(P, u, wb, g, t - nl, f);
+                if (f.live) adj_face_eval<N>(P, f, sm);
+            }
+        }
+        __syncthreads();
+        if constexpr (CURVED)
+        {
+            for (int t = tid; t < nlt; t += BS)
+            {
+                const int le = (t >> 2) / N;
+                if (sm.eid[le] < 0) continue;
+                adj_line<N, true, true>(P, sm, true, le, (t >> 2) - le * N, t & 3);
+            }
+            __syncthreads();
+        }
+        if (!act) return;
+
+        double U[4], Wv[4], Fb[4], Gb[4], ub[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+            U[q]  = sm.U[el][l][sm.pos(l, k, q)];
+            Wv[q] = sm.W[el][l][sm.pos(l, k, q)];
+            Fb[q] = sm.FX[el][l][sm.pos(l, k, q)];
+            Gb[q] = sm.GX[el][l][sm.pos(l, k, q)];
+        }
         flux2_vjp(P.gas, U, Fb, Gb, ub);
         if (P.rinv)
         {
@@ -307,22 +540,22 @@ namespace frsdg
                 radial_source2(P.gas, A, ir, sr);
                 double acc = 0.0;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) acc += sr[q].d * W[q];
+                for (int q = 0; q < 4; ++q) acc += sr[q].d * Wv[q];
                 ub[i] += acc;
             }
         }
-        // ---- gather face contributions of this node ----
+        double o[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
         {
             double s = ub[q];
-            if (k == 0) s += sFc[el][0][l][q];
-            if (k == N - 1) s += sFc[el][1][l][q];
-            if (l == 0) s += sFc[el][2][k][q];
-            if (l == N - 1) s += sFc[el][3][k][q];
-            ub[q] = ao.at((e * NN + node) * 4 + q, W[q], s);
+            if (k == 0) s += sm.Cs[el][0][l][q];
+            if (k == N - 1) s += sm.Cs[el][1][l][q];
+            if (l == 0) s += sm.Cs[el][2][k][q];
+            if (l == N - 1) s += sm.Cs[el][3][k][q];
+            o[q] = ao.at((e * NN + node) * 4 + q, Wv[q], s);
         }
-        st4d(out + (e * NN + node) * 4, ub);
+        st4d(out + (e * NN + node) * 4, o);
     }
 
     // ------------------------------------------------------------------

@@ -472,11 +472,15 @@ namespace
             const long long nl = c->E * 4 * N;
             hipLaunchKernelGGL((k_adj_lift<N>), dim3(static_cast<unsigned>((nl + 255) / 256)), dim3(256), 0, s,
                                c->E, c->d_tab, w, c->wb);
-            constexpr int EPB = AdjTile<N>::EPB;
-            const long long nb = (c->E + EPB - 1) / EPB;
-            Res2DP P = c->p2;
-            hipLaunchKernelGGL((k_adj2d<N>), dim3(static_cast<unsigned>(nb)), dim3(ADJ_BS), 0, s, P, u, w,
-                               static_cast<const double*>(c->wb), ao, out);
+            using T            = Tile2D<N, double>;
+            const long long nb = tile_count(c->p2.nx, c->p2.ny, T::TX, T::TY);
+            const double*   wb = c->wb;
+            if (c->p2.curved)
+                hipLaunchKernelGGL((k_adj2d_tile<N, true>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, c->p2,
+                                   u, w, wb, ao, out);
+            else
+                hipLaunchKernelGGL((k_adj2d_tile<N, false>), dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, c->p2,
+                                   u, w, wb, ao, out);
             return hip_rc(hipGetLastError());
         }
     };
@@ -1130,7 +1134,7 @@ int frsdg_ssprk_run(frsdg_ctx* c, double t, double dt, double* u, int stages, in
         int f = 0;
         FR_TRY(hipMemcpyAsync(&f, c->health, sizeof(int), hipMemcpyDeviceToHost, s));
         FR_TRY(hipStreamSynchronize(s));
-        return f ? FRSDG_ERR_NUMERIC : FRSDG_OK;
+        return (f & c->stop_mask) ? FRSDG_ERR_NUMERIC : FRSDG_OK;
     };
     if (nsteps < CH) // short runs: plain launches
     {
@@ -1165,7 +1169,7 @@ int frsdg_ssprk_run(frsdg_ctx* c, double t, double dt, double* u, int stages, in
     }
     const int full = nsteps / CH, rem = nsteps % CH;
     // -ts_error_if_step_fails: stop after the first graph of steps whose stage
-    // outputs went non-finite or non-positive (checked once per graph)
+    // outputs raised a flag of the stop mask (checked once per graph)
     for (int k = 0; k < full; ++k)
     {
         FR_TRY(hipGraphLaunch(c->gexec, s));
@@ -1174,6 +1178,13 @@ int frsdg_ssprk_run(frsdg_ctx* c, double t, double dt, double* u, int stages, in
     for (int k = 0; k < rem; ++k)
         if ((rc = do_step(c, dt, u, stages, s))) return rc;
     return healthy();
+}
+
+int frsdg_set_stop_mask(frsdg_ctx* c, int mask)
+{
+    if (!c || (mask & ~7)) return FRSDG_ERR_INVALID;
+    c->stop_mask = mask;
+    return FRSDG_OK;
 }
 
 int frsdg_step_health(frsdg_ctx* c, int* flags, int clear, void* stream)

@@ -38,6 +38,7 @@ struct frsdg_ctx
     double* wbeta = nullptr; // WENO smoothness of every element line [E][2][n][4]
     int*      health          = nullptr; // stage-output health flags of explicit stepping (device)
     int*      health_chk      = nullptr; // frsdg_check_state flags (device)
+    int       stop_mask       = 1;       // health flags that stop frsdg_ssprk_run (default: non-finite)
     double*   adj_pool        = nullptr; // unsteady adjoint: snapshots + work states
     long long adj_pool_states = 0;
     // WENO limiter constants (apply_weno without the spectral filter)

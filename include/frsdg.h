@@ -151,13 +151,20 @@ int frsdg_boundary_weights(frsdg_ctx* ctx, int dir, double* out, void* stream);
  * residual; post-step filter (TSSetPostStep, petsc_solver_impl.hpp:604-605). */
 int frsdg_ssprk_step(frsdg_ctx* ctx, double t, double dt, double* u, int stages, void* stream);
 /* nsteps steps of frsdg_ssprk_step, captured once into a hipGraph (16 steps) and
- * replayed.  Every stage output is checked where it is stored (a non-finite
- * value, rho <= 0 or internal energy <= 0 at a node); the run stops after the
- * first 16-step graph (or the remaining plain steps) with such a state and
- * returns FRSDG_ERR_NUMERIC (-ts_error_if_step_fails, tools/check_radial.py:208-210),
- * u holding the last state computed.  Synchronises the stream once per graph. */
+ * replayed.  Every stage output is checked where it is stored (health flags:
+ * bit 1 a non-finite value, bit 2 rho <= 0, bit 4 internal energy <= 0 at a
+ * node); the run stops after the first 16-step graph (or the remaining plain
+ * steps) that raised a flag of the stop mask and returns FRSDG_ERR_NUMERIC
+ * (-ts_error_if_step_fails, tools/check_radial.py:208-210), u holding the last
+ * state computed.  Synchronises the stream once per graph. */
 int frsdg_ssprk_run(frsdg_ctx* ctx, double t, double dt, double* u, int stages, int nsteps,
                     void* stream);
+/* Health flags that stop frsdg_ssprk_run (a subset of 1 | 2 | 4); the default 1
+ * stops on non-finite states only: the reference's explicit runs carry on
+ * through transient non-positive nodal values (the floors of the flux keep
+ * them finite), which frsdg_step_health still reports.  No counterpart in the
+ * reference (its step-failure test is PETSc's). */
+int frsdg_set_stop_mask(frsdg_ctx* ctx, int mask);
 /* The stage-output health flags that frsdg_ssprk_step calls accumulated (bit 1
  * non-finite, bit 2 rho <= 0, bit 4 internal energy <= 0), cleared when clear
  * is set; frsdg_ssprk_run clears them when it starts.  Synchronises the stream. */

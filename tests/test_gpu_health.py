@@ -63,6 +63,36 @@ def test_forced_bad_node_is_flagged_and_stops_the_run(name, bad):
     assert ev.step_health() != 0
     assert ev.step_health() == 0  # cleared by the previous call
     # the graph-replayed run stops with a numerical failure after the first graph
+    # (stop mask: every flag; the default stops on non-finite states only)
+    ev.set_stop_mask(fr.HEALTH_NONFINITE | fr.HEALTH_DENSITY | fr.HEALTH_ENERGY)
     x = v.clone()
     with pytest.raises(fr.FrsdgError, match="error 5"):
         ev.ssprk_run(x, 1e-4, 48)
+
+
+@pytest.mark.parametrize("name", sorted(CFGS))
+def test_default_stop_mask_runs_through_non_positive_nodes(name):
+    # a finite state with one node of negative internal energy: flagged, but
+    # under the default mask (non-finite only) one graph of steps runs through
+    # unless the state itself turns non-finite
+    torch, fr, ev, u = _setup(CFGS[name])
+    cfg = CFGS[name]
+    h = u.cpu().numpy().copy()
+    n = cfg["p"] + 1
+    if cfg["dims"] == 2:
+        i = (h.size // 4 // 2) * 4
+        h[i + 3] = 0.1 * (h[i + 1] ** 2 + h[i + 2] ** 2) / (2.0 * h[i]) - 1e-3  # rho E below the kinetic part
+    else:
+        i = (cfg["nx"] // 2) * 3 * n + 2 * n + 1
+        h[i] = -1e-3
+    v = torch.from_numpy(h).cuda()
+    assert ev.check_state(v) & fr.HEALTH_ENERGY
+    x = v.clone()
+    try:
+        ev.ssprk_run(x, 1e-6, 16)
+        nonfinite = False
+    except fr.FrsdgError:
+        nonfinite = True
+    assert bool(ev.check_state(x) & fr.HEALTH_NONFINITE) == nonfinite
+    with pytest.raises(fr.FrsdgError, match="invalid"):
+        ev.set_stop_mask(8)
