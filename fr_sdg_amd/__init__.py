@@ -97,7 +97,7 @@ EXPORTS = [
     "frsdg_ssprk_step", "frsdg_ssprk_run", "frsdg_impose_ic", "frsdg_l2_error", "frsdg_strerror",
     "frsdg_beuler_step", "frsdg_batch_create", "frsdg_batch_destroy", "frsdg_batch_ndofs",
     "frsdg_batch_offsets", "frsdg_batch_instance", "frsdg_batch_set_dt", "frsdg_batch_set_tuning",
-    "frsdg_batch_ssprk_step", "frsdg_batch_ssprk_run", "frsdg_batch_ssprk_step_each",
+    "frsdg_batch_ssprk_step", "frsdg_batch_ssprk_run", "frsdg_batch_ssprk_step_each", "frsdg_batch_set_tile_shape",
     "frsdg_filter_transpose", "frsdg_mass", "frsdg_boundary_weights", "frsdg_adjoint_solve", "frsdg_pseudo_run",
     "frsdg_color_jacobian", "frsdg_ilu0_factor", "frsdg_ilu0_apply", "frsdg_linear_solve",
     "frsdg_ilu0_apply_transpose", "frsdg_l2_error_noalloc", "frsdg_unsteady_adjoint", "frsdg_step_health",
@@ -158,6 +158,7 @@ def load_library(path=LIB_PATH):
     L.frsdg_batch_instance.argtypes = [vp, i, ctypes.POINTER(ctypes.c_void_p)]
     L.frsdg_batch_set_dt.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     L.frsdg_batch_set_tuning.argtypes = [vp, i, i, i]
+    L.frsdg_batch_set_tile_shape.argtypes = [vp, i, i]
     L.frsdg_batch_ssprk_step.argtypes = [vp, dp, vp]
     L.frsdg_batch_ssprk_run.argtypes = [vp, dp, i, vp]
     L.frsdg_batch_ssprk_step_each.argtypes = [vp, ctypes.POINTER(ctypes.c_double), dp, vp]
@@ -552,6 +553,14 @@ class Batch:
 
     def set_tuning(self, grid, epb_div, mode=1):
         _check(self.lib.frsdg_batch_set_tuning(self.h, grid, epb_div, mode))
+
+    def set_tile_shape(self, n, shape):
+        """Element tile of the order-n instances: 0 default, 1 halved, 2 the
+        other tile with the fewest rim faces per element."""
+        _check(self.lib.frsdg_batch_set_tile_shape(self.h, n, shape))
+
+    def orders(self):
+        return sorted({c["p"] + 1 for c in self.cfgs})
 
     def ssprk_step(self, u):
         _check(self.lib.frsdg_batch_ssprk_step(self.h, _ptr(u), _stream()))

@@ -1,16 +1,18 @@
 """Launch-shape autotuner for the batched sweep.
 
-Knobs (frsdg_batch_set_tuning): the number of persistent workgroups of the
-stage kernels (or one workgroup per element tile), the element tile shape
-(the default tile of each order, or that tile with its rows halved: more,
-shorter work items, better balance across uneven instances), and the launch
-mode (one kernel per stage over all orders; one specialised kernel per order
-and mesh kind; or the same cut into cache-sized runs of instances that take
-their three stages back to back, so stages 2 and 3 find their inputs in the
-last-level cache).  Each candidate is timed over
-a few graph-free batched steps on a scratch copy of the state; the winner is
-cached in a JSON file keyed by the batch signature (per-instance mesh size,
-order, gas model) and the device name.
+Knobs: the number of persistent workgroups of the stage kernels (or one
+workgroup per element tile), the launch mode (one kernel per stage over all
+orders; one specialised kernel per order and mesh kind; or the same cut into
+cache-sized runs of instances that take their three stages back to back, so
+stages 2 and 3 find their inputs in the last-level cache), both through
+frsdg_batch_set_tuning, and the element tile of each order
+(frsdg_batch_set_tile_shape: the default tile of the order, that tile halved,
+or the other tile with the fewest rim faces per element).  The search runs in
+two passes, each candidate timed over a few graph-free batched steps on a
+scratch copy of the state: (1) grid x {default, halved tiles} x mode over the
+whole batch; (2) from the winner, a coordinate pass over the tile shape of one
+order at a time.  The result is cached in a JSON file keyed by the batch
+signature (per-instance mesh size, order, gas model) and the device name.
 """
 import hashlib
 import json
@@ -52,31 +54,60 @@ def candidates():
     return [(g * ncu, d, m) for m in (1, 2, 0) for g in (0, 1, 2, 4, 8) for d in (1, 2)]
 
 
+def _time(batch, u, scratch, steps):
+    s = torch.cuda.current_stream()
+    scratch.copy_(u)
+    batch.ssprk_step(scratch)  # warm
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(s)
+    for _ in range(steps):
+        batch.ssprk_step(scratch)
+    b.record(s)
+    b.synchronize()
+    return a.elapsed_time(b) / steps
+
+
+def _apply(batch, best):
+    g, d, m, shapes = best
+    batch.set_tuning(g, d, m)
+    for n, sh in shapes.items():
+        batch.set_tile_shape(int(n), int(sh))
+
+
 def tune_batch(batch, u, steps=3, use_cache=True):
-    """Pick (grid, epb_div, mode) for `batch` by timing; returns (best, table)."""
+    """Pick (grid, epb_div, mode, {order: tile shape}) for `batch` by timing;
+    returns (best, table): table rows [grid, epb_div, mode, shapes, ms]."""
     key = signature(batch)
     db = _load() if use_cache else {}
     if use_cache and key in db:
-        g, d, m = db[key]["best"]
-        batch.set_tuning(g, d, m)
-        return (g, d, m), db[key]["table"]
+        best = tuple(db[key]["best"])
+        _apply(batch, best)
+        return best, db[key]["table"]
     scratch = u.clone()
     table = []
-    s = torch.cuda.current_stream()
     for g, d, m in candidates():
         batch.set_tuning(g, d, m)
-        scratch.copy_(u)
-        batch.ssprk_step(scratch)  # warm
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(s)
-        for _ in range(steps):
-            batch.ssprk_step(scratch)
-        b.record(s)
-        b.synchronize()
-        table.append([g, d, m, a.elapsed_time(b) / steps])
-    best = min(table, key=lambda r: r[3])
-    batch.set_tuning(best[0], best[1], best[2])
-    db[key] = {"best": best[:3], "table": table}
+        table.append([g, d, m, {}, _time(batch, u, scratch, steps)])
+    g, d, m, _, t_best = min(table, key=lambda r: r[4])
+    # pass 2: tile shape per order, one order at a time from the pass-1 winner
+    shapes = {n: d - 1 for n in batch.orders()}
+    batch.set_tuning(g, d, m)
+    for n in batch.orders():
+        for sh in (0, 1, 2):
+            if sh == shapes[n]:
+                continue
+            batch.set_tile_shape(n, sh)
+            trial = dict(shapes)
+            trial[n] = sh
+            t = _time(batch, u, scratch, steps)
+            table.append([g, d, m, trial, t])
+            if t < t_best:
+                t_best, shapes = t, trial
+            else:
+                batch.set_tile_shape(n, shapes[n])
+    best = (g, d, m, shapes)
+    _apply(batch, best)
+    db[key] = {"best": list(best), "table": table}
     if use_cache:
         _store(db)
-    return (best[0], best[1], best[2]), table
+    return best, table

@@ -203,16 +203,17 @@ namespace
     // one launch per (order, curvature): the tiles of all instances of the
     // group, fully specialised (registers and LDS sized for this order only)
     // the same with one workgroup per tile (grid = tiles of the group) and the
-    // tile shape fixed at compile time (DIV = 1 default tile, 2 halved): no
+    // tile shape fixed at compile time (SHAPE: batch_tile_dim's choice): no
     // loop and no run-time shape arithmetic, so the register budget is the
     // single-problem kernel's (≤ 128 VGPRs, 4 waves per SIMD; a cap of 96 for
     // 5 waves spills 60-88 bytes per thread at n = 5, 7, 8)
-    template <int N, bool CURVED, bool FILT, int DIV>
+    template <int N, bool CURVED, bool FILT, int SHAPE>
     __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_tile_n(const BatchInst* __restrict__ tab, const WorkList W,
                                                           StageArgs A)
     {
-        using T       = Tile2D<N, double>;
-        constexpr int TX = tile_halve(T::TX, T::TY, DIV, true), TY = tile_halve(T::TX, T::TY, DIV, false);
+        using T          = Tile2D<N, double>;
+        constexpr int TX = batch_tile_dim(N, SHAPE, true), TY = batch_tile_dim(N, SHAPE, false);
+        static_assert(TX * TY <= T::EPB, "tile larger than the LDS of the order");
         __shared__ TileSmem<N, double, CURVED> sm;
         long long       blk;
         const int       k  = work_find(W, xcd_tile(blockIdx.x, gridDim.x), blk);
@@ -251,7 +252,7 @@ struct frsdg_batch
     long long               total_tiles = 0;
     long long               nd          = 0;
     int                     grid        = 0; // persistent workgroups (0: one workgroup per tile)
-    int                     epb_div     = 1;
+    int                     shape[10]   = {}; // element tile shape choice per order n (batch_tile_dim)
     hipGraph_t              graph = nullptr;
     hipGraphExec_t          gexec = nullptr;
     double*                 g_u   = nullptr;
@@ -294,7 +295,8 @@ namespace
         long long t = 0;
         for (auto& I : b->inst)
         {
-            tile_shape(I.n, b->epb_div, I.tx, I.ty);
+            I.tx = batch_tile_dim(I.n, b->shape[I.n], true);
+            I.ty = batch_tile_dim(I.n, b->shape[I.n], false);
             I.tile0  = t;
             I.ntiles = tile_count(I.P.nx, I.P.ny, I.tx, I.ty);
             t += I.ntiles;
@@ -401,7 +403,11 @@ namespace
         {                                                                                                     \
             if (last) FR_BK(k_batch_stage_n, NN, C, true); else FR_BK(k_batch_stage_n, NN, C, false);         \
         }                                                                                                     \
-        else if (b->epb_div == 1)                                                                             \
+        else if (b->shape[NN] == 0)                                                                           \
+        {                                                                                                     \
+            if (last) FR_BK(k_batch_tile_n, NN, C, true, 0); else FR_BK(k_batch_tile_n, NN, C, false, 0);     \
+        }                                                                                                     \
+        else if (b->shape[NN] == 1)                                                                           \
         {                                                                                                     \
             if (last) FR_BK(k_batch_tile_n, NN, C, true, 1); else FR_BK(k_batch_tile_n, NN, C, false, 1);     \
         }                                                                                                     \
@@ -546,9 +552,16 @@ int frsdg_batch_instance(frsdg_batch* b, int k, frsdg_ctx** out)
 int frsdg_batch_set_tuning(frsdg_batch* b, int grid, int epb_div, int mode)
 {
     if (!b || grid < 0 || (epb_div != 1 && epb_div != 2) || mode < 0 || mode > 2) return FRSDG_ERR_INVALID;
-    b->grid    = grid;
-    b->epb_div = epb_div;
-    b->mode    = mode;
+    b->grid = grid;
+    for (int& sh : b->shape) sh = epb_div - 1;
+    b->mode = mode;
+    return rebuild_tiles(b);
+}
+
+int frsdg_batch_set_tile_shape(frsdg_batch* b, int n, int shape)
+{
+    if (!b || n < 2 || n > 9 || shape < 0 || shape > 2) return FRSDG_ERR_INVALID;
+    b->shape[n] = shape;
     return rebuild_tiles(b);
 }
 

@@ -265,6 +265,37 @@ namespace frsdg
         }
         return want_tx ? tx : ty;
     }
+    // element tile of order n, shape choice s: 0 the default tile, 1 the
+    // default tile halved (tile_halve with div 2), 2 the tile other than the
+    // default with the fewest rim faces per element, (tx + ty) / (tx ty), over
+    // tx >= ty, tx ty <= EPB (n = 7: 2 x 2 instead of 5 x 1; n = 5: 3 x 3
+    // instead of 5 x 2)
+    __host__ __device__ constexpr int batch_tile_dim(int n, int s, bool want_tx)
+    {
+        const int epb = (256 / (n * n)) > 0 ? 256 / (n * n) : 1;
+        const int ty0 = tile_rows(epb), tx0 = epb / ty0;
+        if (s == 1) return tile_halve(tx0, ty0, 2, want_tx);
+        if (s == 2)
+        {
+            int bx = tx0, by = ty0, num = -1, den = 1;
+            for (int ty = 1; ty * ty <= epb; ++ty)
+                for (int tx = ty; tx * ty <= epb; ++tx)
+                {
+                    if (tx == tx0 && ty == ty0) continue;
+                    // (tx + ty) / (tx ty) < num / den, ties to the larger tile
+                    const long a = static_cast<long>(tx + ty) * den, b = static_cast<long>(num) * tx * ty;
+                    if (num < 0 || a < b || (a == b && tx * ty > bx * by))
+                    {
+                        bx  = tx;
+                        by  = ty;
+                        num = tx + ty;
+                        den = tx * ty;
+                    }
+                }
+            return want_tx ? bx : by;
+        }
+        return want_tx ? tx0 : ty0;
+    }
     inline void tile_shape(int n, int div, int& tx, int& ty)
     {
         const int epb = (256 / (n * n)) > 0 ? 256 / (n * n) : 1;

@@ -397,6 +397,11 @@ int       frsdg_batch_set_dt(frsdg_batch* b, const double* dt);
  * whose stage states fit in about half of the last-level cache, each run taking its
  * three stages back to back) */
 int       frsdg_batch_set_tuning(frsdg_batch* b, int grid, int epb_div, int mode);
+/* element tile of the instances of order n = p+1 (after frsdg_batch_set_tuning,
+ * which resets every order to its epb_div choice): shape 0 the default tile of
+ * the order, 1 that tile halved, 2 the other tile with the fewest rim faces per
+ * element (n = 7: 2 x 2 elements instead of 5 x 1) */
+int       frsdg_batch_set_tile_shape(frsdg_batch* b, int n, int shape);
 int       frsdg_batch_ssprk_step(frsdg_batch* b, double* u, void* stream);
 /* nsteps steps replayed from a captured hipGraph */
 int       frsdg_batch_ssprk_run(frsdg_batch* b, double* u, int nsteps, void* stream);

@@ -434,6 +434,17 @@ def test_batched_sweep_matches_per_instance_and_oracle():
         b.ssprk_step_each(ub)
         # same tile code in a different kernel: equal up to FMA-contraction choices
         assert C.rel_l2(_host(ua), _host(ub)) <= 1e-14, (grid, div, mode)
+    # the other tile shape of every order (fewest rim faces per element), in
+    # the specialised, cache-chunked and all-orders launch modes
+    for grid, mode in ((0, 1), (64, 2), (0, 0)):
+        b.set_tuning(grid, 1, mode)
+        for n in b.orders():
+            b.set_tile_shape(n, 2)
+        ua.copy_(u)
+        b.ssprk_step(ua)
+        assert C.rel_l2(_host(ua), _host(ub)) <= 1e-14, (grid, mode, "shape 2")
+    with pytest.raises(fr.FrsdgError):
+        b.set_tile_shape(3, 3)
     for k, cfg in enumerate(cfgs):
         o = Oracle(cfg)
         ref = o.ssprk_step(u0[b.offsets[k]:b.offsets[k + 1]], dts[k])
