@@ -63,6 +63,7 @@ def parse():
     ap.add_argument("--adj-snapshots", type=int, default=32)
     ap.add_argument("--adj-n", type=int, default=1024)
     ap.add_argument("--kernel-only", choices=["step", "stage", "residual", "tangent", "cstep", "adjoint", "large",
+                                                "tangent_large", "adjoint_large",
                                                 "limiter", "copy", "vcopy", "batch", "batch_each", "ilu"],
                     default=None,
                     help="profiling mode: launch only this kernel `--steps` times, print nothing else")
@@ -357,7 +358,7 @@ def kernel_only(torch, fr, args):
         print(json.dumps({"kernel_only": "ilu", "applies": args.steps, "ndofs": ev.ndofs}))
         return
     k = args.kernel_only
-    if k in ("step", "large", "limiter", "copy"):
+    if k in ("step", "large", "limiter", "copy", "tangent_large", "adjoint_large"):
         ev, u, dt, cfg, _ = stepping_setup(torch, fr, "large")
         if k == "limiter":
             ev = fr.PhysicsEvaluator(dict(cfg, apply_filter=0, apply_weno=1))
@@ -374,6 +375,8 @@ def kernel_only(torch, fr, args):
         "tangent": lambda: ev.jvp(u, v, 1.0, out=out),
         "cstep": lambda: ev.jvp_cstep(u, v, 1.0, 1e-7, out=out),
         "adjoint": lambda: ev.vjp(u, v, 1.0, out=out),
+        "tangent_large": lambda: ev.jvp(u, v, 1.0, out=out),
+        "adjoint_large": lambda: ev.vjp(u, v, 1.0, out=out),
         "copy": lambda: out.copy_(u),
         "vcopy": lambda: out.copy_(u),
     }[k]
@@ -398,7 +401,17 @@ def bench_large(torch, fr, ev, u):
            "dof_rhs_per_s": ndof / t, "gbs_alg": bytes_alg / t / 1e9,
            "hbm_frac": bytes_alg / t / 1e9 / HBM_PEAK_GBS,
            "fp64_tflops": flops_per_element(n) * E / t / 1e12, "finite": ok}
-    del R
+    # derivative products on the same state: forward tangent (24 B/DOF + metrics),
+    # complex step, adjoint (u, w, out, lifted face cotangents: k_adj_lift + k_adj2d_tile)
+    v = torch.randn(ndof, dtype=torch.float64, device="cuda")
+    t_tan = time_kernel(torch, lambda: ev.jvp(u, v, 1.0, out=R), 10)
+    t_cs = time_kernel(torch, lambda: ev.jvp_cstep(u, v, 1.0, 1e-7, out=R), 10)
+    t_vjp = time_kernel(torch, lambda: ev.vjp(u, v, 1.0, out=R), 10)
+    res["derivatives"] = {"tangent_s": t_tan, "complex_step_s": t_cs, "adjoint_s": t_vjp,
+                          "tangent_over_residual": t_tan / t, "complex_step_over_residual": t_cs / t,
+                          "adjoint_over_residual": t_vjp / t,
+                          "tangent_gbs_alg": (bytes_alg + 8.0 * ndof) / t_tan / 1e9}
+    del R, v
     torch.cuda.empty_cache()
     # WENO limiter with shock sensor on the same state (limit_solution with
     # -apply_weno, no filter): smoothness of every element line, then limiting.
