@@ -17,7 +17,11 @@
 //   * the derivative-matrix contraction runs as line tasks (one element row
 //     or column, one equation per thread): the line is read from LDS once into
 //     registers, D comes through the scalar cache, and the metric-weighted
-//     derivatives of all n nodes of the line go back to LDS;
+//     derivatives of all n nodes of the line go back to LDS (curved double
+//     tiles: added by LDS atomics into one zeroed array, so that the line
+//     tasks need not wait for the face tasks; where the face tasks fit in two
+//     wavefronts, as at n = 8, those two evaluate the faces while the other
+//     two run the line tasks);
 //   * per node: volume term + the four VCJH lifts, written once; output
 //     policies fuse rhs = -R, F = udot + R and the SSP-RK stage update (the
 //     last stage of a step optionally applies the post-step spectral filter
@@ -27,7 +31,10 @@
 //     zeros); curved meshes read per-node metrics and per-face (nx, ny, scale)
 //     precomputed once in HBM;
 //   * block index -> tile is remapped so that each XCD walks a contiguous band
-//     of tiles (neighbour traces hit its own L2).
+//     of tiles, a vertical strip of the tile grid walked row by row (neighbour
+//     traces hit its own L2);
+//   * the output's extra operand (the stage's u^n, the ifunction's udot) is
+//     loaded before the last barrier, in flight across the lift phase.
 // The same tile code is instantiated on double (residual, stages) and Cplx
 // (complex step, frsdg_scalar.hpp).  The forward tangent J*v (k_tan2d) runs
 // the same tiles with forward-mode pairs only in the node and face fluxes.
@@ -105,20 +112,30 @@ namespace frsdg
         {
             out[i] = add ? add[i] + r : scale * r;
         }
-        __device__ void st4(long long i, const double*, const double r[4]) const
+        // the added operand of node i, loaded ahead of the last phase (st4p)
+        static constexpr bool pre = true;
+        __device__ void pre4(long long i, double z[4]) const
+        {
+            if (add) ld4d(add + i, z);
+        }
+        __device__ void st4p(long long i, const double*, const double r[4], const double z[4]) const
         {
             double o[4];
             if (add)
             {
-                double a[4];
-                ld4d(add + i, a);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) o[q] = a[q] + r[q];
+                for (int q = 0; q < 4; ++q) o[q] = z[q] + r[q];
             }
             else
 #pragma unroll
                 for (int q = 0; q < 4; ++q) o[q] = scale * r[q];
             st4d(out + i, o);
+        }
+        __device__ void st4(long long i, const double* own, const double r[4]) const
+        {
+            double z[4];
+            pre4(i, z);
+            st4p(i, own, r, z);
         }
     };
     // SSP-RK stage: out = a0*u0 + b*(us + cdt*(-R(us)))  (u0 read only if a0 != 0)
@@ -135,28 +152,46 @@ namespace frsdg
         __device__ void ld4(long long i, double o[4]) const { ld4d(u + i, o); }
         __device__ void st(long long i, double own, double r) const
         {
-            const double s = own + cdt * (-r);
-            out[i]         = (a0 != 0.0) ? a0 * u0[i] + b * s : b * s;
+            const double s = __builtin_fma(-cdt, r, own);
+            out[i]         = (a0 != 0.0) ? __builtin_fma(b, s, a0 * u0[i]) : b * s;
         }
-        __device__ void val4(long long i, const double own[4], const double r[4], double o[4]) const
+        // u0 of node i, loaded ahead of the last phase (val4p / st4p)
+        static constexpr bool pre = true;
+        __device__ void pre4(long long i, double z[4]) const
+        {
+            if (a0 != 0.0) ld4d(u0 + i, z);
+        }
+        // the contractions are written out, so that every kernel that stores
+        // a stage rounds it the same way
+        __device__ void val4p(const double own[4], const double r[4], const double z[4], double o[4]) const
         {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) o[q] = own[q] + cdt * (-r[q]);
+            for (int q = 0; q < 4; ++q) o[q] = __builtin_fma(-cdt, r[q], own[q]);
             if (a0 != 0.0)
             {
-                double z[4];
-                ld4d(u0 + i, z);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) o[q] = a0 * z[q] + b * o[q];
+                for (int q = 0; q < 4; ++q) o[q] = __builtin_fma(b, o[q], a0 * z[q]);
             }
             else
 #pragma unroll
                 for (int q = 0; q < 4; ++q) o[q] = b * o[q];
         }
+        __device__ void val4(long long i, const double own[4], const double r[4], double o[4]) const
+        {
+            double z[4];
+            pre4(i, z);
+            val4p(own, r, z, o);
+        }
         __device__ void put4(long long i, const double o[4]) const
         {
             if (health) health_check(o, health);
             st4d(out + i, o);
+        }
+        __device__ void st4p(long long i, const double own[4], const double r[4], const double z[4]) const
+        {
+            double o[4];
+            val4p(own, r, z, o);
+            put4(i, o);
         }
         __device__ void st4(long long i, const double own[4], const double r[4]) const
         {
@@ -170,6 +205,7 @@ namespace frsdg
     {
         using S = Dual;
         static constexpr bool can_filter = false;
+        static constexpr bool pre        = false;
         const double* __restrict__ u;
         const double* __restrict__ v;
         double* __restrict__ out;
@@ -197,6 +233,7 @@ namespace frsdg
     {
         using S = Cplx;
         static constexpr bool can_filter = false;
+        static constexpr bool pre        = false;
         const double* __restrict__ u;
         const double* __restrict__ v;
         double* __restrict__ out;
@@ -332,10 +369,10 @@ namespace frsdg
         static constexpr int RP = N * 4 + ((RB % 64) == 0 ? 32 / static_cast<int>(sizeof(S)) : 0);
         // position of node k, equation q in row l
         __device__ static __forceinline__ int pos(int l, int k, int q) { return 4 * k + (q ^ (l & 3)); }
-        S         U[EPB][N][RP]; // state; curved meshes: xi-line part of the volume term after the face phase
+        S         U[EPB][N][RP]; // state; complex-step curved tiles: xi-line part of the volume term after the face phase
         S         F[EPB][N][RP]; // affine meshes: xi-line part of the volume term after the line phase
         S         G[EPB][N][RP]; // affine meshes: eta-line part of the volume term after the line phase
-        S         V[CURVED ? EPB : 1][N][CURVED ? N * 4 : 1]; // curved meshes: eta-line part
+        S         V[CURVED ? EPB : 1][N][CURVED ? N * 4 : 1]; // curved: eta-line part (double: xi + eta part)
         S         Del[EPB][4][N][4];
         double    dL[N], dR[N];
         long long eid[EPB];      // global element of each tile slot, -1 if outside the mesh
@@ -360,9 +397,13 @@ namespace frsdg
         }
     }
 
-    // Tile origin and shape.  Tiles cover the element grid row-major; the
-    // last tile of a row or column may hang over the mesh (its outside slots
-    // are idle).
+    // Tile origin and shape.  Tiles are numbered strip by strip: the tile
+    // columns form eight vertical strips of ceil(ntx / 8) tiles (the last one
+    // narrower), each walked row-major, so that the band of tiles one XCD
+    // processes (xcd_tile) is one strip and the tile above a tile follows it
+    // a strip width later, while its bottom trace is still in that XCD's L2.
+    // The last tile of a row or column may hang over the mesh (its outside
+    // slots are idle).
     struct TileGeom
     {
         int i0, j0, tx, ty;
@@ -370,9 +411,14 @@ namespace frsdg
     __device__ __forceinline__ TileGeom tile_geom(const Res2DP& P, long long blk, int tx, int ty)
     {
         const int ntx = (P.nx + tx - 1) / tx;
+        const int nty = (P.ny + ty - 1) / ty;
+        const int sw  = (ntx + 7) / 8;
         const int b   = static_cast<int>(blk);
-        const int tj  = b / ntx;
-        return {(b - tj * ntx) * tx, tj * ty, tx, ty};
+        const int st  = b / (sw * nty);
+        const int w   = min(sw, ntx - st * sw);
+        const int r   = b - st * sw * nty;
+        const int tj  = r / w;
+        return {(st * sw + r - tj * w) * tx, tj * ty, tx, ty};
     }
 
     // One face-node task of a tile.  Each face of the tile's elements is
@@ -615,7 +661,10 @@ namespace frsdg
     // The line is held in registers; D comes through the scalar cache, the
     // metric pair (ma, mb) of the line's nodes from registers.  Affine meshes
     // keep only d/dxi F and d/deta G (the cross terms are exact zeros).
-    template <int N, class S, bool CURVED, class SM>
+    //   With ACC (curved, double) both parts are added into V, zeroed per node
+    //   beforehand, by LDS atomic adds: V = fl(xi + eta) whichever lands first,
+    //   and U keeps the state, so the line tasks can run beside the face tasks.
+    template <int N, class S, bool CURVED, class SM, bool ACC = false>
     __device__ __forceinline__ void volume_line(const Res2DP& P, SM& sm, bool eta, int le,
                                                 int ln, int q, const double* ma = nullptr, const double* mb = nullptr)
     {
@@ -641,7 +690,9 @@ namespace frsdg
                 }
                 // affine: the result replaces row ln of F (equation q), which
                 // only this task reads in this phase
-                if constexpr (CURVED)
+                if constexpr (ACC)
+                    atomicAdd(&sm.V[le][ln][sm.pos(ln, k, q)], ma[k] * dF + mb[k] * dG);
+                else if constexpr (CURVED)
                     sm.U[le][ln][sm.pos(ln, k, q)] = ma[k] * dF + mb[k] * dG;
                 else
                     sm.F[le][ln][sm.pos(ln, k, q)] = P.xi_x * dF;
@@ -668,7 +719,9 @@ namespace frsdg
                 }
                 // affine: the result replaces column ln of G (equation q), which
                 // only this task reads in this phase
-                if constexpr (CURVED)
+                if constexpr (ACC)
+                    atomicAdd(&sm.V[le][l][sm.pos(l, ln, q)], ma[l] * dF + mb[l] * dG);
+                else if constexpr (CURVED)
                     sm.V[le][l][sm.pos(l, ln, q)] = ma[l] * dF + mb[l] * dG;
                 else
                     sm.G[le][l][sm.pos(l, ln, q)] = P.eta_y * dG;
@@ -715,6 +768,8 @@ namespace frsdg
         using S           = typename IO::S;
         constexpr int BS  = Tile2D<N, S>::BS;
         constexpr int NN  = N * N;
+        // curved double tiles add the xi- and eta-parts of the volume term into V
+        constexpr bool ACC = CURVED && std::is_same_v<S, double>;
         const int     tid = threadIdx.x;
         if (tid < N)
         {
@@ -735,17 +790,26 @@ namespace frsdg
         FaceTask<S> f0;
         face_task<N, IO>(P, io, g, tid, f0);
 
+        // Task split of the accumulating (ACC) tiles whose face tasks fit in
+        // the first half of the workgroup and whose line tasks fit twice in
+        // the second half (n = 8: 96 face tasks, 256 line tasks): waves 0-1
+        // evaluate the faces, waves 2-3 the lines, two each, concurrently.
+        // Otherwise every thread takes a face task (if any) and then a line task.
+        const int  nlt   = tx * ty * N * 4;
+        const int  nft   = ((tx + 1) * ty + tx * (ty + 1)) * N;
+        const bool split = ACC && BS == 256 && nft <= BS / 2 && 2 * nlt <= BS;
+        const int  lt0   = split ? tid - BS / 2 : tid; // first line task of this thread (< 0: none)
+        const int  lstep = split ? BS / 2 : BS;
         // curved: metrics of this thread's first line task, in flight with the state
-        const int nlt = tx * ty * N * 4;
-        double    ma[CURVED ? N : 1], mb[CURVED ? N : 1];
+        double ma[CURVED ? N : 1], mb[CURVED ? N : 1];
         if constexpr (CURVED)
         {
-            const bool eta = tid >= nlt;
-            const int  w   = eta ? tid - nlt : tid;
+            const bool eta = lt0 >= nlt;
+            const int  w   = eta ? lt0 - nlt : lt0;
             const int  le  = (w >> 2) / N;
             const int  lb  = le / tx;
             const int  la  = le - lb * tx;
-            if (tid < 2 * nlt && g.i0 + la < P.nx && g.j0 + lb < P.ny)
+            if (lt0 >= 0 && lt0 < 2 * nlt && g.i0 + la < P.nx && g.j0 + lb < P.ny)
                 line_metrics<N>(P.vmet, static_cast<long long>(g.j0 + lb) * P.nx + (g.i0 + la), eta,
                                 (w >> 2) - le * N, ma, mb);
         }
@@ -762,29 +826,55 @@ namespace frsdg
                 sm.U[el][l][sm.pos(l, k, q)] = own[q];
                 sm.F[el][l][sm.pos(l, k, q)] = F[q];
                 sm.G[el][l][sm.pos(l, k, q)] = G[q];
+                if constexpr (ACC) sm.V[el][l][sm.pos(l, k, q)] = 0.0;
             }
         }
         __syncthreads();
 
         // ---- face tasks: every face of the tile once ----
         if (f0.live) face_flux<N, IO, CURVED>(P, f0, sm);
-        const int nft = ((tx + 1) * ty + tx * (ty + 1)) * N;
         for (int t = tid + BS; t < nft; t += BS)
         {
             FaceTask<S> f;
             face_task<N, IO>(P, io, g, t, f);
             if (f.live) face_flux<N, IO, CURVED>(P, f, sm);
         }
-        // curved line tasks write the xi-part over U, which face tasks read;
-        // affine line tasks work in place on F and G, which face tasks do not read
-        if constexpr (CURVED) __syncthreads();
+        // complex-step curved line tasks write the xi-part over U, which face
+        // tasks read; double curved line tasks add into V and affine line tasks
+        // work in place on F and G, neither of which face tasks read
+        if constexpr (CURVED && !ACC) __syncthreads();
 
         // ---- volume term by lines (q fastest).  Curved: xi-line tasks, then
         // eta-line tasks (4 contractions per node); affine: one task does
         // xi-line ln of F and eta-line ln of G (2 contractions per node) ----
+        if (split)
+        {
+            // waves 2-3: line tasks lt0 and lt0 + BS/2, the second one's
+            // metrics loaded while the first one computes
+            if constexpr (ACC)
+            {
+                const int t1 = lt0 + BS / 2;
+                double    mc[N], md[N];
+                const bool eta1 = t1 >= nlt;
+                const int  w1   = eta1 ? t1 - nlt : t1;
+                const int  le1  = (w1 >> 2) / N;
+                const bool has1 = lt0 >= 0 && t1 < 2 * nlt && sm.eid[le1] >= 0;
+                if (has1) line_metrics<N>(P.vmet, sm.eid[le1], eta1, (w1 >> 2) - le1 * N, mc, md);
+                if (lt0 >= 0 && lt0 < 2 * nlt)
+                {
+                    const bool eta = lt0 >= nlt;
+                    const int  w   = eta ? lt0 - nlt : lt0;
+                    const int  le  = (w >> 2) / N;
+                    if (sm.eid[le] >= 0)
+                        volume_line<N, S, CURVED, decltype(sm), ACC>(P, sm, eta, le, (w >> 2) - le * N, w & 3, ma, mb);
+                }
+                if (has1) volume_line<N, S, CURVED, decltype(sm), ACC>(P, sm, eta1, le1, (w1 >> 2) - le1 * N, w1 & 3, mc, md);
+            }
+        }
+        else
         {
             const int nt = CURVED ? 2 * nlt : nlt;
-            for (int t = tid; t < nt; t += BS)
+            for (int t = lt0; t < nt; t += lstep)
             {
                 const bool eta = t >= nlt;
                 const int  w   = eta ? t - nlt : t;
@@ -794,8 +884,8 @@ namespace frsdg
                 if (sm.eid[le] < 0) continue;
                 if constexpr (CURVED)
                 {
-                    if (t >= BS) line_metrics<N>(P.vmet, sm.eid[le], eta, ln, ma, mb); // later passes
-                    volume_line<N, S, CURVED>(P, sm, eta, le, ln, q, ma, mb);
+                    if (t >= lstep) line_metrics<N>(P.vmet, sm.eid[le], eta, ln, ma, mb); // later passes
+                    volume_line<N, S, CURVED, decltype(sm), ACC>(P, sm, eta, le, ln, q, ma, mb);
                 }
                 else
                 {
@@ -803,6 +893,13 @@ namespace frsdg
                     volume_line<N, S, CURVED>(P, sm, true, le, ln, q);
                 }
             }
+        }
+        // the output's extra operand (stage u0, ifunction udot) of this node,
+        // in flight across the barrier and the lift phase
+        double z[4];
+        if constexpr (IO::pre)
+        {
+            if (act) io.pre4((e * NN + node) * 4, z);
         }
         __syncthreads();
 
@@ -812,7 +909,9 @@ namespace frsdg
 #pragma unroll
             for (int q = 0; q < 4; ++q)
             {
-                if constexpr (CURVED)
+                if constexpr (ACC)
+                    r[q] = sm.V[el][l][sm.pos(l, k, q)];
+                else if constexpr (CURVED)
                     r[q] = sm.U[el][l][sm.pos(l, k, q)] + sm.V[el][l][sm.pos(l, k, q)];
                 else
                     r[q] = sm.F[el][l][sm.pos(l, k, q)] + sm.G[el][l][sm.pos(l, k, q)];
@@ -839,7 +938,7 @@ namespace frsdg
                 if (act)
                 {
                     double o[4];
-                    io.val4((e * NN + node) * 4, own, r, o);
+                    io.val4p(own, r, z, o);
 #pragma unroll
                     for (int q = 0; q < 4; ++q) sm.U[el][l][sm.pos(l, k, q)] = o[q];
                 }
@@ -863,7 +962,13 @@ namespace frsdg
                 return;
             }
         }
-        if (act) io.st4((e * NN + node) * 4, own, r);
+        if (act)
+        {
+            if constexpr (IO::pre)
+                io.st4p((e * NN + node) * 4, own, r, z);
+            else
+                io.st4((e * NN + node) * 4, own, r);
+        }
     }
 
     // Waves per SIMD that the LDS of one curved tile allows (160 KB per CU,

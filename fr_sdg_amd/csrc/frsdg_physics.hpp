@@ -46,7 +46,7 @@ namespace frsdg
         if (g.kind == 0) return (g.gamma - 1.0) * rho * e;
         const S covol = 1.0 - g.b * rho;
         const S cvT   = e + g.a * rho;
-        return ((g.gamma - 1.0) * rho * cvT) / covol - g.a * rho * rho;
+        return quot_((g.gamma - 1.0) * rho * cvT, covol) - g.a * rho * rho;
     }
     // sound speed (ideal: sqrt(gamma p / rho) with p passed in)
     template <class S>
@@ -54,7 +54,7 @@ namespace frsdg
     {
         if (g.kind == 0) return sqrt_(g.gamma * p * inv_rho);
         const S covol = 1.0 - g.b * rho;
-        const S inv_d = 1.0 / (covol * covol);
+        const S inv_d = rcp_(covol * covol);
         const S th    = e + g.a * rho;
         const S t1    = (g.gamma * (g.gamma - 1.0) * th) * inv_d;
         const S t2    = ((g.gamma - 1.0) * g.a * rho * (2.0 - g.b * rho)) * inv_d;
@@ -73,7 +73,7 @@ namespace frsdg
     {
         Prim2<S> q;
         q.rho        = Sc<S>::wrap_positive(U[0]);
-        q.inv_rho    = 1.0 / q.rho;
+        q.inv_rho    = rcp_(q.rho);
         q.u          = U[1] * q.inv_rho;
         q.v          = U[2] * q.inv_rho;
         const S totE = Sc<S>::wrap_positive(U[3]);
@@ -160,7 +160,7 @@ namespace frsdg
             const S sR  = smooth_max(unL + cL, unR + cR);
             const S rLd = UL[0] * (sL - unL);
             const S rRd = UR[0] * (sR - unR);
-            const S sM  = (R.p - L.p + unL * rLd - unR * rRd) / (rLd - rRd);
+            const S sM  = quot_(R.p - L.p + unL * rLd - unR * rRd, rLd - rRd);
             // Branch order of the reference: sL >= 0 -> F_L; sM >= 0 -> left
             // star state; sR >= 0 -> right star state; else F_R.  The star
             // state of the selected side is formed once with selects (no
@@ -179,7 +179,7 @@ namespace frsdg
 #pragma unroll
             for (int q = 0; q < 4; ++q) FK[q] = Sc<S>::sel(left, FL[q], FR[q]);
             const S ps  = pK + rK * (sK - unK) * (sM - unK);
-            const S inv = 1.0 / (sK - sM);
+            const S inv = rcp_(sK - sM);
             S       st[4];
             st[0] = (sM * (sK * rK - FK[0])) * inv;
             st[1] = (sM * (sK * mK - FK[1]) + sK * ps) * inv;

@@ -30,7 +30,7 @@ namespace frsdg
         t.U3       = U[3];
         t.rho_live = DBL_MIN < U[0];
         t.rho      = t.rho_live ? U[0] : DBL_MIN;
-        t.ir       = 1.0 / t.rho;
+        t.ir       = rcp_(t.rho);
         t.u        = U[1] * t.ir;
         t.v        = U[2] * t.ir;
         t.E_live   = DBL_MIN < U[3];
@@ -188,7 +188,7 @@ namespace frsdg
             const double rLd = UL[0] * (sL - unL);
             const double rRd = UR[0] * (sR - unR);
             const double num = R.p - L.p + unL * rLd - unR * rRd, den = rLd - rRd;
-            const double sM  = num / den;
+            const double sM  = quot_(num, den);
             const bool   supL = sL >= 0.0;
             const bool   left = sM >= 0.0;
             const bool   supR = !left && !(sR >= 0.0);
@@ -216,7 +216,7 @@ namespace frsdg
                 const double FK1 = mK * unK + pK;
                 const double FK3 = (EK + pK) * unK;
                 const double ps  = pK + rK * (sK - unK) * (sM - unK);
-                const double inv = 1.0 / (sK - sM);
+                const double inv = rcp_(sK - sM);
                 const double A0 = sK * rK - FK0, A1 = sK * mK - FK1, A3 = sK * EK - FK3;
                 const double st0 = (sM * A0) * inv;
                 const double B1  = sM * A1 + sK * ps, B3 = sM * A3 + sK * ps * sM;

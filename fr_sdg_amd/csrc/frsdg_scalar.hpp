@@ -132,6 +132,37 @@ namespace frsdg
         static FR_HD Cplx frozen(Cplx x) { return {x.re, 0.0}; }
         static FR_HD Cplx sel(bool k, Cplx a, Cplx b) { return {k ? a.re : b.re, k ? a.im : b.im}; }
     };
+    // Reciprocal and quotient of doubles from the hardware reciprocal estimate,
+    // refined by two Newton steps (r <- r + r (1 - x r)) and, for the
+    // quotient, one residual correction q <- q + r (a - b q): within one unit
+    // in the last place of the correctly rounded result for normal operands,
+    // without the range scaling of the IEEE divide sequence (a zero or
+    // infinite divisor gives NaN).  Dual and Cplx divide as written.
+    FR_HD double rcp_(double x)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        double r = __builtin_amdgcn_rcp(x);
+        r        = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+        r        = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+        return r;
+#else
+        return 1.0 / x;
+#endif
+    }
+    FR_HD double quot_(double a, double b)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        const double r = rcp_(b);
+        const double q = a * r;
+        return __builtin_fma(__builtin_fma(-b, q, a), r, q);
+#else
+        return a / b;
+#endif
+    }
+    FR_HD Dual rcp_(Dual x) { return 1.0 / x; }
+    FR_HD Cplx rcp_(Cplx x) { return 1.0 / x; }
+    FR_HD Dual quot_(Dual a, Dual b) { return a / b; }
+    FR_HD Cplx quot_(Cplx a, Cplx b) { return a / b; }
     FR_HD double sqrt_(double x) { return ::sqrt(x); }
     FR_HD Dual sqrt_(Dual x) { return sqrt(x); }
     FR_HD Cplx sqrt_(Cplx x) { return sqrt(x); }
