@@ -779,6 +779,19 @@ int frsdg_create(const frsdg_desc* d, frsdg_ctx** out)
     tab.insert(tab.end(), c->tb.dL.begin(), c->tb.dL.end());
     tab.insert(tab.end(), c->tb.dR.begin(), c->tb.dR.end());
     tab.insert(tab.end(), c->tb.filt.begin(), c->tb.filt.end());
+    // even-odd halves of D and of the filter for the line kernels (h = n / 2):
+    // A[k][m] = (M[k][m] + M[n-1-k][m]) / 2, B[k][m] = (M[k][m] - M[n-1-k][m]) / 2
+    {
+        const int h = n / 2;
+        for (const std::vector<double>* M : {&c->tb.D, &c->tb.filt})
+            for (int sgn = 0; sgn < 2; ++sgn)
+                for (int k = 0; k < h; ++k)
+                    for (int m = 0; m < h; ++m)
+                    {
+                        const double x = (*M)[k * n + m], y = (*M)[(n - 1 - k) * n + m];
+                        tab.push_back(0.5 * (sgn == 0 ? x + y : x - y));
+                    }
+    }
     int rc = FRSDG_OK;
     auto fail = [&](int code)
     {

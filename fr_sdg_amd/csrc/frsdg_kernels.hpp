@@ -654,6 +654,92 @@ namespace frsdg
         }
     }
 
+    // Even-odd product of an n x n line operator M whose rows mirror with
+    // M[n-1-k][n-1-m] = -M[k][m] (ANTI: the derivative matrix on the symmetric
+    // GLL nodes) or +M[k][m] (the filter): with the halves A, B of the table
+    // (A[k][m] = (M[k][m] + M[n-1-k][m]) / 2, B[k][m] = (M[k][m] - M[n-1-k][m]) / 2,
+    // k, m < n/2) and the mirrored sums e[m] = x[m] + x[n-1-m], differences
+    // d[m] = x[m] - x[n-1-m]:
+    //   y[k] = A p + B q,  y[n-1-k] = A p - B q,  (p, q) = (d, e) if ANTI else (e, d),
+    // n^2 / 2 + 2n operations instead of n^2 (even n).
+    template <int N, class S, bool ANTI>
+    __device__ __forceinline__ void eo_line(const double* __restrict__ A, const double* __restrict__ B, const S x[N],
+                                            S y[N])
+    {
+        constexpr int H = N / 2;
+        S             p[H], q[H];
+#pragma unroll
+        for (int m = 0; m < H; ++m)
+        {
+            const S e = x[m] + x[N - 1 - m], d = x[m] - x[N - 1 - m];
+            p[m]      = ANTI ? d : e;
+            q[m]      = ANTI ? e : d;
+        }
+#pragma unroll
+        for (int k = 0; k < H; ++k)
+        {
+            S a = A[k * H] * p[0], b = B[k * H] * q[0];
+#pragma unroll
+            for (int m = 1; m < H; ++m)
+            {
+                a = a + A[k * H + m] * p[m];
+                b = b + B[k * H + m] * q[m];
+            }
+            y[k]         = a + b;
+            y[N - 1 - k] = a - b;
+        }
+    }
+    // offsets of the even-odd halves in the table: D's A, B, then the filter's
+    template <int N>
+    __device__ __forceinline__ const double* eo_tab(const double* tab, int which)
+    {
+        return tab + 2 * N * N + 2 * N + which * (N / 2) * (N / 2);
+    }
+
+    // Volume term of one line by the even-odd contraction (eo_line) of F and
+    // G, for the curved double tiles that add both parts into V: the mirrored
+    // outputs k, n-1-k are formed and added together (even n).
+    template <int N, bool ETA, class SM>
+    __device__ __forceinline__ void volume_line_eo(const Res2DP& P, SM& sm, int le, int ln, int q,
+                                                   const double* ma, const double* mb)
+    {
+        constexpr int H  = N / 2;
+        const double* DA = eo_tab<N>(P.tab, 0);
+        const double* DB = eo_tab<N>(P.tab, 1);
+        double        pf[H], qf[H], pg[H], qg[H];
+#pragma unroll
+        for (int m = 0; m < H; ++m)
+        {
+            const int    l0 = ETA ? m : ln, k0 = ETA ? ln : m;
+            const int    l1 = ETA ? N - 1 - m : ln, k1 = ETA ? ln : N - 1 - m;
+            const double f0 = sm.F[le][l0][sm.pos(l0, k0, q)], f1 = sm.F[le][l1][sm.pos(l1, k1, q)];
+            const double g0 = sm.G[le][l0][sm.pos(l0, k0, q)], g1 = sm.G[le][l1][sm.pos(l1, k1, q)];
+            pf[m]           = f0 - f1;
+            qf[m]           = f0 + f1;
+            pg[m]           = g0 - g1;
+            qg[m]           = g0 + g1;
+        }
+#pragma unroll
+        for (int k = 0; k < H; ++k)
+        {
+            double af = DA[k * H] * pf[0], bf = DB[k * H] * qf[0];
+            double ag = DA[k * H] * pg[0], bg = DB[k * H] * qg[0];
+#pragma unroll
+            for (int m = 1; m < H; ++m)
+            {
+                af = af + DA[k * H + m] * pf[m];
+                bf = bf + DB[k * H + m] * qf[m];
+                ag = ag + DA[k * H + m] * pg[m];
+                bg = bg + DB[k * H + m] * qg[m];
+            }
+            const int k1 = N - 1 - k;
+            const int la = ETA ? k : ln, ka = ETA ? ln : k;
+            const int lb = ETA ? k1 : ln, kb = ETA ? ln : k1;
+            atomicAdd(&sm.V[le][la][sm.pos(la, ka, q)], ma[k] * (af + bf) + mb[k] * (ag + bg));
+            atomicAdd(&sm.V[le][lb][sm.pos(lb, kb, q)], ma[k1] * (af - bf) + mb[k1] * (ag - bg));
+        }
+    }
+
     // Volume term along one line of one element for equation q.
     //   xi-line  (row l):    U[l][k] <- xi_x d/dxi F + xi_y d/dxi G   (k = 0..N-1)
     //   eta-line (column k): V[l][k] <- eta_x d/deta F + eta_y d/deta G (l = 0..N-1)
@@ -669,6 +755,14 @@ namespace frsdg
                                                 int ln, int q, const double* ma = nullptr, const double* mb = nullptr)
     {
         const double* D = P.tab;
+        if constexpr (ACC && N % 2 == 0)
+        {
+            if (eta)
+                volume_line_eo<N, true>(P, sm, le, ln, q, ma, mb);
+            else
+                volume_line_eo<N, false>(P, sm, le, ln, q, ma, mb);
+            return;
+        }
         if (!eta)
         {
             S f[N], g[N];
@@ -740,6 +834,20 @@ namespace frsdg
         double        x[N];
 #pragma unroll
         for (int c = 0; c < N; ++c) x[c] = eta ? sm.U[le][c][sm.pos(c, ln, q)] : sm.U[le][ln][sm.pos(ln, c, q)];
+        if constexpr (N % 2 == 0)
+        {
+            double y[N];
+            eo_line<N, double, false>(eo_tab<N>(P.tab, 2), eo_tab<N>(P.tab, 3), x, y);
+#pragma unroll
+            for (int r = 0; r < N; ++r)
+            {
+                if (eta)
+                    sm.G[le][r][sm.pos(r, ln, q)] = y[r];
+                else
+                    sm.F[le][ln][sm.pos(ln, r, q)] = y[r];
+            }
+            return;
+        }
 #pragma unroll
         for (int r = 0; r < N; ++r)
         {
@@ -849,8 +957,9 @@ namespace frsdg
         // xi-line ln of F and eta-line ln of G (2 contractions per node) ----
         if (split)
         {
-            // waves 2-3: line tasks lt0 and lt0 + BS/2, the second one's
-            // metrics loaded while the first one computes
+            // waves 2-3: line tasks lt0 and lt0 + BS/2 (the second one's
+            // metrics loaded after the first one: held across it, they cost
+            // spills at n = 8)
             if constexpr (ACC)
             {
                 const int t1 = lt0 + BS / 2;
@@ -859,7 +968,6 @@ namespace frsdg
                 const int  w1   = eta1 ? t1 - nlt : t1;
                 const int  le1  = (w1 >> 2) / N;
                 const bool has1 = lt0 >= 0 && t1 < 2 * nlt && sm.eid[le1] >= 0;
-                if (has1) line_metrics<N>(P.vmet, sm.eid[le1], eta1, (w1 >> 2) - le1 * N, mc, md);
                 if (lt0 >= 0 && lt0 < 2 * nlt)
                 {
                     const bool eta = lt0 >= nlt;
@@ -868,6 +976,7 @@ namespace frsdg
                     if (sm.eid[le] >= 0)
                         volume_line<N, S, CURVED, decltype(sm), ACC>(P, sm, eta, le, (w >> 2) - le * N, w & 3, ma, mb);
                 }
+                if (has1) line_metrics<N>(P.vmet, sm.eid[le1], eta1, (w1 >> 2) - le1 * N, mc, md);
                 if (has1) volume_line<N, S, CURVED, decltype(sm), ACC>(P, sm, eta1, le1, (w1 >> 2) - le1 * N, w1 & 3, mc, md);
             }
         }
@@ -906,6 +1015,12 @@ namespace frsdg
         S r[4];
         if (act)
         {
+            // ACC tiles keep the state in U to the end: reread, not held in registers
+            if constexpr (ACC)
+            {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) own[q] = sm.U[el][l][sm.pos(l, k, q)];
+            }
 #pragma unroll
             for (int q = 0; q < 4; ++q)
             {

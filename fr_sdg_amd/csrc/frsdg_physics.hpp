@@ -39,22 +39,25 @@ namespace frsdg
         return 0.5 * (a + b - smooth_abs(a - b));
     }
 
-    // pressure from density and specific internal energy
+    // pressure from density and specific internal energy.  Van der Waals:
+    // the reciprocal of the co-volume factor 1 - b rho is formed once per
+    // state; the sound speed below forms the same one (one reciprocal
+    // after common-subexpression elimination where both are evaluated)
     template <class S>
     FR_HD S gas_pressure(const GasP& g, S rho, S e)
     {
         if (g.kind == 0) return (g.gamma - 1.0) * rho * e;
-        const S covol = 1.0 - g.b * rho;
-        const S cvT   = e + g.a * rho;
-        return quot_((g.gamma - 1.0) * rho * cvT, covol) - g.a * rho * rho;
+        const S rcov = rcp_(1.0 - g.b * rho);
+        const S cvT  = e + g.a * rho;
+        return ((g.gamma - 1.0) * rho * cvT) * rcov - g.a * rho * rho;
     }
     // sound speed (ideal: sqrt(gamma p / rho) with p passed in)
     template <class S>
     FR_HD S gas_sound(const GasP& g, S rho, S e, S p, S inv_rho)
     {
         if (g.kind == 0) return sqrt_(g.gamma * p * inv_rho);
-        const S covol = 1.0 - g.b * rho;
-        const S inv_d = rcp_(covol * covol);
+        const S rcov  = rcp_(1.0 - g.b * rho);
+        const S inv_d = rcov * rcov;
         const S th    = e + g.a * rho;
         const S t1    = (g.gamma * (g.gamma - 1.0) * th) * inv_d;
         const S t2    = ((g.gamma - 1.0) * g.a * rho * (2.0 - g.b * rho)) * inv_d;

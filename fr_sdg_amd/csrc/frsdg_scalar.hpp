@@ -23,6 +23,54 @@ namespace frsdg
         double re, im;
     };
 
+    // Reciprocal and quotient of doubles from the hardware reciprocal estimate,
+    // refined by two Newton steps (r <- r + r (1 - x r)) and, for the
+    // quotient, one residual correction q <- q + r (a - b q): within one unit
+    // in the last place of the correctly rounded result for normal operands,
+    // without the range scaling of the IEEE divide sequence (a zero or
+    // infinite divisor gives NaN).  Dual forms its value part with them (the
+    // same primal as the double kernels); Cplx divides as written.
+    FR_HD double rcp_(double x)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        double r = __builtin_amdgcn_rcp(x);
+        r        = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+        r        = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+        return r;
+#else
+        return 1.0 / x;
+#endif
+    }
+    FR_HD double quot_(double a, double b)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        const double r = rcp_(b);
+        const double q = a * r;
+        return __builtin_fma(__builtin_fma(-b, q, a), r, q);
+#else
+        return a / b;
+#endif
+    }
+    // Square root of a double from the hardware reciprocal-square-root
+    // estimate y: g = x y, h = y / 2, one Goldschmidt step, then two residual
+    // corrections g <- g + h (x - g^2); within one unit in the last place for
+    // positive normal x, +-0 kept, NaN for x < 0 (no range scaling: infinite
+    // and subnormal x are outside its domain).
+    FR_HD double sqrt_(double x)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        const double y = __builtin_amdgcn_rsq(x);
+        double       g = x * y, h = 0.5 * y;
+        const double r = __builtin_fma(-g, h, 0.5);
+        g              = __builtin_fma(g, r, g);
+        h              = __builtin_fma(h, r, h);
+        g              = __builtin_fma(__builtin_fma(-g, g, x), h, g);
+        g              = __builtin_fma(__builtin_fma(-g, g, x), h, g);
+        return (x == 0.0) ? x : g;
+#else
+        return ::sqrt(x);
+#endif
+    }
     // ---------------- Dual ----------------
     FR_HD Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
     FR_HD Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
@@ -47,7 +95,7 @@ namespace frsdg
     }
     FR_HD Dual sqrt(Dual a)
     {
-        const double s = ::sqrt(a.v);
+        const double s = sqrt_(a.v);
         return {s, a.d / (2.0 * s)};
     }
 
@@ -132,38 +180,20 @@ namespace frsdg
         static FR_HD Cplx frozen(Cplx x) { return {x.re, 0.0}; }
         static FR_HD Cplx sel(bool k, Cplx a, Cplx b) { return {k ? a.re : b.re, k ? a.im : b.im}; }
     };
-    // Reciprocal and quotient of doubles from the hardware reciprocal estimate,
-    // refined by two Newton steps (r <- r + r (1 - x r)) and, for the
-    // quotient, one residual correction q <- q + r (a - b q): within one unit
-    // in the last place of the correctly rounded result for normal operands,
-    // without the range scaling of the IEEE divide sequence (a zero or
-    // infinite divisor gives NaN).  Dual and Cplx divide as written.
-    FR_HD double rcp_(double x)
+    // Dual: the value part as for doubles (the same primal as the residual)
+    FR_HD Dual rcp_(Dual x)
     {
-#if defined(__HIP_DEVICE_COMPILE__)
-        double r = __builtin_amdgcn_rcp(x);
-        r        = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
-        r        = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
-        return r;
-#else
-        return 1.0 / x;
-#endif
+        const double r = rcp_(x.v);
+        return {r, -(r * r) * x.d};
     }
-    FR_HD double quot_(double a, double b)
+    FR_HD Dual quot_(Dual a, Dual b)
     {
-#if defined(__HIP_DEVICE_COMPILE__)
-        const double r = rcp_(b);
-        const double q = a * r;
-        return __builtin_fma(__builtin_fma(-b, q, a), r, q);
-#else
-        return a / b;
-#endif
+        const double r = rcp_(b.v);
+        const double q = quot_(a.v, b.v);
+        return {q, (a.d - q * b.d) * r};
     }
-    FR_HD Dual rcp_(Dual x) { return 1.0 / x; }
     FR_HD Cplx rcp_(Cplx x) { return 1.0 / x; }
-    FR_HD Dual quot_(Dual a, Dual b) { return a / b; }
     FR_HD Cplx quot_(Cplx a, Cplx b) { return a / b; }
-    FR_HD double sqrt_(double x) { return ::sqrt(x); }
     FR_HD Dual sqrt_(Dual x) { return sqrt(x); }
     FR_HD Cplx sqrt_(Cplx x) { return sqrt(x); }
     // exponential (change of variables of the log-variable implicit path)
