@@ -779,11 +779,14 @@ int frsdg_create(const frsdg_desc* d, frsdg_ctx** out)
     tab.insert(tab.end(), c->tb.dL.begin(), c->tb.dL.end());
     tab.insert(tab.end(), c->tb.dR.begin(), c->tb.dR.end());
     tab.insert(tab.end(), c->tb.filt.begin(), c->tb.filt.end());
-    // even-odd halves of D and of the filter for the line kernels (h = n / 2):
-    // A[k][m] = (M[k][m] + M[n-1-k][m]) / 2, B[k][m] = (M[k][m] - M[n-1-k][m]) / 2
+    // even-odd tables of D and of the filter for the line kernels (eo_line,
+    // h = n / 2, c = h): A[k][m] = (M[k][m] + M[n-1-k][m]) / 2,
+    // B[k][m] = (M[k][m] - M[n-1-k][m]) / 2 (k, m < h); odd n: M[k][c] (k < h),
+    // M[c][m] (m < h), M[c][c]
     {
         const int h = n / 2;
         for (const std::vector<double>* M : {&c->tb.D, &c->tb.filt})
+        {
             for (int sgn = 0; sgn < 2; ++sgn)
                 for (int k = 0; k < h; ++k)
                     for (int m = 0; m < h; ++m)
@@ -791,6 +794,13 @@ int frsdg_create(const frsdg_desc* d, frsdg_ctx** out)
                         const double x = (*M)[k * n + m], y = (*M)[(n - 1 - k) * n + m];
                         tab.push_back(0.5 * (sgn == 0 ? x + y : x - y));
                     }
+            if (n % 2)
+            {
+                for (int k = 0; k < h; ++k) tab.push_back((*M)[k * n + h]);
+                for (int m = 0; m < h; ++m) tab.push_back((*M)[h * n + m]);
+                tab.push_back((*M)[h * n + h]);
+            }
+        }
     }
     int rc = FRSDG_OK;
     auto fail = [&](int code)

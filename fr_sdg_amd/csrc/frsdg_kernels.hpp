@@ -654,89 +654,175 @@ namespace frsdg
         }
     }
 
-    // Even-odd product of an n x n line operator M whose rows mirror with
-    // M[n-1-k][n-1-m] = -M[k][m] (ANTI: the derivative matrix on the symmetric
-    // GLL nodes) or +M[k][m] (the filter): with the halves A, B of the table
-    // (A[k][m] = (M[k][m] + M[n-1-k][m]) / 2, B[k][m] = (M[k][m] - M[n-1-k][m]) / 2,
-    // k, m < n/2) and the mirrored sums e[m] = x[m] + x[n-1-m], differences
-    // d[m] = x[m] - x[n-1-m]:
-    //   y[k] = A p + B q,  y[n-1-k] = A p - B q,  (p, q) = (d, e) if ANTI else (e, d),
-    // n^2 / 2 + 2n operations instead of n^2 (even n).
-    template <int N, class S, bool ANTI>
-    __device__ __forceinline__ void eo_line(const double* __restrict__ A, const double* __restrict__ B, const S x[N],
-                                            S y[N])
+    // Even-odd form of an n x n line operator M on the symmetric GLL nodes,
+    // whose entries mirror as M[n-1-k][n-1-m] = -M[k][m] (ANTI: the derivative
+    // matrix) or +M[k][m] (SYM: the filter).  With h = floor(n/2), the mirrored
+    // sums e[m] = x[m] + x[n-1-m] and differences d[m] = x[m] - x[n-1-m]
+    // (m < h), and the table halves A[k][m] = (M[k][m] + M[n-1-k][m]) / 2,
+    // B[k][m] = (M[k][m] - M[n-1-k][m]) / 2 (k, m < h):
+    //   y[k] = a + b,  y[n-1-k] = a - b,
+    //   ANTI: a = A d, b = B e + C[k] x[c];  SYM: a = A e + C[k] x[c], b = B d,
+    // and for odd n the middle node c = h adds C[k] = M[k][c] (k < h) and
+    //   y[c] = R . (ANTI ? d : e) + M[c][c] x[c],  R[m] = M[c][m].
+    // Per line: n^2/2 + O(n) operations instead of n^2.  Table of one operator
+    // (eo_tab): A [h*h], B [h*h], then for odd n C [h], R [h], M[c][c].
+    template <int N>
+    __host__ __device__ constexpr int eo_size()
     {
-        constexpr int H = N / 2;
-        S             p[H], q[H];
-#pragma unroll
-        for (int m = 0; m < H; ++m)
-        {
-            const S e = x[m] + x[N - 1 - m], d = x[m] - x[N - 1 - m];
-            p[m]      = ANTI ? d : e;
-            q[m]      = ANTI ? e : d;
-        }
-#pragma unroll
-        for (int k = 0; k < H; ++k)
-        {
-            S a = A[k * H] * p[0], b = B[k * H] * q[0];
-#pragma unroll
-            for (int m = 1; m < H; ++m)
-            {
-                a = a + A[k * H + m] * p[m];
-                b = b + B[k * H + m] * q[m];
-            }
-            y[k]         = a + b;
-            y[N - 1 - k] = a - b;
-        }
+        return 2 * (N / 2) * (N / 2) + ((N % 2) ? 2 * (N / 2) + 1 : 0);
     }
-    // offsets of the even-odd halves in the table: D's A, B, then the filter's
+    // the operator's table: which 0 = the derivative matrix, 1 = the filter
     template <int N>
     __device__ __forceinline__ const double* eo_tab(const double* tab, int which)
     {
-        return tab + 2 * N * N + 2 * N + which * (N / 2) * (N / 2);
+        return tab + 2 * N * N + 2 * N + which * eo_size<N>();
+    }
+    // mirrored sums and differences of a line, ordered (for a, for b)
+    template <int N, class S, bool ANTI>
+    __device__ __forceinline__ void eo_split(const S x[N], S pa[N / 2], S pb[N / 2])
+    {
+#pragma unroll
+        for (int m = 0; m < N / 2; ++m)
+        {
+            const S e = x[m] + x[N - 1 - m], d = x[m] - x[N - 1 - m];
+            pa[m]     = ANTI ? d : e;
+            pb[m]     = ANTI ? e : d;
+        }
+    }
+    // outputs k and n-1-k (k < h) from the split line; xc the middle value (odd n)
+    template <int N, class S, bool ANTI>
+    __device__ __forceinline__ void eo_pair(const double* __restrict__ T, int k, const S pa[N / 2], const S pb[N / 2],
+                                            S xc, S& y0, S& y1)
+    {
+        constexpr int H = N / 2;
+        const double* A = T;
+        const double* B = T + H * H;
+        S             a = A[k * H] * pa[0], b = B[k * H] * pb[0];
+#pragma unroll
+        for (int m = 1; m < H; ++m)
+        {
+            a = a + A[k * H + m] * pa[m];
+            b = b + B[k * H + m] * pb[m];
+        }
+        if constexpr (N % 2 == 1)
+        {
+            const double c = T[2 * H * H + k];
+            if constexpr (ANTI)
+                b = b + c * xc;
+            else
+                a = a + c * xc;
+        }
+        y0 = a + b;
+        y1 = a - b;
+    }
+    // middle output (odd n): R . d (ANTI) or R . e (SYM), both held in pa
+    template <int N, class S>
+    __device__ __forceinline__ S eo_mid(const double* __restrict__ T, const S pa[N / 2], S xc)
+    {
+        constexpr int H = N / 2;
+        const double* R = T + 2 * H * H + H;
+        S             y = T[2 * H * H + 2 * H] * xc;
+#pragma unroll
+        for (int m = 0; m < H; ++m) y = y + R[m] * pa[m];
+        return y;
+    }
+    // y = M x for a whole line
+    template <int N, class S, bool ANTI>
+    __device__ __forceinline__ void eo_line(const double* __restrict__ T, const S x[N], S y[N])
+    {
+        constexpr int H = N / 2;
+        S             pa[H > 0 ? H : 1], pb[H > 0 ? H : 1];
+        eo_split<N, S, ANTI>(x, pa, pb);
+        const S xc = x[H];
+#pragma unroll
+        for (int k = 0; k < H; ++k) eo_pair<N, S, ANTI>(T, k, pa, pb, xc, y[k], y[N - 1 - k]);
+        if constexpr (N % 2 == 1) y[H] = eo_mid<N, S>(T, pa, xc);
     }
 
-    // Volume term of one line by the even-odd contraction (eo_line) of F and
-    // G, for the curved double tiles that add both parts into V: the mirrored
-    // outputs k, n-1-k are formed and added together (even n).
-    template <int N, bool ETA, class SM>
+    // Volume term of one line of one element (equation q) of the double
+    // tiles by the even-odd contraction.  Curved: F and G along the line, the
+    // metric pair (ma, mb) of the line's nodes; ACC adds the result into V,
+    // else the xi-line result goes to U and the eta-line result to V.
+    // Affine: xi-line (row ln) of F or eta-line (column ln) of G in place,
+    // scaled by the constant metric.
+    template <int N, bool ETA, bool CURVED, bool ACC, class SM>
     __device__ __forceinline__ void volume_line_eo(const Res2DP& P, SM& sm, int le, int ln, int q,
                                                    const double* ma, const double* mb)
     {
         constexpr int H  = N / 2;
-        const double* DA = eo_tab<N>(P.tab, 0);
-        const double* DB = eo_tab<N>(P.tab, 1);
-        double        pf[H], qf[H], pg[H], qg[H];
-#pragma unroll
-        for (int m = 0; m < H; ++m)
+        const double* T  = eo_tab<N>(P.tab, 0);
+        auto          at = [&](int i, int& l0, int& k0)
         {
-            const int    l0 = ETA ? m : ln, k0 = ETA ? ln : m;
-            const int    l1 = ETA ? N - 1 - m : ln, k1 = ETA ? ln : N - 1 - m;
-            const double f0 = sm.F[le][l0][sm.pos(l0, k0, q)], f1 = sm.F[le][l1][sm.pos(l1, k1, q)];
-            const double g0 = sm.G[le][l0][sm.pos(l0, k0, q)], g1 = sm.G[le][l1][sm.pos(l1, k1, q)];
-            pf[m]           = f0 - f1;
-            qf[m]           = f0 + f1;
-            pg[m]           = g0 - g1;
-            qg[m]           = g0 + g1;
-        }
-#pragma unroll
-        for (int k = 0; k < H; ++k)
+            l0 = ETA ? i : ln;
+            k0 = ETA ? ln : i;
+        };
+        if constexpr (CURVED)
         {
-            double af = DA[k * H] * pf[0], bf = DB[k * H] * qf[0];
-            double ag = DA[k * H] * pg[0], bg = DB[k * H] * qg[0];
+            double pf[H], qf[H], pg[H], qg[H], fc = 0.0, gc = 0.0;
 #pragma unroll
-            for (int m = 1; m < H; ++m)
+            for (int m = 0; m < H; ++m)
             {
-                af = af + DA[k * H + m] * pf[m];
-                bf = bf + DB[k * H + m] * qf[m];
-                ag = ag + DA[k * H + m] * pg[m];
-                bg = bg + DB[k * H + m] * qg[m];
+                int l0, k0, l1, k1;
+                at(m, l0, k0);
+                at(N - 1 - m, l1, k1);
+                const double f0 = sm.F[le][l0][sm.pos(l0, k0, q)], f1 = sm.F[le][l1][sm.pos(l1, k1, q)];
+                const double g0 = sm.G[le][l0][sm.pos(l0, k0, q)], g1 = sm.G[le][l1][sm.pos(l1, k1, q)];
+                pf[m]           = f0 - f1;
+                qf[m]           = f0 + f1;
+                pg[m]           = g0 - g1;
+                qg[m]           = g0 + g1;
             }
-            const int k1 = N - 1 - k;
-            const int la = ETA ? k : ln, ka = ETA ? ln : k;
-            const int lb = ETA ? k1 : ln, kb = ETA ? ln : k1;
-            atomicAdd(&sm.V[le][la][sm.pos(la, ka, q)], ma[k] * (af + bf) + mb[k] * (ag + bg));
-            atomicAdd(&sm.V[le][lb][sm.pos(lb, kb, q)], ma[k1] * (af - bf) + mb[k1] * (ag - bg));
+            if constexpr (N % 2 == 1)
+            {
+                int lc, kc;
+                at(H, lc, kc);
+                fc = sm.F[le][lc][sm.pos(lc, kc, q)];
+                gc = sm.G[le][lc][sm.pos(lc, kc, q)];
+            }
+            auto put = [&](int i, double v)
+            {
+                int l0, k0;
+                at(i, l0, k0);
+                if constexpr (ACC)
+                    atomicAdd(&sm.V[le][l0][sm.pos(l0, k0, q)], v);
+                else if constexpr (ETA)
+                    sm.V[le][l0][sm.pos(l0, k0, q)] = v;
+                else
+                    sm.U[le][l0][sm.pos(l0, k0, q)] = v;
+            };
+#pragma unroll
+            for (int k = 0; k < H; ++k)
+            {
+                double f0, f1, g0, g1;
+                eo_pair<N, double, true>(T, k, pf, qf, fc, f0, f1);
+                eo_pair<N, double, true>(T, k, pg, qg, gc, g0, g1);
+                put(k, ma[k] * f0 + mb[k] * g0);
+                put(N - 1 - k, ma[N - 1 - k] * f1 + mb[N - 1 - k] * g1);
+            }
+            if constexpr (N % 2 == 1)
+                put(H, ma[H] * eo_mid<N, double>(T, pf, fc) + mb[H] * eo_mid<N, double>(T, pg, gc));
+        }
+        else
+        {
+            // the line of F (xi) or G (eta); only this task reads it in this phase
+            auto&        X  = ETA ? sm.G : sm.F;
+            const double sc = ETA ? P.eta_y : P.xi_x;
+            double       x[N], y[N];
+#pragma unroll
+            for (int m = 0; m < N; ++m)
+            {
+                int l0, k0;
+                at(m, l0, k0);
+                x[m] = X[le][l0][sm.pos(l0, k0, q)];
+            }
+            eo_line<N, double, true>(T, x, y);
+#pragma unroll
+            for (int k = 0; k < N; ++k)
+            {
+                int l0, k0;
+                at(k, l0, k0);
+                X[le][l0][sm.pos(l0, k0, q)] = sc * y[k];
+            }
         }
     }
 
@@ -755,12 +841,13 @@ namespace frsdg
                                                 int ln, int q, const double* ma = nullptr, const double* mb = nullptr)
     {
         const double* D = P.tab;
-        if constexpr (ACC && N % 2 == 0)
+        if constexpr (std::is_same_v<S, double> && N >= 2)
         {
+            // double tiles: even-odd contraction
             if (eta)
-                volume_line_eo<N, true>(P, sm, le, ln, q, ma, mb);
+                volume_line_eo<N, true, CURVED, ACC>(P, sm, le, ln, q, ma, mb);
             else
-                volume_line_eo<N, false>(P, sm, le, ln, q, ma, mb);
+                volume_line_eo<N, false, CURVED, ACC>(P, sm, le, ln, q, ma, mb);
             return;
         }
         if (!eta)
@@ -834,10 +921,10 @@ namespace frsdg
         double        x[N];
 #pragma unroll
         for (int c = 0; c < N; ++c) x[c] = eta ? sm.U[le][c][sm.pos(c, ln, q)] : sm.U[le][ln][sm.pos(ln, c, q)];
-        if constexpr (N % 2 == 0)
+        if constexpr (N >= 2)
         {
             double y[N];
-            eo_line<N, double, false>(eo_tab<N>(P.tab, 2), eo_tab<N>(P.tab, 3), x, y);
+            eo_line<N, double, false>(eo_tab<N>(P.tab, 1), x, y);
 #pragma unroll
             for (int r = 0; r < N; ++r)
             {
