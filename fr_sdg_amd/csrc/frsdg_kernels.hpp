@@ -1226,8 +1226,28 @@ namespace frsdg
     // state, V tangent state, F / G the tangent fluxes F'(u) v, G'(u) v; the
     // same 256-thread tiles, face tasks and line tasks as the residual.
     // ------------------------------------------------------------------
+    // LDS of one tangent tile: primal state U and tangent state T of the
+    // nodes (face traces), tangent fluxes F'(u) v, G'(u) v, and (curved) V,
+    // the volume term, both line parts added into it by LDS atomics (zeroed
+    // per node first); the residual tile's padding and equation order.
     template <int N, bool CURVED>
-    __device__ __forceinline__ void face_flux_tan(const Res2DP& P, FaceTask<Dual>& f, TileSmem<N, double, true>& sm)
+    struct TanSmem
+    {
+        static constexpr int NN = N * N, EPB = Tile2D<N, double>::EPB;
+        static constexpr int RP = TileSmem<N, double, true>::RP;
+        __device__ static __forceinline__ int pos(int l, int k, int q) { return 4 * k + (q ^ (l & 3)); }
+        double    U[EPB][N][RP];
+        double    T[EPB][N][RP];
+        double    F[EPB][N][RP];
+        double    G[EPB][N][RP];
+        double    V[CURVED ? EPB : 1][N][CURVED ? N * 4 : 1];
+        double    Del[EPB][4][N][4];
+        double    dL[N], dR[N];
+        long long eid[EPB];
+    };
+
+    template <int N, bool CURVED>
+    __device__ __forceinline__ void face_flux_tan(const Res2DP& P, FaceTask<Dual>& f, TanSmem<N, CURVED>& sm)
     {
         const int kA = f.xdir ? N - 1 : f.idx, lA = f.xdir ? f.idx : N - 1;
         const int kB = f.xdir ? 0 : f.idx, lB = f.xdir ? f.idx : 0;
@@ -1236,13 +1256,13 @@ namespace frsdg
         {
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                UA[q] = {sm.U[f.elA][lA][sm.pos(lA, kA, q)], sm.V[f.elA][lA][sm.pos(lA, kA, q)]};
+                UA[q] = {sm.U[f.elA][lA][sm.pos(lA, kA, q)], sm.T[f.elA][lA][sm.pos(lA, kA, q)]};
         }
         if (f.elB >= 0)
         {
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                UB[q] = {sm.U[f.elB][lB][sm.pos(lB, kB, q)], sm.V[f.elB][lB][sm.pos(lB, kB, q)]};
+                UB[q] = {sm.U[f.elB][lB][sm.pos(lB, kB, q)], sm.T[f.elB][lB][sm.pos(lB, kB, q)]};
         }
         if (f.elA < 0)
         {
@@ -1299,7 +1319,7 @@ namespace frsdg
 
     template <int N, bool CURVED>
     __device__ __forceinline__ void tan2d_tile(const Res2DP& P, const IOTan& io, long long blk, int tx, int ty,
-                                               TileSmem<N, double, true>& sm)
+                                               TanSmem<N, CURVED>& sm)
     {
         constexpr int BS  = 256;
         constexpr int NN  = N * N;
@@ -1323,11 +1343,15 @@ namespace frsdg
         FaceTask<Dual> f0;
         face_task<N, IOTan>(P, io, g, tid, f0);
 
-        // (unlike the residual, the curved line metrics are not prefetched
-        // here: held through the face phase they cost a wave per SIMD, n = 8:
-        // 184 -> 150 VGPRs, 2 -> 3 waves, 1024^2 p = 7 tangent 3.56 -> 2.71 ms)
-        const int nlt = tx * ty * N * 4;
-        double    vt[4]; // own tangent state
+        // curved: the line parts are added into V (no barrier between face and
+        // line tasks), and where the face tasks fit in waves 0-1 and the line
+        // tasks twice in waves 2-3 (n = 8) the two run on separate waves, as in
+        // res2d_tile.  The line metrics are loaded by the line tasks (held
+        // through the face phase they cost a wave per SIMD).
+        const int  nlt   = tx * ty * N * 4;
+        const int  nft   = ((tx + 1) * ty + tx * (ty + 1)) * N;
+        const bool split = CURVED && nft <= BS / 2 && 2 * nlt <= BS;
+        double     vt[4]; // own tangent state
         if (act)
         {
             Dual own[4], F[4], G[4];
@@ -1338,9 +1362,10 @@ namespace frsdg
             {
                 vt[q]                        = own[q].d;
                 sm.U[el][l][sm.pos(l, k, q)] = own[q].v;
-                sm.V[el][l][sm.pos(l, k, q)] = own[q].d;
+                sm.T[el][l][sm.pos(l, k, q)] = own[q].d;
                 sm.F[el][l][sm.pos(l, k, q)] = F[q].d;
                 sm.G[el][l][sm.pos(l, k, q)] = G[q].d;
+                if constexpr (CURVED) sm.V[el][l][sm.pos(l, k, q)] = 0.0;
             }
             if constexpr (!CURVED)
             {
@@ -1360,20 +1385,20 @@ namespace frsdg
         __syncthreads();
 
         if (f0.live) face_flux_tan<N, CURVED>(P, f0, sm);
-        const int nft = ((tx + 1) * ty + tx * (ty + 1)) * N;
         for (int t = tid + BS; t < nft; t += BS)
         {
             FaceTask<Dual> f;
             face_task<N, IOTan>(P, io, g, t, f);
             if (f.live) face_flux_tan<N, CURVED>(P, f, sm);
         }
-        // curved line tasks write over U and V, which face tasks read
-        if constexpr (CURVED) __syncthreads();
 
         {
-            const int nt = CURVED ? 2 * nlt : nlt;
-            for (int t = tid; t < nt; t += BS)
+            const int nt    = CURVED ? 2 * nlt : nlt;
+            const int lt0   = split ? tid - BS / 2 : tid;
+            const int lstep = split ? BS / 2 : BS;
+            for (int t = lt0; t < nt; t += lstep)
             {
+                if (t < 0) break;
                 const bool eta = t >= nlt;
                 const int  w   = eta ? t - nlt : t;
                 const int  q   = w & 3;
@@ -1384,7 +1409,7 @@ namespace frsdg
                 {
                     double ma[N], mb[N];
                     line_metrics<N>(P.vmet, sm.eid[le], eta, ln, ma, mb);
-                    volume_line<N, double, true>(P, sm, eta, le, ln, q, ma, mb);
+                    volume_line<N, double, true, TanSmem<N, CURVED>, true>(P, sm, eta, le, ln, q, ma, mb);
                 }
                 else
                 {
@@ -1402,7 +1427,7 @@ namespace frsdg
         {
             double r;
             if constexpr (CURVED)
-                r = sm.U[el][l][sm.pos(l, k, q)] + sm.V[el][l][sm.pos(l, k, q)];
+                r = sm.V[el][l][sm.pos(l, k, q)];
             else
                 r = sm.F[el][l][sm.pos(l, k, q)] + sm.G[el][l][sm.pos(l, k, q)];
             r    = r + sm.Del[el][0][l][q] * sm.dL[k];
@@ -1427,7 +1452,7 @@ namespace frsdg
     __global__ __launch_bounds__(256) void k_tan2d(const Res2DP P, const IOTan io)
     {
         using T = Tile2D<N, double>;
-        __shared__ TileSmem<N, double, true> sm;
+        __shared__ TanSmem<N, CURVED> sm;
         tan2d_tile<N, CURVED>(P, io, xcd_tile(blockIdx.x, gridDim.x), T::TX, T::TY, sm);
     }
 
