@@ -389,12 +389,16 @@ namespace frsdg
     //   xi-line  (row ln):    FX[ln][k] (+)= sum_m D[m][k] ma[m] W[ln][m], GX likewise with mb
     //   eta-line (column ln): FX[l][ln] (+)= sum_m D[m][l] ma[m] W[m][ln], GX likewise with mb
     // (ma, mb) = (xi_x, xi_y) on xi-lines, (eta_x, eta_y) on eta-lines; affine:
-    // xi-lines write FX from xi_x only, eta-lines GX from eta_y only.
-    template <int N, bool CURVED, bool ACC>
+    // xi-lines write FX from xi_x only, eta-lines GX from eta_y only.  D^T is
+    // centro-antisymmetric like D: the even-odd form (eo_line) with its own
+    // table.  Curved tiles add both lines' parts into FX, GX (zeroed per node
+    // first) by LDS atomics: two addends per entry, fl(xi + eta) whichever
+    // lands first.
+    template <int N, bool CURVED>
     __device__ __forceinline__ void adj_line(const Res2DP& P, AdjSmem<N>& sm, bool eta, int le, int ln, int q)
     {
-        const double* D = P.tab;
-        double        a[N], b[N];
+        const double* T = eo_tab<N>(P.tab, 2);
+        double        a[N], b[N], f[N], g[N];
         if constexpr (CURVED)
         {
             double ma[N], mb[N];
@@ -406,6 +410,15 @@ namespace frsdg
                 a[m]           = ma[m] * w;
                 b[m]           = mb[m] * w;
             }
+            eo_line<N, double, true>(T, a, f);
+            eo_line<N, double, true>(T, b, g);
+#pragma unroll
+            for (int r = 0; r < N; ++r)
+            {
+                const int l = eta ? r : ln, k = eta ? ln : r;
+                atomicAdd(&sm.FX[le][l][sm.pos(l, k, q)], f[r]);
+                atomicAdd(&sm.GX[le][l][sm.pos(l, k, q)], g[r]);
+            }
         }
         else
         {
@@ -413,35 +426,16 @@ namespace frsdg
 #pragma unroll
             for (int m = 0; m < N; ++m)
                 a[m] = mt * (eta ? sm.W[le][m][sm.pos(m, ln, q)] : sm.W[le][ln][sm.pos(ln, m, q)]);
-        }
+            eo_line<N, double, true>(T, a, f);
 #pragma unroll
-        for (int r = 0; r < N; ++r)
-        {
-            double f = 0.0, g = 0.0;
-#pragma unroll
-            for (int m = 0; m < N; ++m)
+            for (int r = 0; r < N; ++r)
             {
-                f += D[m * N + r] * a[m];
-                if constexpr (CURVED) g += D[m * N + r] * b[m];
-            }
-            const int l = eta ? r : ln, k = eta ? ln : r;
-            if constexpr (CURVED)
-            {
-                if constexpr (ACC)
-                {
-                    sm.FX[le][l][sm.pos(l, k, q)] += f;
-                    sm.GX[le][l][sm.pos(l, k, q)] += g;
-                }
+                const int l = eta ? r : ln, k = eta ? ln : r;
+                if (eta)
+                    sm.GX[le][l][sm.pos(l, k, q)] = f[r];
                 else
-                {
-                    sm.FX[le][l][sm.pos(l, k, q)] = f;
-                    sm.GX[le][l][sm.pos(l, k, q)] = g;
-                }
+                    sm.FX[le][l][sm.pos(l, k, q)] = f[r];
             }
-            else if (eta)
-                sm.GX[le][l][sm.pos(l, k, q)] = f;
-            else
-                sm.FX[le][l][sm.pos(l, k, q)] = f;
         }
     }
 
@@ -468,10 +462,9 @@ namespace frsdg
         const long long e   = static_cast<long long>(g.j0 + eb) * P.nx + (g.i0 + ea);
         if (node == 0 && el < tx * ty) sm.eid[el] = act ? e : -1;
 
-        // phase-1 tasks: line tasks first (curved: xi-lines; affine: xi- and
-        // eta-lines), then face tasks
+        // tasks: xi- and eta-line tasks, then face tasks
         const int nlt = tx * ty * N * 4;
-        const int nl  = CURVED ? nlt : 2 * nlt;
+        const int nl  = 2 * nlt;
         const int nft = ((tx + 1) * ty + tx * (ty + 1)) * N;
 
         if (act)
@@ -484,38 +477,62 @@ namespace frsdg
             {
                 sm.U[el][l][sm.pos(l, k, q)] = U[q];
                 sm.W[el][l][sm.pos(l, k, q)] = Wv[q];
+                if constexpr (CURVED)
+                {
+                    sm.FX[el][l][sm.pos(l, k, q)] = 0.0;
+                    sm.GX[el][l][sm.pos(l, k, q)] = 0.0;
+                }
             }
         }
         __syncthreads();
 
-        for (int t = tid; t < nl + nft; t += BS)
+        // one phase: face tasks and line tasks (xi and eta; curved: added into
+        // FX, GX) read U and W and write disjoint arrays.  Where the face tasks
+        // fit in waves 0-1 and the line tasks twice in waves 2-3 (curved n = 8)
+        // the two run on separate waves; otherwise line tasks first, then faces.
+        if (CURVED && nft <= BS / 2 && nl <= BS)
         {
-            if (t < nl)
+            if (tid < BS / 2)
             {
-                const bool eta = t >= nlt;
-                const int  wq  = eta ? t - nlt : t;
-                const int  le  = (wq >> 2) / N;
-                if (sm.eid[le] < 0) continue;
-                adj_line<N, CURVED, false>(P, sm, eta, le, (wq >> 2) - le * N, wq & 3);
+                if (tid < nft)
+                {
+                    AdjFace f;
+                    adj_face_setup<N>(P, u, wb, g, tid, f);
+                    if (f.live) adj_face_eval<N>(P, f, sm);
+                }
             }
             else
             {
-                AdjFace f;
-                adj_face_setup<N>(P, u, wb, g, t - nl, f);
-                if (f.live) adj_face_eval<N>(P, f, sm);
+                for (int t = tid - BS / 2; t < nl; t += BS / 2)
+                {
+                    const bool eta = t >= nlt;
+                    const int  wq  = eta ? t - nlt : t;
+                    const int  le  = (wq >> 2) / N;
+                    if (sm.eid[le] >= 0) adj_line<N, CURVED>(P, sm, eta, le, (wq >> 2) - le * N, wq & 3);
+                }
+            }
+        }
+        else
+        {
+            for (int t = tid; t < nl + nft; t += BS)
+            {
+                if (t < nl)
+                {
+                    const bool eta = t >= nlt;
+                    const int  wq  = eta ? t - nlt : t;
+                    const int  le  = (wq >> 2) / N;
+                    if (sm.eid[le] < 0) continue;
+                    adj_line<N, CURVED>(P, sm, eta, le, (wq >> 2) - le * N, wq & 3);
+                }
+                else
+                {
+                    AdjFace f;
+                    adj_face_setup<N>(P, u, wb, g, t - nl, f);
+                    if (f.live) adj_face_eval<N>(P, f, sm);
+                }
             }
         }
         __syncthreads();
-        if constexpr (CURVED)
-        {
-            for (int t = tid; t < nlt; t += BS)
-            {
-                const int le = (t >> 2) / N;
-                if (sm.eid[le] < 0) continue;
-                adj_line<N, true, true>(P, sm, true, le, (t >> 2) - le * N, t & 3);
-            }
-            __syncthreads();
-        }
         if (!act) return;
 
         double U[4], Wv[4], Fb[4], Gb[4], ub[4] = {0, 0, 0, 0};

@@ -779,13 +779,16 @@ int frsdg_create(const frsdg_desc* d, frsdg_ctx** out)
     tab.insert(tab.end(), c->tb.dL.begin(), c->tb.dL.end());
     tab.insert(tab.end(), c->tb.dR.begin(), c->tb.dR.end());
     tab.insert(tab.end(), c->tb.filt.begin(), c->tb.filt.end());
-    // even-odd tables of D and of the filter for the line kernels (eo_line,
-    // h = n / 2, c = h): A[k][m] = (M[k][m] + M[n-1-k][m]) / 2,
-    // B[k][m] = (M[k][m] - M[n-1-k][m]) / 2 (k, m < h); odd n: M[k][c] (k < h),
-    // M[c][m] (m < h), M[c][c]
+    // even-odd tables of D, of the filter and of D transposed (the adjoint's
+    // lines) for the line kernels (eo_line, h = n / 2, c = h):
+    // A[k][m] = (M[k][m] + M[n-1-k][m]) / 2, B[k][m] = (M[k][m] - M[n-1-k][m]) / 2
+    // (k, m < h); odd n: M[k][c] (k < h), M[c][m] (m < h), M[c][c]
     {
         const int h = n / 2;
-        for (const std::vector<double>* M : {&c->tb.D, &c->tb.filt})
+        std::vector<double> Dt(static_cast<size_t>(n) * n);
+        for (int k = 0; k < n; ++k)
+            for (int m = 0; m < n; ++m) Dt[k * n + m] = c->tb.D[m * n + k];
+        for (const std::vector<double>* M : {&c->tb.D, &c->tb.filt, &Dt})
         {
             for (int sgn = 0; sgn < 2; ++sgn)
                 for (int k = 0; k < h; ++k)

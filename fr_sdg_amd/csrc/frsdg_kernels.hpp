@@ -671,7 +671,8 @@ namespace frsdg
     {
         return 2 * (N / 2) * (N / 2) + ((N % 2) ? 2 * (N / 2) + 1 : 0);
     }
-    // the operator's table: which 0 = the derivative matrix, 1 = the filter
+    // the operator's table: which 0 = the derivative matrix, 1 = the filter,
+    // 2 = the derivative matrix transposed
     template <int N>
     __device__ __forceinline__ const double* eo_tab(const double* tab, int which)
     {
