@@ -285,10 +285,62 @@ namespace frsdg
 
     constexpr int GT = 144; // largest block factored in a register tile (256 threads x 9 x 9)
 
-    // A_II = L_II U_II (unpivoted, right-looking) for nd <= 144: the block
-    // lives in registers, thread (tx, ty) holding rows tx + 16 i and columns
-    // ty + 16 j, and each elimination step broadcasts the pivot row and column
-    // through LDS.
+    // Register tiles of the diagonal blocks (nd <= 144): 256 threads, thread
+    // (tx, ty) = (tid & 15, tid >> 4) holding x[i][j] = entry (tx + 16 i,
+    // ty + 16 j).  The elimination steps k of one 16-row block KB = k / 16 are
+    // instantiated with KB known at compile time, so a step touches only the
+    // live part of the tile (rows and columns of blocks >= KB for the LU) with
+    // fixed register indices; the pivot row and column go through LDS, double
+    // buffered by step (one barrier per step: a thread passing step k+1's
+    // barrier has finished reading step k's buffers).
+
+    // A_II = L_II U_II (unpivoted, right-looking), steps of block KB
+    template <int KB>
+    __device__ __forceinline__ void lu_block_steps(int nd, double (&x)[9][9], double* __restrict__ bufA,
+                                                   double* __restrict__ bufB, int* __restrict__ info, int tx, int ty)
+    {
+#pragma unroll 1
+        for (int kt = 0; kt < 16; ++kt)
+        {
+            const int k = 16 * KB + kt;
+            if (k >= nd) return;
+            double* rowk = bufA + (k & 1) * GT;
+            double* colk = bufB + (k & 1) * GT;
+            if (tx == kt)
+#pragma unroll
+                for (int j = KB; j < 9; ++j) rowk[ty + 16 * j] = x[KB][j];
+            if (ty == kt)
+#pragma unroll
+                for (int i = KB; i < 9; ++i) colk[tx + 16 * i] = x[i][KB];
+            __syncthreads();
+            const double piv = rowk[k];
+            if (threadIdx.x == 0 && ::fabs(piv) <= 1e-12) info[0] = 1; // PETSc's default zero-pivot tolerance
+            const double rpiv = 1.0 / piv;
+            // rows at or above the pivot get a zero multiplier and columns at or
+            // left of it a zero pivot-row entry (selects); blocks below KB are done
+            double w[9], l[9];
+#pragma unroll
+            for (int j = KB; j < 9; ++j)
+            {
+                const double v = rowk[ty + 16 * j];
+                w[j]           = (j > KB || ty > kt) ? v : 0.0;
+            }
+#pragma unroll
+            for (int i = KB; i < 9; ++i)
+            {
+                const double v = colk[tx + 16 * i];
+                l[i]           = ((i > KB || tx > kt) && tx + 16 * i < nd) ? v * rpiv : 0.0;
+            }
+#pragma unroll
+            for (int i = KB; i < 9; ++i)
+#pragma unroll
+                for (int j = KB; j < 9; ++j) x[i][j] = ::fma(-l[i], w[j], x[i][j]);
+            // the pivot column below the diagonal becomes the multipliers
+            if (ty == kt)
+#pragma unroll
+                for (int i = KB; i < 9; ++i) x[i][KB] = (i > KB || tx > kt) ? l[i] : x[i][KB];
+        }
+    }
     __device__ inline void diag_lu_reg(int nd, double* __restrict__ D, double* __restrict__ bufA,
                                        double* __restrict__ bufB, int* __restrict__ info)
     {
@@ -302,59 +354,15 @@ namespace frsdg
                 const int r = tx + 16 * i, c = ty + 16 * j;
                 x[i][j] = (r < nd && c < nd) ? D[r + static_cast<long long>(c) * nd] : 0.0;
             }
-        // pivot row and column of step k in buffer k & 1: one barrier per step (a
-        // thread passing step k+1's barrier has finished reading step k's buffers,
-        // so step k+2 may overwrite them)
-        for (int k = 0; k < nd; ++k)
-        {
-            const int kt = k & 15, kb = k >> 4;
-            double* rowk = bufA + (k & 1) * GT;
-            double* colk = bufB + (k & 1) * GT;
-            if (tx == kt)
-#pragma unroll
-                for (int i = 0; i < 9; ++i)
-                    if (i == kb)
-#pragma unroll
-                        for (int j = 0; j < 9; ++j) rowk[ty + 16 * j] = x[i][j];
-            if (ty == kt)
-#pragma unroll
-                for (int j = 0; j < 9; ++j)
-                    if (j == kb)
-#pragma unroll
-                        for (int i = 0; i < 9; ++i) colk[tx + 16 * i] = x[i][j];
-            __syncthreads();
-            const double piv = rowk[k];
-            if (tid == 0 && ::fabs(piv) <= 1e-12) info[0] = 1; // PETSc's default zero-pivot tolerance
-            const double rpiv = 1.0 / piv;
-            // every thread runs the same 81 FMAs: rows at or above the pivot get
-            // a zero multiplier and columns at or left of it a zero pivot-row
-            // entry (selects, not per-element branches); all 144 entries of both
-            // buffers were written this step
-            double w[9], l[9];
-#pragma unroll
-            for (int j = 0; j < 9; ++j)
-            {
-                const double v = rowk[ty + 16 * j];
-                w[j] = (ty + 16 * j > k) ? v : 0.0;
-            }
-#pragma unroll
-            for (int i = 0; i < 9; ++i)
-            {
-                const double v = colk[tx + 16 * i];
-                l[i] = (tx + 16 * i > k && tx + 16 * i < nd) ? v * rpiv : 0.0;
-            }
-#pragma unroll
-            for (int i = 0; i < 9; ++i)
-#pragma unroll
-                for (int j = 0; j < 9; ++j) x[i][j] = ::fma(-l[i], w[j], x[i][j]);
-            // the pivot column below the diagonal becomes the multipliers
-            if (ty == kt)
-#pragma unroll
-                for (int j = 0; j < 9; ++j)
-                    if (j == kb)
-#pragma unroll
-                        for (int i = 0; i < 9; ++i) x[i][j] = (tx + 16 * i > k) ? l[i] : x[i][j];
-        }
+        lu_block_steps<0>(nd, x, bufA, bufB, info, tx, ty);
+        lu_block_steps<1>(nd, x, bufA, bufB, info, tx, ty);
+        lu_block_steps<2>(nd, x, bufA, bufB, info, tx, ty);
+        lu_block_steps<3>(nd, x, bufA, bufB, info, tx, ty);
+        lu_block_steps<4>(nd, x, bufA, bufB, info, tx, ty);
+        lu_block_steps<5>(nd, x, bufA, bufB, info, tx, ty);
+        lu_block_steps<6>(nd, x, bufA, bufB, info, tx, ty);
+        lu_block_steps<7>(nd, x, bufA, bufB, info, tx, ty);
+        lu_block_steps<8>(nd, x, bufA, bufB, info, tx, ty);
 #pragma unroll
         for (int i = 0; i < 9; ++i)
 #pragma unroll
@@ -366,51 +374,62 @@ namespace frsdg
         __syncthreads();
     }
 
-    // L_II^-1 of the factored block D (unit lower part), in a register tile;
-    // the columns of L are read from D one step ahead.
+    // L_II^-1 of the factored block D (unit lower part): X = I; for k
+    // ascending, rows r > k: X_r -= L_rk X_k.  Row k of X is zero right of
+    // column k, so a step of block KB updates rows of blocks >= KB and columns
+    // of blocks <= KB; the column of L is read from D one step ahead.
+    template <int KB>
+    __device__ __forceinline__ void linv_block_steps(int nd, const double* __restrict__ D, double (&x)[9][9],
+                                                     double& nxt, double* __restrict__ bufA,
+                                                     double* __restrict__ bufB, int tx, int ty, int tid)
+    {
+#pragma unroll 1
+        for (int kt = 0; kt < 16; ++kt)
+        {
+            const int k = 16 * KB + kt;
+            if (k >= nd - 1) return;
+            double* rowk = bufA + (k & 1) * GT;
+            double* colk = bufB + (k & 1) * GT;
+            if (tx == kt)
+#pragma unroll
+                for (int j = 0; j <= KB; ++j) rowk[ty + 16 * j] = x[KB][j];
+            if (tid > k && tid < nd) colk[tid] = nxt;
+            if (tid < nd) nxt = D[tid + static_cast<long long>(k + 1) * nd];
+            __syncthreads();
+            double w[9], l[9];
+#pragma unroll
+            for (int j = 0; j <= KB; ++j) w[j] = rowk[ty + 16 * j];
+#pragma unroll
+            for (int i = KB; i < 9; ++i)
+            {
+                const double v = colk[tx + 16 * i];
+                l[i]           = ((i > KB || tx > kt) && tx + 16 * i < nd) ? v : 0.0;
+            }
+#pragma unroll
+            for (int i = KB; i < 9; ++i)
+#pragma unroll
+                for (int j = 0; j <= KB; ++j) x[i][j] = ::fma(-l[i], w[j], x[i][j]);
+        }
+    }
     __device__ inline void diag_linv_reg(int nd, const double* __restrict__ D, double* __restrict__ Li,
                                          double* __restrict__ bufA, double* __restrict__ bufB)
     {
         const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
         double x[9][9];
-        // L^-1: X = I; for k ascending, rows r > k: X_r -= L_rk X_k (buffers
-        // alternate by step: one barrier per step)
 #pragma unroll
         for (int i = 0; i < 9; ++i)
 #pragma unroll
             for (int j = 0; j < 9; ++j) x[i][j] = (tx + 16 * i == ty + 16 * j) ? 1.0 : 0.0;
         double nxt = (tid < nd) ? D[tid] : 0.0; // column k of D, loaded one step ahead
-        for (int k = 0; k < nd - 1; ++k)
-        {
-            const int kt = k & 15, kb = k >> 4;
-            double* rowk = bufA + (k & 1) * GT;
-            double* colk = bufB + (k & 1) * GT;
-            if (tx == kt)
-#pragma unroll
-                for (int i = 0; i < 9; ++i)
-                    if (i == kb)
-#pragma unroll
-                        for (int j = 0; j < 9; ++j) rowk[ty + 16 * j] = x[i][j];
-            if (tid > k && tid < nd) colk[tid] = nxt;
-            if (tid < nd) nxt = D[tid + static_cast<long long>(k + 1) * nd];
-            __syncthreads();
-            // row k of X is zero right of column k (X is unit lower triangular),
-            // so every thread runs the same 81 FMAs and rows at or above the
-            // pivot get a zero multiplier (selects, not per-element branches)
-            double w[9], l[9];
-#pragma unroll
-            for (int j = 0; j < 9; ++j) w[j] = rowk[ty + 16 * j];
-#pragma unroll
-            for (int i = 0; i < 9; ++i)
-            {
-                const double v = colk[tx + 16 * i];
-                l[i] = (tx + 16 * i > k && tx + 16 * i < nd) ? v : 0.0;
-            }
-#pragma unroll
-            for (int i = 0; i < 9; ++i)
-#pragma unroll
-                for (int j = 0; j < 9; ++j) x[i][j] = ::fma(-l[i], w[j], x[i][j]);
-        }
+        linv_block_steps<0>(nd, D, x, nxt, bufA, bufB, tx, ty, tid);
+        linv_block_steps<1>(nd, D, x, nxt, bufA, bufB, tx, ty, tid);
+        linv_block_steps<2>(nd, D, x, nxt, bufA, bufB, tx, ty, tid);
+        linv_block_steps<3>(nd, D, x, nxt, bufA, bufB, tx, ty, tid);
+        linv_block_steps<4>(nd, D, x, nxt, bufA, bufB, tx, ty, tid);
+        linv_block_steps<5>(nd, D, x, nxt, bufA, bufB, tx, ty, tid);
+        linv_block_steps<6>(nd, D, x, nxt, bufA, bufB, tx, ty, tid);
+        linv_block_steps<7>(nd, D, x, nxt, bufA, bufB, tx, ty, tid);
+        linv_block_steps<8>(nd, D, x, nxt, bufA, bufB, tx, ty, tid);
 #pragma unroll
         for (int i = 0; i < 9; ++i)
 #pragma unroll
@@ -421,25 +440,21 @@ namespace frsdg
             }
     }
 
-    // U_II^-1 of the factored block D (upper part, with the diagonal), in a
-    // register tile.
-    __device__ inline void diag_uinv_reg(int nd, const double* __restrict__ D, double* __restrict__ Ui,
-                                         double* __restrict__ bufA, double* __restrict__ bufB)
+    // U_II^-1 of the factored block D (upper part, with the diagonal): X = I;
+    // for k descending, X_k /= U_kk, rows r < k: X_r -= U_rk X_k.  Row k of X is
+    // zero left of column k, so a step of block KB updates rows of blocks <= KB
+    // and columns of blocks >= KB; the diagonal entry and the column of U are
+    // loaded one step ahead.
+    template <int KB>
+    __device__ __forceinline__ void uinv_block_steps(int nd, const double* __restrict__ D, double (&x)[9][9],
+                                                     double& nxt, double& ukk, double* __restrict__ bufA,
+                                                     double* __restrict__ bufB, int tx, int ty, int tid)
     {
-        const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-        double x[9][9];
-        // U^-1: X = I; for k descending, X_k /= U_kk, rows r < k: X_r -= U_rk X_k
-        // (buffers alternate by step; the diagonal entry and the column of U
-        // are loaded one step ahead)
-#pragma unroll
-        for (int i = 0; i < 9; ++i)
-#pragma unroll
-            for (int j = 0; j < 9; ++j) x[i][j] = (tx + 16 * i == ty + 16 * j) ? 1.0 : 0.0;
-        double nxt = (tid < nd) ? D[tid + static_cast<long long>(nd - 1) * nd] : 0.0;
-        double ukk = D[(nd - 1) + static_cast<long long>(nd - 1) * nd];
-        for (int k = nd - 1; k >= 0; --k)
+#pragma unroll 1
+        for (int kt = 15; kt >= 0; --kt)
         {
-            const int kt = k & 15, kb = k >> 4;
+            const int k = 16 * KB + kt;
+            if (k >= nd) continue;
             double* rowk = bufA + (k & 1) * GT;
             double* colk = bufB + (k & 1) * GT;
             if (tid < k) colk[tid] = nxt; // column k of U above the diagonal
@@ -451,32 +466,47 @@ namespace frsdg
             }
             if (tx == kt)
 #pragma unroll
-                for (int i = 0; i < 9; ++i)
-                    if (i == kb)
-#pragma unroll
-                        for (int j = 0; j < 9; ++j)
-                        {
-                            x[i][j] *= rukk;
-                            rowk[ty + 16 * j] = x[i][j];
-                        }
+                for (int j = KB; j < 9; ++j)
+                {
+                    x[KB][j] *= rukk;
+                    rowk[ty + 16 * j] = x[KB][j];
+                }
             __syncthreads();
-            // row k of X is zero left of column k (X is upper triangular), so
-            // every thread runs the same 81 FMAs and rows at or below the pivot
-            // get a zero multiplier (selects, not per-element branches)
             double w[9], u[9];
 #pragma unroll
-            for (int j = 0; j < 9; ++j) w[j] = rowk[ty + 16 * j];
+            for (int j = KB; j < 9; ++j) w[j] = rowk[ty + 16 * j];
 #pragma unroll
-            for (int i = 0; i < 9; ++i)
+            for (int i = 0; i <= KB; ++i)
             {
                 const double v = colk[tx + 16 * i];
-                u[i] = (tx + 16 * i < k) ? v : 0.0;
+                u[i]           = (i < KB || tx < kt) ? v : 0.0;
             }
 #pragma unroll
-            for (int i = 0; i < 9; ++i)
+            for (int i = 0; i <= KB; ++i)
 #pragma unroll
-                for (int j = 0; j < 9; ++j) x[i][j] = ::fma(-u[i], w[j], x[i][j]);
+                for (int j = KB; j < 9; ++j) x[i][j] = ::fma(-u[i], w[j], x[i][j]);
         }
+    }
+    __device__ inline void diag_uinv_reg(int nd, const double* __restrict__ D, double* __restrict__ Ui,
+                                         double* __restrict__ bufA, double* __restrict__ bufB)
+    {
+        const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+        double x[9][9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+#pragma unroll
+            for (int j = 0; j < 9; ++j) x[i][j] = (tx + 16 * i == ty + 16 * j) ? 1.0 : 0.0;
+        double nxt = (tid < nd) ? D[tid + static_cast<long long>(nd - 1) * nd] : 0.0;
+        double ukk = D[(nd - 1) + static_cast<long long>(nd - 1) * nd];
+        uinv_block_steps<8>(nd, D, x, nxt, ukk, bufA, bufB, tx, ty, tid);
+        uinv_block_steps<7>(nd, D, x, nxt, ukk, bufA, bufB, tx, ty, tid);
+        uinv_block_steps<6>(nd, D, x, nxt, ukk, bufA, bufB, tx, ty, tid);
+        uinv_block_steps<5>(nd, D, x, nxt, ukk, bufA, bufB, tx, ty, tid);
+        uinv_block_steps<4>(nd, D, x, nxt, ukk, bufA, bufB, tx, ty, tid);
+        uinv_block_steps<3>(nd, D, x, nxt, ukk, bufA, bufB, tx, ty, tid);
+        uinv_block_steps<2>(nd, D, x, nxt, ukk, bufA, bufB, tx, ty, tid);
+        uinv_block_steps<1>(nd, D, x, nxt, ukk, bufA, bufB, tx, ty, tid);
+        uinv_block_steps<0>(nd, D, x, nxt, ukk, bufA, bufB, tx, ty, tid);
 #pragma unroll
         for (int i = 0; i < 9; ++i)
 #pragma unroll
