@@ -721,6 +721,15 @@ namespace
     }
 } // namespace
 
+int frsdg::ifunction_be(frsdg_ctx* c, const double* x, const double* un, double sigma, double* F, hipStream_t s)
+{
+    if (c->d.dims != 2 || c->d.cond_sys) return FRSDG_ERR_UNSUPPORTED;
+    IORes io{x, F, nullptr, 1.0};
+    io.un    = un;
+    io.shift = sigma;
+    return launch_res(c, io, s);
+}
+
 extern "C" {
 
 const char* frsdg_strerror(int code)

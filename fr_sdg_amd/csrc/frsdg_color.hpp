@@ -60,6 +60,7 @@ namespace frsdg
         std::vector<int> cls_count; // elements per colour class (host)
         int* nbr  = nullptr; // [E][S] neighbour element per slot, -1 none
         int* cls  = nullptr; // [E] colour class
+        int* cslot = nullptr; // [ncls][E] slot of row e whose neighbour has colour class c, -1 none
         int* low  = nullptr; // [E][CMAXN] lower slots (neighbour index < e) in increasing neighbour order
         int* upd  = nullptr; // [E][CMAXN][2 CMAXP] (slot of J in row I, slot of J in row K) per lower slot
         int* upp  = nullptr; // [E][CMAXN] upper slots
@@ -245,40 +246,42 @@ namespace frsdg
     // Column kdof of the blocks whose neighbour is in colour class cc:
     // (F(w3) - F(x)) / h (modes 0, 1) or the product J e_colour (mode 2).
     static __global__ __launch_bounds__(256) void k_color_store(long long E, int S, int nd, const int* __restrict__ nbr,
-                                                                const int* __restrict__ cls, int cc, int kdof,
+                                                                const int* __restrict__ cslot, int cc, int kdof,
                                                                 const double* __restrict__ f1,
                                                                 const double* __restrict__ f0,
                                                                 const double* __restrict__ x, double dxinv_wp,
                                                                 double eps, double umin, int mode,
-                                                                double* __restrict__ P)
+                                                                double* __restrict__ P, double* __restrict__ w3)
     {
         const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
         if (idx >= E * nd) return;
         const long long R = idx / nd;
         const int r = static_cast<int>(idx - R * nd);
-        for (int s = 0; s < S; ++s)
+        // the one neighbour of this block row in colour class cc (each block row
+        // meets at most one element of a class)
+        const int s = cslot[cc * E + R];
+        if (s < 0) return;
+        const int m = nbr[R * S + s];
+        double val;
+        if (mode == 2)
+            val = f1[idx];
+        else
         {
-            const int m = nbr[R * S + s];
-            if (m < 0 || cls[m] != cc) continue;
-            double val;
-            if (mode == 2)
-                val = f1[idx];
+            const double dy = f1[idx] - f0[idx];
+            if (mode == 0)
+                val = dy * dxinv_wp;
             else
             {
-                const double dy = f1[idx] - f0[idx];
-                if (mode == 0)
-                    val = dy * dxinv_wp;
-                else
-                {
-                    double dx = x[static_cast<long long>(m) * nd + kdof];
-                    if (::fabs(dx) < umin) dx = dx >= 0.0 ? umin : -umin;
-                    dx *= eps;
-                    val = dy * (1.0 / dx);
-                }
+                double dx = x[static_cast<long long>(m) * nd + kdof];
+                if (::fabs(dx) < umin) dx = dx >= 0.0 ? umin : -umin;
+                dx *= eps;
+                val = dy * (1.0 / dx);
             }
-            P[((R * S + s) * nd + kdof) * static_cast<long long>(nd) + r] = val;
-            break; // one neighbour of a colour class per block row
         }
+        P[((R * S + s) * nd + kdof) * static_cast<long long>(nd) + r] = val;
+        // rows of the class itself put back the colour's entry of the perturbed
+        // copy (x, or zero for the unit colour vectors): F(w3) has been read
+        if (s == S / 2 && r == kdof) w3[idx] = mode == 2 ? 0.0 : x[idx];
     }
 
     // ---- block ILU(0) -----------------------------------------------------------

@@ -74,4 +74,7 @@ namespace frsdg
 
     // defined in frsdg_newton.hip
     void free_solver(frsdg_ctx* c);
+    // defined in frsdg_capi.hip: F = sigma (x - un) + R(x) in one residual
+    // launch (2D, conservative variables); FRSDG_ERR_UNSUPPORTED elsewhere
+    int ifunction_be(frsdg_ctx* c, const double* x, const double* un, double sigma, double* F, hipStream_t s);
 } // namespace frsdg

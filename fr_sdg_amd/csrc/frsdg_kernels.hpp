@@ -97,7 +97,9 @@ namespace frsdg
         if (f) atomicOr(flag, f);
     }
 
-    // out = add + scale * R   (residual: scale 1; rhs: scale -1; ifunction: add = udot)
+    // out = add + scale * R   (residual: scale 1; rhs: scale -1; ifunction: add = udot);
+    // with un (and add null): out = shift (u - un) + R, the implicit function of
+    // a backward-Euler stage with its time derivative formed in the store
     struct IORes
     {
         using S = double;
@@ -106,6 +108,8 @@ namespace frsdg
         double* __restrict__ out;
         const double* __restrict__ add;
         double scale;
+        const double* __restrict__ un = nullptr;
+        double shift                  = 0.0;
         __device__ double ld(long long i) const { return u[i]; }
         __device__ void ld4(long long i, double o[4]) const { ld4d(u + i, o); }
         __device__ void st(long long i, double, double r) const
@@ -116,15 +120,23 @@ namespace frsdg
         static constexpr bool pre = true;
         __device__ void pre4(long long i, double z[4]) const
         {
-            if (add) ld4d(add + i, z);
+            if (add)
+                ld4d(add + i, z);
+            else if (un)
+                ld4d(un + i, z);
         }
-        __device__ void st4p(long long i, const double*, const double r[4], const double z[4]) const
+        __device__ void st4p(long long i, const double* own, const double r[4], const double z[4]) const
         {
             double o[4];
             if (add)
             {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) o[q] = z[q] + r[q];
+            }
+            else if (un)
+            {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) o[q] = __builtin_fma(shift, own[q] - z[q], r[q]);
             }
             else
 #pragma unroll

@@ -68,7 +68,7 @@ namespace frsdg
             if (e) (void) hipEventDestroy(e);
         if (frsdg::ColorPlan* cp = s->cp)
         {
-            for (int* p : {cp->nbr, cp->cls, cp->low, cp->upd, cp->upp, cp->lvl, cp->lpos, cp->upos, cp->info})
+            for (int* p : {cp->nbr, cp->cls, cp->cslot, cp->low, cp->upd, cp->upp, cp->lvl, cp->lpos, cp->upos, cp->info})
                 if (p) (void) hipFree(p);
             for (double* p : {cp->P, cp->Dinv, cp->y, cp->v, cp->f0, cp->f1, cp->T, cp->t})
                 if (p) (void) hipFree(p);
@@ -373,8 +373,17 @@ namespace
             FR_TRY(hipMemcpy(d, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
             return FRSDG_OK;
         };
-        if ((rc = up(cp->nbr, nbr)) || (rc = up(cp->cls, cls)) || (rc = up(cp->low, low)) || (rc = up(cp->upd, upd)) ||
-            (rc = up(cp->upp, upp)) || (rc = up(cp->lvl, lvl)) || (rc = up(cp->lpos, lpos)) || (rc = up(cp->upos, upos)))
+        // slot of each block row's neighbour in each colour class
+        std::vector<int> cslot(static_cast<size_t>(cp->ncls) * cp->E, -1);
+        for (long long e = 0; e < cp->E; ++e)
+            for (int sl = 0; sl < cp->S; ++sl)
+            {
+                const int m = nbr[e * cp->S + sl];
+                if (m >= 0) cslot[static_cast<size_t>(cls[m]) * cp->E + e] = sl;
+            }
+        if ((rc = up(cp->nbr, nbr)) || (rc = up(cp->cls, cls)) || (rc = up(cp->cslot, cslot)) || (rc = up(cp->low, low)) ||
+            (rc = up(cp->upd, upd)) || (rc = up(cp->upp, upp)) || (rc = up(cp->lvl, lvl)) || (rc = up(cp->lpos, lpos)) ||
+            (rc = up(cp->upos, upos)))
             return rc;
         FR_TRY(hipMalloc(&cp->info, sizeof(int)));
         const long long bsz = static_cast<long long>(nd) * nd;
@@ -654,10 +663,9 @@ namespace
                     else
                         rc = evalF(cp.v, cp.f1);
                     if (rc) return rc;
+                    // the store also restores the colour's entries of the perturbed copy
                     hipLaunchKernelGGL(frsdg::k_color_store, dim3(grid(N)), dim3(256), 0, st, cp.E, cp.S, cp.nd, cp.nbr,
-                                       cp.cls, cc, k, cp.f1, cp.f0, u, dxinv, eps, umin, method, P);
-                    hipLaunchKernelGGL(frsdg::k_color_perturb, dim3(ge), dim3(256), 0, st, cp.E, cp.nd, cp.cls, cc, k, u,
-                                       cp.v, h_wp, eps, umin, method, 1);
+                                       cp.cslot, cc, k, cp.f1, cp.f0, u, dxinv, eps, umin, method, P, cp.v);
                 }
             }
             return hip_rc(hipGetLastError());
@@ -947,6 +955,7 @@ extern "C" int frsdg_beuler_step(frsdg_ctx* c, double t, double dt, double* u, c
     auto eval_F = [&](const double* x, double* F) -> int
     {
         S->res_evals++;
+        if (!cs && c->d.dims == 2) return frsdg::ifunction_be(c, x, s->un, sigma, F, st);
         if (cs)
         {
             hipLaunchKernelGGL(k_p2c, dim3(gn), dim3(256), 0, st, c->nm, x, c->w2);
@@ -1205,6 +1214,7 @@ extern "C" int frsdg_adjoint_solve(frsdg_ctx* c, double t, double sigma, const d
         auto eval_F = [&](const double* xx, double* F) -> int
         {
             S->res_evals++;
+            if (!cs && c->d.dims == 2) return frsdg::ifunction_be(c, xx, s->un, sigma, F, st);
             if (cs)
             {
                 hipLaunchKernelGGL(k_p2c, dim3(gn), dim3(256), 0, st, c->nm, xx, c->w2);
@@ -1262,6 +1272,7 @@ extern "C" int frsdg_color_jacobian(frsdg_ctx* c, double t, const double* u, con
     Ctx K{c, s, st, &oo, t, sigma, u, &ns};
     auto eval_F = [&](const double* x, double* F) -> int
     {
+        if (!cs && c->d.dims == 2) return frsdg::ifunction_be(c, x, s->un, sigma, F, st);
         if (cs)
         {
             hipLaunchKernelGGL(k_p2c, dim3(gn), dim3(256), 0, st, c->nm, x, c->w2);
@@ -1335,6 +1346,7 @@ extern "C" int frsdg_linear_solve(frsdg_ctx* c, double t, double sigma, const do
         auto eval_F = [&](const double* xx, double* F) -> int
         {
             S->res_evals++;
+            if (!cs && c->d.dims == 2) return frsdg::ifunction_be(c, xx, s->un, sigma, F, st);
             if (cs)
             {
                 hipLaunchKernelGGL(k_p2c, dim3(gn), dim3(256), 0, st, c->nm, xx, c->w2);
