@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--adj-steps", type=int, default=2000)
     ap.add_argument("--adj-snapshots", type=int, default=32)
     ap.add_argument("--adj-n", type=int, default=1024)
+    ap.add_argument("--adj-fd-eps", default="1e-2,1e-3,1e-4",
+                    help="comma-separated steps of the central finite difference that checks the adjoint")
     ap.add_argument("--kernel-only", choices=["step", "stage", "residual", "tangent", "cstep", "adjoint", "large",
                                                 "tangent_large", "adjoint_large",
                                                 "limiter", "copy", "vcopy", "batch", "batch_each", "ilu"],
@@ -798,20 +800,23 @@ def adjoint_large(torch, fr, args):
     prm_p[1] *= 1.0 + dp
     prm_m[1] *= 1.0 - dp
     v = (ev.impose_ic(fr.IC_EXPLOSION, prm_p) - ev.impose_ic(fr.IC_EXPLOSION, prm_m)) / (2.0 * dp * CF.WARPED_PRM[1])
-    eps = 1e-3
-
     def J(u):
         return A.rho_squared(ev, A.UnsteadyAdjoint(ev, dt, T).run(u), m)[0]
-    (jp, jm), t_fd = timed(lambda: (J(u0 + eps * v), J(u0 - eps * v)))
-    fd = (jp - jm) / (2 * eps)
     ad = float(torch.dot(lam, v))
+    fds = []
+    for eps in (float(x) for x in args.adj_fd_eps.split(",")):
+        (jp, jm), t_fd = timed(lambda: (J(u0 + eps * v), J(u0 - eps * v)))
+        fds.append({"eps": eps, "dJ_v": (jp - jm) / (2 * eps), "rel_diff": abs(ad - (jp - jm) / (2 * eps)) /
+                    abs((jp - jm) / (2 * eps)), "seconds": t_fd})
+        print(f"[adjoint-large] FD eps {eps:g}: {fds[-1]['dJ_v']:.9e} (adjoint {ad:.9e})", file=sys.stderr, flush=True)
+    fd, eps, t_fd = fds[0]["dJ_v"], fds[0]["eps"], fds[0]["seconds"]
     return {"workload": f"warped {args.adj_n}^2 p=7 VdW HLLC+filter, SSP-RK3 x {T}, J = int rho^2",
             "ndofs": ev.ndofs, "dt": dt, "snapshots": args.adj_snapshots,
             "repetition_number": A.repetitions(T, args.adj_snapshots),
             "forward_s": t_fwd, "adjoint_s": t_adj, "adjoint_over_forward": t_adj / t_fwd,
             "forward_health_flags": health, "recomputed_forward_steps": adj.forward_steps,
             "peak_snapshots": adj.peak_snapshots, "J": J0, "dJ_v_adjoint": ad, "dJ_v_central_fd": fd,
-            "rel_diff": abs(ad - fd) / abs(fd), "fd_eps": eps, "fd_seconds": t_fd}
+            "rel_diff": abs(ad - fd) / abs(fd), "fd_eps": eps, "fd_seconds": t_fd, "fd_by_eps": fds}
 
 
 def cpu_baseline(args, dt):
