@@ -52,6 +52,12 @@ RESIDUAL_CASES = {
     "explosion_hllc_vdw_p2": C.explosion_cfg(7, 4, 2, gas=1, riemann=1),
     "bump_p5": C.bump_cfg(),
     "warped_vdw_p7": C.warped_cfg(),
+    # curved meshes at low and odd orders: the even-odd line contractions with a
+    # middle node (n = 3, 5), curved tiles of 64 and 10 elements, the LDS of the
+    # largest tangent and adjoint tiles (n = 2)
+    "warped_vdw_p1": C.warped_cfg(6, 5, 1),
+    "bump_p2": C.bump_cfg(9, 4, 2),
+    "warped_vdw_p4": C.warped_cfg(6, 4, 4),
     "wall_p2_rusanov": C.wall_cfg(),
     "wall_p3_hllc": C.wall_cfg(3, 5, 3, 1),
     "vcjh_c_p3": dict(C.vortex_cfg(4, 4, 3), c_vcjh=0.01),
@@ -99,7 +105,7 @@ def test_complex_step_parity(name):
 
 
 @pytest.mark.parametrize("name", ["vortex_p4", "explosion_hllc_vdw_p2", "warped_vdw_p7", "wall_p3_hllc",
-                                  "sod1d_p4_hllc_periodic"])
+                                  "sod1d_p4_hllc_periodic", "warped_vdw_p1", "warped_vdw_p4"])
 def test_tangent_vs_complex_step(name):
     """Forward tangent J*v against the oracle's complex step (h = 1e-7, error O(h^2))."""
     torch, fr, Oracle = _mods()
@@ -155,7 +161,8 @@ def test_filter_parity(name):
 
 
 @pytest.mark.parametrize("name,stages", [("vortex_p4", 3), ("vortex_p4", 4), ("warped_vdw_p7", 3),
-                                         ("explosion_rusanov_vdw_p6", 3), ("sod1d_p3_rusanov", 3)])
+                                         ("explosion_rusanov_vdw_p6", 3), ("sod1d_p3_rusanov", 3),
+                                         ("warped_vdw_p4", 3), ("warped_vdw_p1", 3)])
 def test_ssprk_parity(name, stages):
     torch, fr, Oracle = _mods()
     cfg = dict(RESIDUAL_CASES[name])
@@ -211,7 +218,8 @@ def test_initial_condition_and_error(name, kind, prm):
 
 ADJ_CASES = ["vortex_p4", "vortex_p4_1x1", "vortex_p1", "vortex_p8", "explosion_rusanov_ideal_p3",
              "explosion_hllc_vdw_p2", "bump_p5", "warped_vdw_p7", "wall_p3_hllc", "vcjh_c_p3",
-             "sod1d_p3_rusanov", "sod1d_p4_hllc_periodic", "sod1d_wall_p2"]
+             "sod1d_p3_rusanov", "sod1d_p4_hllc_periodic", "sod1d_wall_p2", "warped_vdw_p1", "bump_p2",
+             "warped_vdw_p4"]
 
 
 @pytest.mark.parametrize("name", ADJ_CASES)
