@@ -604,11 +604,12 @@ namespace frsdg
         dbl4 acc[3];
 #pragma unroll
         for (int m = 0; m < 3; ++m) acc[m] = dbl4{0.0, 0.0, 0.0, 0.0};
-        // 48 x 16 values of A (rows r0.., k) and 16 x 48 of B (k, columns c0..) per slab, 3 each per thread
-        double pa[3], pb[3];
+        // TT x TKB values of A (rows r0.., k) and TKB x TT of B (k, columns c0..) per slab, FM each per thread
+        constexpr int FM = TT * TKB / 256;
+        double pa[FM], pb[FM];
         auto fetch = [&](int k0) {
 #pragma unroll
-            for (int m = 0; m < 3; ++m)
+            for (int m = 0; m < FM; ++m)
             {
                 const int o = tid + 256 * m;
                 const int q = o / TT, s = o - q * TT;    // A: column k0 + q of the tile rows, s down the column
@@ -623,7 +624,7 @@ namespace frsdg
         for (int k0 = 0; k0 < nd; k0 += TKB)
         {
 #pragma unroll
-            for (int m = 0; m < 3; ++m)
+            for (int m = 0; m < FM; ++m)
             {
                 const int o = tid + 256 * m;
                 const int q = o / TT, s = o - q * TT, cq = o / TKB, cs = o - cq * TKB;
@@ -833,7 +834,8 @@ namespace frsdg
     // A level holds few rows; spreading each over nd/16 workgroups puts the
     // whole chip on it.  ND = block size as a template parameter (0: run time)
     // so that a thread's loads are all in flight together.
-    constexpr int SWR = 16; // rows per workgroup
+    constexpr int SWR = 16;       // rows per workgroup
+    constexpr int SWG = 256 / SWR; // column groups: thread (row, g) sums columns g, g + SWG, ...
     template <int ND, class T>
     __device__ inline double sweep_dot(int nd, int r, int g, const T* __restrict__ B,
                                        const double* __restrict__ v)
@@ -842,11 +844,11 @@ namespace frsdg
         if constexpr (ND > 0)
         {
 #pragma unroll
-            for (int c = g; c < ND; c += 16) acc = ::fma(static_cast<double>(B[r + c * ND]), v[c], acc);
+            for (int c = g; c < ND; c += SWG) acc = ::fma(static_cast<double>(B[r + c * ND]), v[c], acc);
         }
         else
         {
-            for (int c = g; c < nd; c += 16)
+            for (int c = g; c < nd; c += SWG)
                 acc = ::fma(static_cast<double>(B[r + static_cast<long long>(c) * nd]), v[c], acc);
         }
         return acc;
@@ -861,12 +863,12 @@ namespace frsdg
                                                        const T* __restrict__ Dinv,
                                                        const double* __restrict__ in, double* out)
     {
-        __shared__ double red[16][SWR + 1];
+        __shared__ double red[SWG][SWR + 1];
         if constexpr (ND > 0) nd = ND;
         const long long pos = off + blockIdx.x;
         const long long I   = lvl[pos];
         const long long bsz = static_cast<long long>(nd) * nd;
-        const int       rl  = threadIdx.x & (SWR - 1), g = threadIdx.x >> 4;
+        const int       rl  = threadIdx.x % SWR, g = threadIdx.x / SWR;
         const int       r   = blockIdx.y * SWR + rl;
         int sl[CMAXN], el[CMAXN];
 #pragma unroll
@@ -885,7 +887,7 @@ namespace frsdg
             // clamped into the block (rows and columns past ND are discarded or
             // multiplied by zero), so no load is masked.  Per thread the sums
             // run in the same order as sweep_dot's.
-            constexpr int NC = (ND + 15) / 16, SWB = 5;
+            constexpr int NC = (ND + SWG - 1) / SWG, SWB = 5;
             int nb = 1;
 #pragma unroll
             for (int q = 0; q < CMAXN; ++q) nb += sl[q] >= 0 ? 1 : 0;
@@ -907,7 +909,7 @@ namespace frsdg
 #pragma unroll
                         for (int i = 0; i < NC; ++i)
                         {
-                            const int c = g + 16 * i, cc = c < ND ? c : ND - 1;
+                            const int c = g + SWG * i, cc = c < ND ? c : ND - 1;
                             bv[t][i] = static_cast<double>(B[rr + cc * ND]);
                             const double x = V[cc];
                             vv[t][i] = c < ND ? x : 0.0;
@@ -948,7 +950,7 @@ namespace frsdg
         {
             double sum = 0.0;
 #pragma unroll
-            for (int k = 0; k < 16; ++k) sum += red[k][rl];
+            for (int k = 0; k < SWG; ++k) sum += red[k][rl];
             out[I * nd + r] = sum;
         }
     }
