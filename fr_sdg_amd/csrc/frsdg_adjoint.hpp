@@ -325,25 +325,56 @@ namespace frsdg
 #pragma unroll
                 for (int q = 0; q < 4; ++q) UB[q] = f.Uext[q];
         }
-        // Riemann flux cotangent: both copies of the face (the outer copy of a
-        // wrap with differing ends separately)
-        double cs[4], bA[4] = {0, 0, 0, 0}, bB[4] = {0, 0, 0, 0};
+        // The cotangents of the Riemann flux (both copies of the face; the outer
+        // copy of a wrap with differing ends separately) are formed first, and
+        // the own-normal-flux parts -s Wbar . n of the tile sides go to the
+        // face slots of Cs (each slot belongs to this face task alone) before
+        // the Riemann sweep, whose part is added afterwards: neither the lifted
+        // cotangents wA, wB nor those parts are held in registers across it.
+        double cs[4], c2[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
         {
             if (!f.wrap)
                 cs[q] = f.sc * f.wA[q] + f.sc * f.wB[q];
             else
+            {
                 cs[q] = f.sc * (f.elA >= 0 ? f.wA[q] : f.wB[q]);
+                c2[q] = f.sc1 * (f.elA >= 0 ? f.wB[q] : f.wA[q]);
+            }
         }
-        riemann2_vjp(P.gas, P.riemann, UA, UB, f.fnx, f.fny, cs, bA, bB);
-        if (f.wrap)
+        const int fa = f.xdir ? 1 : 3, fb = f.xdir ? 0 : 2;
+        if (f.elA >= 0)
         {
-            double c2[4];
+            double Fb[4], Gb[4], b[4] = {0, 0, 0, 0};
 #pragma unroll
-            for (int q = 0; q < 4; ++q) c2[q] = f.sc1 * (f.elA >= 0 ? f.wB[q] : f.wA[q]);
-            riemann2_vjp(P.gas, P.riemann, UA, UB, f.nx1, f.ny1, c2, bA, bB);
+            for (int q = 0; q < 4; ++q)
+            {
+                const double cn = -f.sc * f.wA[q];
+                Fb[q]           = cn * f.fnx;
+                Gb[q]           = cn * f.fny;
+            }
+            flux2_vjp(P.gas, UA, Fb, Gb, b);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sm.Cs[f.elA][fa][f.idx][q] = b[q];
         }
+        if (f.elB >= 0)
+        {
+            double Fb[4], Gb[4], b[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                const double cn = -f.sc * f.wB[q];
+                Fb[q]           = cn * f.fnx;
+                Gb[q]           = cn * f.fny;
+            }
+            flux2_vjp(P.gas, UB, Fb, Gb, b);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sm.Cs[f.elB][fb][f.idx][q] = b[q];
+        }
+        double bA[4] = {0, 0, 0, 0}, bB[4] = {0, 0, 0, 0};
+        riemann2_vjp(P.gas, P.riemann, UA, UB, f.fnx, f.fny, cs, bA, bB);
+        if (f.wrap) riemann2_vjp(P.gas, P.riemann, UA, UB, f.nx1, f.ny1, c2, bA, bB);
         // physical boundary: the exterior state is a function of the own trace
         if (f.bdir >= 0)
         {
@@ -352,36 +383,15 @@ namespace frsdg
             else
                 bc_vjp(P.bc, f.bdir, UA, f.fnx, f.fny, bB, bA);
         }
-        // own normal flux -s Wbar . n of each tile side
         if (f.elA >= 0)
         {
-            double Fb[4], Gb[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-            {
-                const double cn = -f.sc * f.wA[q];
-                Fb[q]           = cn * f.fnx;
-                Gb[q]           = cn * f.fny;
-            }
-            flux2_vjp(P.gas, UA, Fb, Gb, bA);
-            const int fa = f.xdir ? 1 : 3;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) sm.Cs[f.elA][fa][f.idx][q] = bA[q];
+            for (int q = 0; q < 4; ++q) sm.Cs[f.elA][fa][f.idx][q] += bA[q];
         }
         if (f.elB >= 0)
         {
-            double Fb[4], Gb[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-            {
-                const double cn = -f.sc * f.wB[q];
-                Fb[q]           = cn * f.fnx;
-                Gb[q]           = cn * f.fny;
-            }
-            flux2_vjp(P.gas, UB, Fb, Gb, bB);
-            const int fb = f.xdir ? 0 : 2;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) sm.Cs[f.elB][fb][f.idx][q] = bB[q];
+            for (int q = 0; q < 4; ++q) sm.Cs[f.elB][fb][f.idx][q] += bB[q];
         }
     }
 
@@ -439,11 +449,22 @@ namespace frsdg
         }
     }
 
+    // Register budget of the adjoint tiles: 168 VGPRs (three waves per SIMD,
+    // three tiles per CU, which the LDS of n >= 3 allows) for 8-18 spilled
+    // registers in the reverse Riemann sweep, against 180-190 VGPRs and two
+    // waves unconstrained: measured 17 % faster on warped 1024^2 p=7 (4.44 ->
+    // 3.74 ms) and 11 % on the 128^2 p=4 vortex.  n = 2 tiles (66 KB of LDS,
+    // two per CU) stay unconstrained.
+    template <int N>
+    constexpr int adj_waves()
+    {
+        return N >= 3 ? 3 : 1;
+    }
+
     template <int N, bool CURVED>
-    __global__ __launch_bounds__(256) void k_adj2d_tile(const Res2DP P, const double* __restrict__ u,
-                                                        const double* __restrict__ w,
-                                                        const double* __restrict__ wb, const AdjOut ao,
-                                                        double* __restrict__ out)
+    __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(adj_waves<N>())))
+    void k_adj2d_tile(const Res2DP P, const double* __restrict__ u, const double* __restrict__ w,
+                      const double* __restrict__ wb, const AdjOut ao, double* __restrict__ out)
     {
         using T           = Tile2D<N, double>;
         constexpr int BS  = 256;
@@ -489,7 +510,9 @@ namespace frsdg
         // one phase: face tasks and line tasks (xi and eta; curved: added into
         // FX, GX) read U and W and write disjoint arrays.  Where the face tasks
         // fit in waves 0-1 and the line tasks twice in waves 2-3 (curved n = 8)
-        // the two run on separate waves; otherwise line tasks first, then faces.
+        // the two run on separate waves; otherwise line tasks first, then faces
+        // (two loops: one loop over both kinds holds a line task's registers
+        // across the face sweep, 250 VGPRs against 190).
         if (CURVED && nft <= BS / 2 && nl <= BS)
         {
             if (tid < BS / 2)
@@ -514,22 +537,19 @@ namespace frsdg
         }
         else
         {
-            for (int t = tid; t < nl + nft; t += BS)
+            for (int t = tid; t < nl; t += BS)
             {
-                if (t < nl)
-                {
-                    const bool eta = t >= nlt;
-                    const int  wq  = eta ? t - nlt : t;
-                    const int  le  = (wq >> 2) / N;
-                    if (sm.eid[le] < 0) continue;
-                    adj_line<N, CURVED>(P, sm, eta, le, (wq >> 2) - le * N, wq & 3);
-                }
-                else
-                {
-                    AdjFace f;
-                    adj_face_setup<N>(P, u, wb, g, t - nl, f);
-                    if (f.live) adj_face_eval<N>(P, f, sm);
-                }
+                const bool eta = t >= nlt;
+                const int  wq  = eta ? t - nlt : t;
+                const int  le  = (wq >> 2) / N;
+                if (sm.eid[le] < 0) continue;
+                adj_line<N, CURVED>(P, sm, eta, le, (wq >> 2) - le * N, wq & 3);
+            }
+            for (int t = tid; t < nft; t += BS)
+            {
+                AdjFace f;
+                adj_face_setup<N>(P, u, wb, g, t, f);
+                if (f.live) adj_face_eval<N>(P, f, sm);
             }
         }
         __syncthreads();

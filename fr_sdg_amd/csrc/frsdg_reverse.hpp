@@ -57,17 +57,18 @@ namespace frsdg
         else if (g.kind == 0)
         {
             // c = sqrt(gamma p ir)
-            const double k = cb * g.gamma / (2.0 * t.c);
+            const double k = cb * g.gamma * (0.5 * rcp_(t.c));
             pb += k * t.ir;
             irb += k * t.p;
         }
         else
         {
             const double cov  = 1.0 - g.b * t.rho;
-            const double invd = 1.0 / (cov * cov);
-            const double dinv = 2.0 * g.b * invd / cov; // d invd / d rho
+            const double icov = rcp_(cov);
+            const double invd = icov * icov;
+            const double dinv = 2.0 * g.b * invd * icov; // d invd / d rho
             const double th   = t.e + g.a * t.rho;
-            const double c2b  = cb / (2.0 * t.c);
+            const double c2b  = cb * (0.5 * rcp_(t.c));
             eb += c2b * g.gamma * gm * invd;
             rb += c2b * (g.gamma * gm * (g.a * invd + th * dinv) + gm * g.a * (2.0 - 2.0 * g.b * t.rho) * invd +
                          gm * g.a * t.rho * (2.0 - g.b * t.rho) * dinv - 2.0 * g.a);
@@ -80,10 +81,10 @@ namespace frsdg
         }
         else
         {
-            const double cov = 1.0 - g.b * t.rho;
-            const double th  = t.e + g.a * t.rho;
-            eb += pb * gm * t.rho / cov;
-            rb += pb * (gm * (th + g.a * t.rho) / cov + gm * t.rho * th * g.b / (cov * cov) - 2.0 * g.a * t.rho);
+            const double icov = rcp_(1.0 - g.b * t.rho);
+            const double th   = t.e + g.a * t.rho;
+            eb += pb * gm * t.rho * icov;
+            rb += pb * (gm * (th + g.a * t.rho) * icov + gm * t.rho * th * g.b * (icov * icov) - 2.0 * g.a * t.rho);
         }
         // e = E ir - (u^2 + v^2)/2
         const double Eb = eb * t.ir;
@@ -124,7 +125,7 @@ namespace frsdg
     FR_HD double sabs_d(double x) // d smooth_abs / dx
     {
         const double eps = 1e-7;
-        return x / ::sqrt(x * x + eps * eps);
+        return quot_(x, sqrt_(x * x + eps * eps));
     }
 
     // ULb += (dF*/dUL)^T fb, URb += (dF*/dUR)^T fb   (riemann2, double)
@@ -258,7 +259,8 @@ namespace frsdg
                     sRb += sKb; unRb += unKb; utRb += utKb; U0Rb += rKb; mRb += mKb; U3Rb += EKb; pRb += pKb;
                 }
                 // sM = num / den
-                const double numb = sMb / den, denb = -sMb * sM / den;
+                const double iden = rcp_(den);
+                const double numb = sMb * iden, denb = -numb * sM;
                 pRb += numb;
                 pLb -= numb;
                 unLb += numb * rLd;
@@ -327,7 +329,7 @@ namespace frsdg
         {
             if (bc.freeze) return; // lambda-bound: no derivative (complex-step semantics)
             // ex = [r, r (u - 2 vn nx), r (v - 2 vn ny), E], u = U1/r, v = U2/r, vn = u nx + v ny
-            const double r = U[0], u = U[1] / r, v = U[2] / r, vn = u * nx + v * ny;
+            const double r = U[0], ir = rcp_(r), u = U[1] * ir, v = U[2] * ir, vn = u * nx + v * ny;
             const double e1 = u - 2.0 * vn * nx, e2 = v - 2.0 * vn * ny;
             double       rb = exb[0] + exb[1] * e1 + exb[2] * e2;
             const double e1b = exb[1] * r, e2b = exb[2] * r;
@@ -335,9 +337,9 @@ namespace frsdg
             const double vnb = -2.0 * (e1b * nx + e2b * ny);
             ub += vnb * nx;
             vb += vnb * ny;
-            Ub[1] += ub / r;
-            Ub[2] += vb / r;
-            rb -= (ub * U[1] + vb * U[2]) / (r * r);
+            Ub[1] += ub * ir;
+            Ub[2] += vb * ir;
+            rb -= (ub * U[1] + vb * U[2]) * (ir * ir);
             Ub[0] += rb;
             Ub[3] += exb[3];
         }
