@@ -1239,18 +1239,21 @@ namespace frsdg
     // state, V tangent state, F / G the tangent fluxes F'(u) v, G'(u) v; the
     // same 256-thread tiles, face tasks and line tasks as the residual.
     // ------------------------------------------------------------------
-    // LDS of one tangent tile: primal state U and tangent state T of the
-    // nodes (face traces), tangent fluxes F'(u) v, G'(u) v, and (curved) V,
-    // the volume term, both line parts added into it by LDS atomics (zeroed
-    // per node first); the residual tile's padding and equation order.
+    // LDS of one tangent tile: the primal and tangent traces of each element
+    // face (Tr[el][face][idx]: u then v of the four equations; faces 0 left,
+    // 1 right, 2 bottom, 3 top, as Del), which is all the face tasks read of
+    // the states, tangent fluxes F'(u) v, G'(u) v, and (curved) V, the volume
+    // term, both line parts added into it by LDS atomics (zeroed per node
+    // first); the residual tile's padding and equation order.  Keeping the
+    // traces instead of both whole states holds the curved n = 8 tile to
+    // 39.1 KB (four tiles per CU).
     template <int N, bool CURVED>
     struct TanSmem
     {
         static constexpr int NN = N * N, EPB = Tile2D<N, double>::EPB;
         static constexpr int RP = TileSmem<N, double, true>::RP;
         __device__ static __forceinline__ int pos(int l, int k, int q) { return 4 * k + (q ^ (l & 3)); }
-        double    U[EPB][N][RP];
-        double    T[EPB][N][RP];
+        double    Tr[EPB][4][N][8];
         double    F[EPB][N][RP];
         double    G[EPB][N][RP];
         double    V[CURVED ? EPB : 1][N][CURVED ? N * 4 : 1];
@@ -1264,18 +1267,19 @@ namespace frsdg
     {
         const int kA = f.xdir ? N - 1 : f.idx, lA = f.xdir ? f.idx : N - 1;
         const int kB = f.xdir ? 0 : f.idx, lB = f.xdir ? f.idx : 0;
+        const int fA = f.xdir ? 1 : 3, fB = f.xdir ? 0 : 2;
         Dual      UA[4], UB[4], Fs[4];
         if (f.elA >= 0)
         {
+            const double* t = sm.Tr[f.elA][fA][f.idx];
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                UA[q] = {sm.U[f.elA][lA][sm.pos(lA, kA, q)], sm.T[f.elA][lA][sm.pos(lA, kA, q)]};
+            for (int q = 0; q < 4; ++q) UA[q] = {t[q], t[4 + q]};
         }
         if (f.elB >= 0)
         {
+            const double* t = sm.Tr[f.elB][fB][f.idx];
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                UB[q] = {sm.U[f.elB][lB][sm.pos(lB, kB, q)], sm.T[f.elB][lB][sm.pos(lB, kB, q)]};
+            for (int q = 0; q < 4; ++q) UB[q] = {t[q], t[4 + q]};
         }
         if (f.elA < 0)
         {
@@ -1298,7 +1302,6 @@ namespace frsdg
             }
         }
         riemann2(P.gas, P.riemann, UA, UB, f.fnx, f.fny, Fs);
-        const int fA = f.xdir ? 1 : 3, fB = f.xdir ? 0 : 2;
         // own-side normal flux tangent: curved tiles read F'v, G'v from LDS;
         // affine tiles find it in their own Del slot, put there by the trace
         // node (the line tasks overwrite F, G in place meanwhile)
@@ -1374,12 +1377,25 @@ namespace frsdg
             for (int q = 0; q < 4; ++q)
             {
                 vt[q]                        = own[q].d;
-                sm.U[el][l][sm.pos(l, k, q)] = own[q].v;
-                sm.T[el][l][sm.pos(l, k, q)] = own[q].d;
                 sm.F[el][l][sm.pos(l, k, q)] = F[q].d;
                 sm.G[el][l][sm.pos(l, k, q)] = G[q].d;
                 if constexpr (CURVED) sm.V[el][l][sm.pos(l, k, q)] = 0.0;
             }
+            // the node's traces (corner nodes lie on two faces)
+            auto trace = [&](int fc, int idx)
+            {
+                double* t = sm.Tr[el][fc][idx];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                {
+                    t[q]     = own[q].v;
+                    t[4 + q] = own[q].d;
+                }
+            };
+            if (k == 0) trace(0, l);
+            if (k == N - 1) trace(1, l);
+            if (l == 0) trace(2, k);
+            if (l == N - 1) trace(3, k);
             if constexpr (!CURVED)
             {
                 // affine faces have normals (1, 0) and (0, 1): the trace nodes
