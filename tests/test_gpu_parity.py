@@ -58,6 +58,10 @@ RESIDUAL_CASES = {
     "warped_vdw_p1": C.warped_cfg(6, 5, 1),
     "bump_p2": C.bump_cfg(9, 4, 2),
     "warped_vdw_p4": C.warped_cfg(6, 4, 4),
+    # curved and periodic, with x-ends whose face metrics differ (X from -1 to
+    # 0.5): each copy of a wrap face carries its own metrics, so the adjoint
+    # evaluates the outer copy's Riemann flux a second time
+    "warped_vdw_p3_periodic": dict(C.warped_cfg(5, 4, 3), bc_type=C.PER, lx=1.5),
     "wall_p2_rusanov": C.wall_cfg(),
     "wall_p3_hllc": C.wall_cfg(3, 5, 3, 1),
     "vcjh_c_p3": dict(C.vortex_cfg(4, 4, 3), c_vcjh=0.01),
@@ -105,7 +109,8 @@ def test_complex_step_parity(name):
 
 
 @pytest.mark.parametrize("name", ["vortex_p4", "explosion_hllc_vdw_p2", "warped_vdw_p7", "wall_p3_hllc",
-                                  "sod1d_p4_hllc_periodic", "warped_vdw_p1", "warped_vdw_p4"])
+                                  "sod1d_p4_hllc_periodic", "warped_vdw_p1", "warped_vdw_p4",
+                                  "warped_vdw_p3_periodic"])
 def test_tangent_vs_complex_step(name):
     """Forward tangent J*v against the oracle's complex step (h = 1e-7, error O(h^2))."""
     torch, fr, Oracle = _mods()
@@ -162,7 +167,7 @@ def test_filter_parity(name):
 
 @pytest.mark.parametrize("name,stages", [("vortex_p4", 3), ("vortex_p4", 4), ("warped_vdw_p7", 3),
                                          ("explosion_rusanov_vdw_p6", 3), ("sod1d_p3_rusanov", 3),
-                                         ("warped_vdw_p4", 3), ("warped_vdw_p1", 3)])
+                                         ("warped_vdw_p4", 3), ("warped_vdw_p1", 3), ("warped_vdw_p3_periodic", 3)])
 def test_ssprk_parity(name, stages):
     torch, fr, Oracle = _mods()
     cfg = dict(RESIDUAL_CASES[name])
@@ -219,7 +224,7 @@ def test_initial_condition_and_error(name, kind, prm):
 ADJ_CASES = ["vortex_p4", "vortex_p4_1x1", "vortex_p1", "vortex_p8", "explosion_rusanov_ideal_p3",
              "explosion_hllc_vdw_p2", "bump_p5", "warped_vdw_p7", "wall_p3_hllc", "vcjh_c_p3",
              "sod1d_p3_rusanov", "sod1d_p4_hllc_periodic", "sod1d_wall_p2", "warped_vdw_p1", "bump_p2",
-             "warped_vdw_p4"]
+             "warped_vdw_p4", "warped_vdw_p3_periodic"]
 
 
 @pytest.mark.parametrize("name", ADJ_CASES)
@@ -272,7 +277,7 @@ def test_adjoint_hllc_imaginary_sound_speed():
     assert abs(lhs - rhs) <= 1e-12 * float(torch.linalg.norm(w) * torch.linalg.norm(jv)), (lhs, rhs)
 
 
-@pytest.mark.parametrize("name", ["vortex_p4", "bump_p5", "sod1d_p3_rusanov"])
+@pytest.mark.parametrize("name", ["vortex_p4", "bump_p5", "sod1d_p3_rusanov", "warped_vdw_p3_periodic"])
 def test_adjoint_columns_vs_complex_step(name):
     """J^T e_i against rows of J built column by column from the oracle's complex step
     on a tiny mesh (reference-compatible BC mode, so complex step is exact)."""
