@@ -38,7 +38,9 @@ namespace frsdg
         }
     };
 
-    // lifted cotangents of the face copies: Wbar[e][f][idx][q]
+    // lifted cotangents of the face copies: Wbar[e][f][idx][q].  One thread
+    // per element line: row idx gives the x faces 0 (with dL) and 1 (dR),
+    // column idx the y faces 2 and 3, from one read of the line.
     template <int N>
     __global__ __launch_bounds__(256) void k_adj_lift(long long E, const double* __restrict__ tab,
                                                        const double* __restrict__ w,
@@ -46,21 +48,36 @@ namespace frsdg
     {
         constexpr int NN = N * N;
         const long long t = blockIdx.x * 256LL + threadIdx.x;
-        if (t >= E * 4 * N) return;
-        const long long e   = t / (4 * N);
-        const int       ft  = static_cast<int>(t - e * 4 * N);
-        const int       f   = ft / N, idx = ft - (ft / N) * N;
-        const double*   g   = tab + NN + ((f == 0 || f == 2) ? 0 : N); // dL or dR
+        if (t >= E * 2 * N) return;
+        const long long e   = t / (2 * N);
+        const int       pt  = static_cast<int>(t - e * 2 * N);
+        const int       yf  = pt / N, idx = pt - yf * N; // yf: column (y faces) or row (x faces)
+        const double*   gL  = tab + NN;
+        const double*   gR  = tab + NN + N;
         const double*   we  = w + e * NN * 4;
-        double          acc[4] = {0, 0, 0, 0};
+        double          a0[4] = {0, 0, 0, 0}, a1[4] = {0, 0, 0, 0};
+#pragma unroll
         for (int k = 0; k < N; ++k)
         {
-            const int node = (f < 2) ? (idx * N + k) : (k * N + idx); // x faces: row l=idx; y faces: column k=idx
+            const int    node = yf ? (k * N + idx) : (idx * N + k);
+            const double2 v0  = *reinterpret_cast<const double2*>(we + node * 4);
+            const double2 v1  = *reinterpret_cast<const double2*>(we + node * 4 + 2);
+            const double  x[4] = {v0.x, v0.y, v1.x, v1.y};
+            const double  l = gL[k], r = gR[k];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc[q] += we[node * 4 + q] * g[k];
+            for (int q = 0; q < 4; ++q)
+            {
+                a0[q] += x[q] * l;
+                a1[q] += x[q] * r;
+            }
         }
+        double* o = wb + ((e * 4 + 2 * yf) * N + idx) * 4; // face 2 yf, then face 2 yf + 1 (N entries on)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) wb[t * 4 + q] = acc[q];
+        for (int q = 0; q < 4; ++q)
+        {
+            o[q]         = a0[q];
+            o[N * 4 + q] = a1[q];
+        }
     }
 
     // ------------------------------------------------------------------

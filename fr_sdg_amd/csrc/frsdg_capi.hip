@@ -469,7 +469,7 @@ namespace
         static int run(const frsdg_ctx* c, const double* u, const double* w, const AdjOut& ao, double* out,
                        hipStream_t s)
         {
-            const long long nl = c->E * 4 * N;
+            const long long nl = c->E * 2 * N;
             hipLaunchKernelGGL((k_adj_lift<N>), dim3(static_cast<unsigned>((nl + 255) / 256)), dim3(256), 0, s,
                                c->E, c->d_tab, w, c->wb);
             using T            = Tile2D<N, double>;
