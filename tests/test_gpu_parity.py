@@ -475,3 +475,42 @@ def test_batched_sweep_matches_per_instance_and_oracle():
             b.ssprk_step(ud)
         assert np.isfinite(_host(uc)).all()
         assert np.array_equal(_host(uc), _host(ud)), (grid, mode)
+
+
+def test_batched_tile_shapes_every_order():
+    """Every order n = 3..9 on small ragged meshes (two per order, ideal and
+    van der Waals): the three element tile shapes of the batched sweep
+    (default, halved, fewest rim faces per element) against the per-instance
+    launches, and against the oracle's step."""
+    torch, fr, Oracle = _mods()
+    from fr_sdg_amd import configs as CF
+    cfgs = []
+    for p in range(2, 9):
+        cfgs.append(CF.explosion(7, p, vdw=False))
+        cfgs.append(dict(CF.explosion(5, p, vdw=True), ny=6))
+    b = fr.Batch(cfgs)
+    u = b.empty()
+    for k, iv in enumerate(b.instances):
+        iv.impose_ic(fr.IC_EXPLOSION, CF.EXPLOSION_SMOOTH_PRM, out=b.view(u, k))
+    dts = [2e-4 * (1 + (k % 3)) for k in range(len(cfgs))]
+    b.set_dt(dts)
+    u0 = _host(u).copy()
+    ue = u.clone()
+    b.ssprk_step_each(ue)
+    ref = _host(ue)
+    for shape in (0, 1, 2):
+        print(f"[parity] batched tile shape {shape}", flush=True)
+        b.set_tuning(0, 1, 1)
+        for n in b.orders():
+            b.set_tile_shape(n, shape)
+        ua = u.clone()
+        b.ssprk_step(ua)
+        got = _host(ua)
+        assert np.isfinite(got).all(), shape
+        for k in range(len(cfgs)):
+            sl = slice(b.offsets[k], b.offsets[k + 1])
+            assert C.rel_l2(got[sl], ref[sl]) <= 1e-14, (shape, k, cfgs[k]["p"])
+    for k, cfg in enumerate(cfgs):
+        o = Oracle(cfg)
+        r = o.ssprk_step(u0[b.offsets[k]:b.offsets[k + 1]], dts[k])
+        assert C.rel_l2(ref[b.offsets[k]:b.offsets[k + 1]], r) <= TOL, (k, cfg["p"])
