@@ -1,5 +1,6 @@
 """Pins of the oracle's pointwise physics and residual by identities the
-mathematics fixes (the reference ships no golden vectors: SURVEY 8c).
+mathematics fixes (the reference's own tests hold no golden vectors for the
+residual).
 
 * Riemann solvers: consistency F*(U,U,n) = F(U).n, rotational invariance,
   conservation F*(UL,UR,n) = -F*(UR,UL,-n), supersonic upwinding.
