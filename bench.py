@@ -201,7 +201,7 @@ def read_traffic(name):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only (no multi-GPU path); use --gpus 1",
+        print(json.dumps({"error": "libfrsdg runs on one GPU only (no multi-GPU path); use --gpus 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
