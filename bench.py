@@ -499,11 +499,16 @@ def bench_ilu(torch, fr):
     del P, Dinv
     torch.cuda.empty_cache()
     its = {}
-    for name, pc in (("block_jacobi", 1), ("gauss_seidel", 2), ("ilu_fd", 3), ("ilu_exact", 4)):
+    # the last two: flexible GMRES with the ILU(0) factors applied in double and
+    # in single precision (pc_fp32: half the bytes per application)
+    for name, pc, extra in (("block_jacobi", 1, {}), ("gauss_seidel", 2, {}), ("ilu_fd", 3, {}), ("ilu_exact", 4, {}),
+                            ("ilu_fd_fgmres", 3, {"ksp_pc_side": 2}),
+                            ("ilu_fd_fgmres_fp32", 3, {"ksp_pc_side": 2, "pc_fp32": 1})):
         u = u0.clone()
         t, st = wall(lambda: ev.beuler_step(u, 1.0 / sigma, jac_type=0, precond=pc, ksp_restart=60,
-                                            ksp_max_it=3000, ksp_rtol=1e-3, snes_rtol=1e-3, snes_max_it=1))
-        its[name] = {"gmres_its": st["ksp_its"], "s": t}
+                                            ksp_max_it=3000, ksp_rtol=1e-3, snes_rtol=1e-3, snes_max_it=1, **extra))
+        its[name] = {"gmres_its": st["ksp_its"], "s": t,
+                     "apply_ms_per_it": 1e3 * st["t_pc_apply"] / max(1, st["ksp_its"])}
         progress(f"ilu: one Newton iteration with {name}: {st['ksp_its']} GMRES iterations")
     out["one_newton_iteration"] = its
     del ev, u0
