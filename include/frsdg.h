@@ -239,7 +239,10 @@ typedef struct frsdg_newton_opts
                            (PETSc default) */
     int    pc_fp32;     /* preconditioners 3/4: apply single-precision copies of the ILU(0)
                            factors (accumulation in double); halves the bytes each
-                           application reads.  The transposed apply stays double. */
+                           application reads, but where the factors are badly
+                           conditioned (the 256x64 p=5 bump matrix at free stream)
+                           their rounding leaves no useful preconditioner.  The
+                           transposed apply stays double. */
 } frsdg_newton_opts;
 
 typedef struct frsdg_newton_stats
