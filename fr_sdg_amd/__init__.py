@@ -101,7 +101,7 @@ EXPORTS = [
     "frsdg_filter_transpose", "frsdg_mass", "frsdg_boundary_weights", "frsdg_adjoint_solve", "frsdg_pseudo_run",
     "frsdg_color_jacobian", "frsdg_ilu0_factor", "frsdg_ilu0_apply", "frsdg_linear_solve",
     "frsdg_ilu0_apply_transpose", "frsdg_l2_error_noalloc", "frsdg_unsteady_adjoint", "frsdg_step_health",
-    "frsdg_check_state", "frsdg_set_stop_mask",
+    "frsdg_check_state", "frsdg_set_stop_mask", "frsdg_batch_health",
 ]
 
 # health flags of stored states (frsdg_step_health / frsdg_check_state)
@@ -161,6 +161,7 @@ def load_library(path=LIB_PATH):
     L.frsdg_batch_set_tile_shape.argtypes = [vp, i, i]
     L.frsdg_batch_ssprk_step.argtypes = [vp, dp, vp]
     L.frsdg_batch_ssprk_run.argtypes = [vp, dp, i, vp]
+    L.frsdg_batch_health.argtypes = [vp, ctypes.POINTER(ctypes.c_int), i, vp]
     L.frsdg_batch_ssprk_step_each.argtypes = [vp, ctypes.POINTER(ctypes.c_double), dp, vp]
     L.frsdg_filter_transpose.argtypes = [vp, dp, dp, vp]
     L.frsdg_mass.argtypes = [vp, dp, vp]
@@ -569,6 +570,13 @@ class Batch:
     def ssprk_run(self, u, nsteps):
         _check(self.lib.frsdg_batch_ssprk_run(self.h, _ptr(u), nsteps, _stream()))
         return u
+
+    def health(self, clear=False):
+        """Health flags of each instance's stage outputs since they were last
+        cleared (bit 1 non-finite, bit 2 rho <= 0, bit 4 internal energy <= 0)."""
+        f = (ctypes.c_int * self.count)()
+        _check(self.lib.frsdg_batch_health(self.h, f, int(clear), _stream()))
+        return list(f)
 
     def ssprk_step_each(self, u):
         _check(self.lib.frsdg_batch_ssprk_step_each(self.h, (ctypes.c_double * self.count)(*self.dts),

@@ -56,6 +56,7 @@ namespace
         const double* dt; // per-instance time step (device)
         double        a0, b, cfac;
         int           last;
+        int*          health; // per-instance health flags of the stored stage outputs (device)
     };
 
     // Work list of one launch: tile ranges of its instances.  Up to KMAX
@@ -182,7 +183,7 @@ namespace
             long long       blk;
             const int       k   = work_find(W, t, blk);
             const BatchInst I   = load_inst(tab, k);
-            const IOStage    io  = {A.us + I.off, A.u0 + I.off, A.out + I.off, A.a0, A.b, A.cfac * A.dt[k]};
+            const IOStage    io  = {A.us + I.off, A.u0 + I.off, A.out + I.off, A.a0, A.b, A.cfac * A.dt[k], A.health + k};
             const bool       flt = A.last && I.filt;
 #define FR_BT(NN)                                                                                            \
     case NN:                                                                                                 \
@@ -218,7 +219,7 @@ namespace
         long long       blk;
         const int       k  = work_find(W, xcd_tile(blockIdx.x, gridDim.x), blk);
         const BatchInst I  = load_inst(tab, k);
-        const IOStage   io = {A.us + I.off, A.u0 + I.off, A.out + I.off, A.a0, A.b, A.cfac * A.dt[k]};
+        const IOStage   io = {A.us + I.off, A.u0 + I.off, A.out + I.off, A.a0, A.b, A.cfac * A.dt[k], A.health + k};
         res2d_tile<N, IOStage, CURVED, FILT>(I.P, io, blk, TX, TY, I.filt, sm);
     }
 
@@ -233,7 +234,7 @@ namespace
             long long       blk;
             const int       k  = work_find(W, t, blk);
             const BatchInst I  = load_inst(tab, k);
-            const IOStage    io = {A.us + I.off, A.u0 + I.off, A.out + I.off, A.a0, A.b, A.cfac * A.dt[k]};
+            const IOStage    io = {A.us + I.off, A.u0 + I.off, A.out + I.off, A.a0, A.b, A.cfac * A.dt[k], A.health + k};
             res2d_tile<N, IOStage, CURVED, FILT>(I.P, io, blk, I.tx, I.ty, I.filt, sm);
             __syncthreads();
         }
@@ -249,6 +250,7 @@ struct frsdg_batch
     double*                 d_dt  = nullptr;
     double*                 w1    = nullptr;
     double*                 w2    = nullptr;
+    int*                    d_health = nullptr; // per instance: health flags of its stage outputs (frsdg_batch_health)
     long long               total_tiles = 0;
     long long               nd          = 0;
     int                     grid        = 0; // persistent workgroups (0: one workgroup per tile)
@@ -432,9 +434,9 @@ namespace
 #undef FR_BK
             return hip_rc(hipGetLastError());
         };
-        const StageArgs S1{u, u, b->w1, b->d_dt, 0.0, 1.0, 1.0, 0};
-        const StageArgs S2{b->w1, u, b->w2, b->d_dt, 0.75, 0.25, 1.0, 0};
-        const StageArgs S3{b->w2, u, u, b->d_dt, 1.0 / 3.0, 2.0 / 3.0, 1.0, 1};
+        const StageArgs S1{u, u, b->w1, b->d_dt, 0.0, 1.0, 1.0, 0, b->d_health};
+        const StageArgs S2{b->w1, u, b->w2, b->d_dt, 0.75, 0.25, 1.0, 0, b->d_health};
+        const StageArgs S3{b->w2, u, u, b->d_dt, 1.0 / 3.0, 2.0 / 3.0, 1.0, 1, b->d_health};
         int             rc;
         if (b->mode == 2)
         {
@@ -509,6 +511,8 @@ int frsdg_batch_create(const frsdg_desc* descs, int count, frsdg_batch** out)
     if (hipMalloc(&b->d_dt, sizeof(double) * count) != hipSuccess) return fail(FRSDG_ERR_HIP);
     if (hipMalloc(&b->w1, sizeof(double) * b->nd) != hipSuccess) return fail(FRSDG_ERR_HIP);
     if (hipMalloc(&b->w2, sizeof(double) * b->nd) != hipSuccess) return fail(FRSDG_ERR_HIP);
+    if (hipMalloc(&b->d_health, sizeof(int) * count) != hipSuccess) return fail(FRSDG_ERR_HIP);
+    if (hipMemset(b->d_health, 0, sizeof(int) * count) != hipSuccess) return fail(FRSDG_ERR_HIP);
     int dev = 0, ncu = 256;
     (void) hipGetDevice(&dev);
     (void) hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -527,7 +531,7 @@ int frsdg_batch_destroy(frsdg_batch* b)
     free_groups(b);
     for (auto* c : b->ctx) frsdg_destroy(c);
     for (void* p : {static_cast<void*>(b->d_tab), static_cast<void*>(b->d_dt), static_cast<void*>(b->w1),
-                    static_cast<void*>(b->w2)})
+                    static_cast<void*>(b->w2), static_cast<void*>(b->d_health)})
         if (p) (void) hipFree(p);
     delete b;
     return FRSDG_OK;
@@ -585,6 +589,20 @@ int frsdg_batch_ssprk_run(frsdg_batch* b, double* u, int nsteps, void* stream)
     if (!aligned(u)) return FRSDG_ERR_ALIGN;
     hipStream_t s = static_cast<hipStream_t>(stream);
     constexpr int CH = 8; // steps per captured graph
+    const int     K  = static_cast<int>(b->ctx.size());
+    // the health flags of this run (outside any capture, so a memset is fine);
+    // a run stops after the first graph of steps in which an instance raised a
+    // flag of its own stop mask (frsdg_set_stop_mask on frsdg_batch_instance)
+    FR_TRY(hipMemsetAsync(b->d_health, 0, sizeof(int) * K, s));
+    std::vector<int> flags(K);
+    auto healthy = [&]() -> int
+    {
+        FR_TRY(hipMemcpyAsync(flags.data(), b->d_health, sizeof(int) * K, hipMemcpyDeviceToHost, s));
+        FR_TRY(hipStreamSynchronize(s));
+        for (int k = 0; k < K; ++k)
+            if (flags[k] & b->ctx[k]->stop_mask) return FRSDG_ERR_NUMERIC;
+        return FRSDG_OK;
+    };
     if (!b->gexec || b->g_u != u)
     {
         if (b->gexec) (void) hipGraphExecDestroy(b->gexec);
@@ -608,12 +626,25 @@ int frsdg_batch_ssprk_run(frsdg_batch* b, double* u, int nsteps, void* stream)
         FR_TRY(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
         b->g_u = u;
     }
-    for (int k = 0; k < nsteps / CH; ++k) FR_TRY(hipGraphLaunch(b->gexec, s));
-    for (int k = 0; k < nsteps % CH; ++k)
+    int rc;
+    for (int k = 0; k < nsteps / CH; ++k)
     {
-        int rc = batch_step(b, u, s);
-        if (rc) return rc;
+        FR_TRY(hipGraphLaunch(b->gexec, s));
+        if ((rc = healthy())) return rc;
     }
+    for (int k = 0; k < nsteps % CH; ++k)
+        if ((rc = batch_step(b, u, s))) return rc;
+    return healthy();
+}
+
+int frsdg_batch_health(frsdg_batch* b, int* flags, int clear, void* stream)
+{
+    if (!b || !flags) return FRSDG_ERR_INVALID;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t bytes = sizeof(int) * b->ctx.size();
+    FR_TRY(hipMemcpyAsync(flags, b->d_health, bytes, hipMemcpyDeviceToHost, s));
+    if (clear) FR_TRY(hipMemsetAsync(b->d_health, 0, bytes, s));
+    FR_TRY(hipStreamSynchronize(s));
     return FRSDG_OK;
 }
 

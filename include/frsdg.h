@@ -405,9 +405,19 @@ int       frsdg_batch_set_tuning(frsdg_batch* b, int grid, int epb_div, int mode
  * the order, 1 that tile halved, 2 the other tile with the fewest rim faces per
  * element (n = 7: 2 x 2 elements instead of 5 x 1) */
 int       frsdg_batch_set_tile_shape(frsdg_batch* b, int n, int shape);
+/* Every stage output is checked in the stage kernel's store, as in
+ * frsdg_ssprk_step, into a flag word per instance (bit 1 non-finite, bit 2
+ * rho <= 0, bit 4 internal energy <= 0). */
 int       frsdg_batch_ssprk_step(frsdg_batch* b, double* u, void* stream);
-/* nsteps steps replayed from a captured hipGraph */
+/* nsteps steps replayed from a captured hipGraph; clears the flags when it
+ * starts and stops with FRSDG_ERR_NUMERIC after the first graph of steps in
+ * which an instance raised a flag of its own stop mask (frsdg_set_stop_mask on
+ * frsdg_batch_instance; default: non-finite states).  Synchronises the stream
+ * once per graph. */
 int       frsdg_batch_ssprk_run(frsdg_batch* b, double* u, int nsteps, void* stream);
+/* the flag words of the instances (host array of count ints), cleared when clear
+ * is set; synchronises the stream.  No counterpart in the reference. */
+int       frsdg_batch_health(frsdg_batch* b, int* flags, int clear, void* stream);
 /* the same step with one set of launches per instance (comparison point) */
 int       frsdg_batch_ssprk_step_each(frsdg_batch* b, const double* dt, double* u, void* stream);
 

@@ -96,3 +96,50 @@ def test_default_stop_mask_runs_through_non_positive_nodes(name):
     assert bool(ev.check_state(x) & fr.HEALTH_NONFINITE) == nonfinite
     with pytest.raises(fr.FrsdgError, match="invalid"):
         ev.set_stop_mask(8)
+
+
+@pytest.mark.parametrize("bad", ["negative_density", "nan"])
+def test_batched_sweep_flags_the_instance_with_a_bad_node(bad):
+    """Batched sweep: per-instance flag words; a bad node forced from the host
+    into one instance is flagged for that instance alone, and a batched run
+    stops after the graph in which an instance raised a flag of its stop mask
+    (non-finite by default; here the density bit too for that instance)."""
+    import torch
+
+    import fr_sdg_amd as fr
+    from fr_sdg_amd import configs as CF
+    cfgs = [CF.explosion(6, 2, vdw=False), CF.explosion(5, 4, vdw=True), CF.explosion(4, 6, vdw=False)]
+    b = fr.Batch(cfgs)
+    u = b.empty()
+    # the radial jump spread over a tanh layer wider than these coarse elements
+    for k, iv in enumerate(b.instances):
+        iv.impose_ic(fr.IC_EXPLOSION, CF.EXPLOSION_PRM + [0.6], out=b.view(u, k))
+    b.set_dt([1e-4] * len(cfgs))
+    good = u.clone()
+    b.ssprk_run(good, 10)                  # one graph of steps and plain steps
+    assert b.health() == [0, 0, 0]
+    # a bad density node in the middle of instance 1
+    h = u.cpu().numpy().copy()
+    mid = b.offsets[1] + ((b.offsets[2] - b.offsets[1]) // 8) * 4
+    h[mid] = -0.5 if bad == "negative_density" else math.nan
+    v = torch.from_numpy(h).cuda()
+    w = v.clone()
+    b.ssprk_step(w)
+    f = b.health(clear=True)
+    want = fr.HEALTH_DENSITY if bad == "negative_density" else fr.HEALTH_NONFINITE
+    assert f[1] & want and f[0] == 0 and f[2] == 0, f
+    assert b.health() == [0, 0, 0]
+    # the run stops on the instance's stop mask
+    b.instances[1].set_stop_mask(fr.HEALTH_NONFINITE | fr.HEALTH_DENSITY)
+    w = v.clone()
+    with pytest.raises(fr.FrsdgError):
+        b.ssprk_run(w, 16)
+    assert b.health()[1] & want
+    b.instances[1].set_stop_mask(fr.HEALTH_NONFINITE)
+    # the other instances step as in the healthy run (instances do not interact)
+    g = u.clone()
+    b.ssprk_step(g)
+    w = v.clone()
+    b.ssprk_step(w)
+    for k in (0, 2):
+        assert torch.equal(b.view(w, k), b.view(g, k)), k
