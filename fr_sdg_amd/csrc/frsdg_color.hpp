@@ -68,6 +68,10 @@ namespace frsdg
         int* lpos = nullptr; // [E][2][CMAXN] per level position: lower (slot, element) pairs, slot -1 ends
         int* upos = nullptr; // [E][2][CMAXN] per level position: upper (slot, element) pairs
         int* info = nullptr; // zero-pivot flag
+        int* df   = nullptr; // dependence-ordered sweeps: [0] [1] ticket counters (forward, backward),
+                             // [2] wait-limit flag, [3, 3 + 2E) chunks stored per row (forward, backward)
+        double* ydf = nullptr; // their forward / backward results, rows padded to df_stride(nd)
+        double* zdf = nullptr;
         double* P    = nullptr; // E*S*nd*nd for the Newton preconditioner (allocated on first use)
         double* Dinv = nullptr; // E*2*nd*nd: L_II^-1, U_II^-1 (with P)
         float*  P32    = nullptr; // single-precision copies of P and Dinv after the factorisation
@@ -952,6 +956,200 @@ namespace frsdg
 #pragma unroll
             for (int k = 0; k < SWG; ++k) sum += red[k][rl];
             out[I * nd + r] = sum;
+        }
+    }
+
+    // One sweep direction of k_ilu_sweep in a single launch over every block
+    // row, ordered by data dependence instead of by level launches.  A
+    // workgroup draws a ticket from *ctr when it starts; ticket t is chunk
+    // t % RCH of the block row at level-order position t / RCH (from the end
+    // for the backward sweep).  It loads its first SWB blocks and its own
+    // input, then waits until every chunk of each neighbour row it reads has
+    // been stored (done[el] == RCH), sums exactly as k_ilu_sweep does (same
+    // per-thread order, so the same results) and counts its own chunk into
+    // done[I].  A neighbour row sits at an earlier level, so its chunks hold
+    // earlier tickets: their workgroups were already running when this one
+    // drew, and every wait ends without a grid barrier or co-residency
+    // assumption.  As a guard, a wait longer than DF_SPIN polls, or one that
+    // finds *err set, sets *err and gives up (the host reports it); every
+    // workgroup therefore exits.  ctr, done and err start at zero (reset by
+    // k_ilu_df_reset before each apply).
+    //
+    // Visibility without cache maintenance: out rows are padded to whole
+    // 128-byte lines (stride NDP) and stored at device scope (written
+    // through to the point of coherence; the wave waits for the stores before
+    // it counts the chunk), and a row is read only after all its chunks were
+    // counted, so no cache line of a row is ever read before the row is
+    // final and plain loads see the stored values.  Release / acquire fences
+    // per workgroup (a whole-L2 write-back and invalidation each) made this
+    // kernel 4x slower than the level launches.
+    constexpr int DF_SPIN = 1 << 22;
+    __host__ __device__ constexpr int df_stride(int nd) { return (nd + 15) / 16 * 16; }
+    __global__ void k_ilu_df_reset(int n, int* __restrict__ w)
+    {
+        const int i = blockIdx.x * blockDim.x + threadIdx.x;
+        if (i < n) w[i] = 0;
+    }
+    template <int ND>
+    __global__ void k_ilu_df_unpad(long long E, const double* __restrict__ zp, double* __restrict__ z)
+    {
+        const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+        if (i >= E * ND) return;
+        const long long e = i / ND;
+        z[i]              = zp[e * df_stride(ND) + (i - e * ND)];
+    }
+    template <bool FWD, int ND, class T>
+    __global__ __launch_bounds__(256) void k_ilu_sweep_df(int S, int E, const int* __restrict__ pos_tab,
+                                                          const int* __restrict__ lvl, const T* __restrict__ P,
+                                                          const T* __restrict__ Dinv, const double* __restrict__ in,
+                                                          double* out, int* ctr, int* done, int* err)
+    {
+        static_assert(ND > 0 && ND <= 144, "compile-time block size up to 144");
+        constexpr int NC = (ND + SWG - 1) / SWG, RCH = (ND + SWR - 1) / SWR, SWB = 5;
+        constexpr int NDP = df_stride(ND), IS = FWD ? ND : NDP; // out / in row strides
+        constexpr long long bsz = static_cast<long long>(ND) * ND;
+        __shared__ double red[SWG][SWR + 1];
+        __shared__ int    tk;
+        if (threadIdx.x == 0) tk = atomicAdd(ctr, 1);
+        __syncthreads();
+        const int tkt = tk;
+        if (tkt >= E * RCH) return; // uniform; not reached with E * RCH workgroups
+        const int       p   = tkt / RCH;
+        const long long pos = FWD ? p : E - 1 - p;
+        const long long I   = lvl[pos];
+        const int       rl  = threadIdx.x % SWR, g = threadIdx.x / SWR;
+        const int       r   = (tkt - p * RCH) * SWR + rl;
+        int sl[CMAXN], el[CMAXN];
+#pragma unroll
+        for (int q = 0; q < CMAXN; ++q)
+        {
+            sl[q] = pos_tab[pos * 2 * CMAXN + q];
+            el[q] = pos_tab[pos * 2 * CMAXN + CMAXN + q];
+        }
+        int nb = 1;
+#pragma unroll
+        for (int q = 0; q < CMAXN; ++q) nb += sl[q] >= 0 ? 1 : 0;
+        const int rr = r < ND ? r : ND - 1;
+        auto blk = [&](int qq) -> const T*
+        { return qq == 0 ? Dinv + I * 2 * bsz + (FWD ? 0 : bsz) : P + (I * S + sl[qq - 1]) * bsz; };
+        // the first batch of blocks and the row's own input: nothing to wait for
+        double bv[SWB][NC], vv[SWB][NC];
+#pragma unroll
+        for (int t = 0; t < SWB; ++t)
+        {
+            if (t < nb)
+            {
+                const T* B = blk(t);
+#pragma unroll
+                for (int i = 0; i < NC; ++i)
+                {
+                    const int c = g + SWG * i, cc = c < ND ? c : ND - 1;
+                    bv[t][i]    = static_cast<double>(B[rr + cc * ND]);
+                }
+            }
+            else
+#pragma unroll
+                for (int i = 0; i < NC; ++i) bv[t][i] = 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < NC; ++i)
+        {
+            const int    c = g + SWG * i, cc = c < ND ? c : ND - 1;
+            const double x = in[I * IS + cc];
+            vv[0][i]       = c < ND ? x : 0.0;
+        }
+        // the diagonal block's part first (it is the first term of the sum)
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) acc = ::fma(bv[0][i], vv[0][i], acc);
+        if (threadIdx.x == 0)
+        {
+            for (int q = 0; q + 1 < nb; ++q)
+            {
+                const int* f = done + el[q];
+                int        n = 0;
+                while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < RCH)
+                {
+                    if (++n > DF_SPIN || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    {
+                        atomicOr(err, 1);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int b0 = 0; b0 <= CMAXN; b0 += SWB)
+        {
+            if (b0 >= nb) break; // uniform per workgroup
+#pragma unroll
+            for (int t = 0; t < SWB; ++t)
+            {
+                const int qq = b0 + t;
+                if (qq > CMAXN) break;
+                if (b0 > 0)
+                {
+                    if (qq < nb)
+                    {
+                        const T* B = blk(qq);
+#pragma unroll
+                        for (int i = 0; i < NC; ++i)
+                        {
+                            const int c = g + SWG * i, cc = c < ND ? c : ND - 1;
+                            bv[t][i]    = static_cast<double>(B[rr + cc * ND]);
+                        }
+                    }
+                    else
+#pragma unroll
+                        for (int i = 0; i < NC; ++i) bv[t][i] = 0.0;
+                }
+                if (qq == 0) continue; // the row's own input, loaded above
+                if (qq < nb)
+                {
+                    const double* V = out + static_cast<long long>(el[qq - 1]) * NDP;
+#pragma unroll
+                    for (int i = 0; i < NC; ++i)
+                    {
+                        const int    c = g + SWG * i, cc = c < ND ? c : ND - 1;
+                        const double x = V[cc];
+                        vv[t][i]       = c < ND ? x : 0.0;
+                    }
+                }
+                else
+#pragma unroll
+                    for (int i = 0; i < NC; ++i) vv[t][i] = 0.0;
+            }
+#pragma unroll
+            for (int t = 0; t < SWB; ++t)
+            {
+                const int qq = b0 + t;
+                if (qq > CMAXN) break;
+                if (qq == 0) continue;
+                double part = 0.0;
+#pragma unroll
+                for (int i = 0; i < NC; ++i) part = ::fma(bv[t][i], vv[t][i], part);
+                if (qq < nb) acc -= part;
+            }
+        }
+        red[g][rl] = acc;
+        __syncthreads();
+        if (g == 0 && r < ND)
+        {
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < SWG; ++k) sum += red[k][rl];
+            __hip_atomic_store(out + I * NDP + r, sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // the stores above come from lanes 0..15 of wave 0: that wave waits
+        // until they have completed, then counts the chunk as stored
+        if (threadIdx.x < 64)
+        {
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_s_waitcnt(0);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            if (threadIdx.x == 0) __hip_atomic_fetch_add(done + I, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 

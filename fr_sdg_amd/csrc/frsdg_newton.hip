@@ -68,9 +68,9 @@ namespace frsdg
             if (e) (void) hipEventDestroy(e);
         if (frsdg::ColorPlan* cp = s->cp)
         {
-            for (int* p : {cp->nbr, cp->cls, cp->cslot, cp->low, cp->upd, cp->upp, cp->lvl, cp->lpos, cp->upos, cp->info})
+            for (int* p : {cp->nbr, cp->cls, cp->cslot, cp->low, cp->upd, cp->upp, cp->lvl, cp->lpos, cp->upos, cp->info, cp->df})
                 if (p) (void) hipFree(p);
-            for (double* p : {cp->P, cp->Dinv, cp->y, cp->v, cp->f0, cp->f1, cp->T, cp->t})
+            for (double* p : {cp->P, cp->Dinv, cp->y, cp->v, cp->f0, cp->f1, cp->T, cp->t, cp->ydf, cp->zdf})
                 if (p) (void) hipFree(p);
             for (float* p : {cp->P32, cp->Dinv32})
                 if (p) (void) hipFree(p);
@@ -386,6 +386,9 @@ namespace
             (rc = up(cp->upos, upos)))
             return rc;
         FR_TRY(hipMalloc(&cp->info, sizeof(int)));
+        FR_TRY(hipMalloc(&cp->df, sizeof(int) * (3 + 2 * cp->E)));
+        FR_TRY(hipMemset(cp->df, 0, sizeof(int) * (3 + 2 * cp->E)));
+        for (double** p : {&cp->ydf, &cp->zdf}) FR_TRY(hipMalloc(p, sizeof(double) * cp->E * frsdg::df_stride(nd)));
         const long long bsz = static_cast<long long>(nd) * nd;
         int width = 1;
         for (size_t l = 0; l + 1 < cp->lvl_ptr.size(); ++l) width = std::max(width, cp->lvl_ptr[l + 1] - cp->lvl_ptr[l]);
@@ -452,13 +455,33 @@ namespace
         return info ? FRSDG_ERR_NUMERIC : FRSDG_OK;
     }
 
-    // z = (L U)^-1 x: forward sweep over the levels, then backward (z may alias x).
+    // z = (L U)^-1 x: forward sweep over the levels, then backward (z may alias x);
+    // compile-time block sizes run each direction as one dependence-ordered launch
     template <int ND, class T = double>
     void ilu_sweeps(const frsdg::ColorPlan& cp, hipStream_t st, const T* P, const T* Dinv, const double* x,
                     double* z)
     {
         const int nlev = static_cast<int>(cp.lvl_ptr.size()) - 1;
         const unsigned rch = static_cast<unsigned>((cp.nd + frsdg::SWR - 1) / frsdg::SWR);
+        if constexpr (ND > 0 && ND <= 144 && std::is_same_v<T, double>)
+        {
+            // one dependence-ordered launch per direction (k_ilu_sweep_df; the
+            // single-precision factors keep the level launches, 6.5 vs 7.9 ms,
+            // and so do blocks above 144, whose registers allow one wave per
+            // SIMD; at 144 two, and a cap at three spills and runs 2.4x slower)
+            const int      E  = static_cast<int>(cp.E);
+            const int      nw = 3 + 2 * E;
+            const unsigned wg = static_cast<unsigned>(E) * rch;
+            hipLaunchKernelGGL(frsdg::k_ilu_df_reset, dim3(static_cast<unsigned>((nw + 255) / 256)), dim3(256), 0, st,
+                               nw, cp.df);
+            hipLaunchKernelGGL((frsdg::k_ilu_sweep_df<true, ND, T>), dim3(wg), dim3(256), 0, st, cp.S, E, cp.lpos,
+                               cp.lvl, P, Dinv, x, cp.ydf, cp.df, cp.df + 3, cp.df + 2);
+            hipLaunchKernelGGL((frsdg::k_ilu_sweep_df<false, ND, T>), dim3(wg), dim3(256), 0, st, cp.S, E, cp.upos,
+                               cp.lvl, P, Dinv, cp.ydf, cp.zdf, cp.df + 1, cp.df + 3 + E, cp.df + 2);
+            hipLaunchKernelGGL((frsdg::k_ilu_df_unpad<ND>), dim3(static_cast<unsigned>((cp.E * ND + 255) / 256)), dim3(256),
+                               0, st, cp.E, cp.zdf, z);
+            return;
+        }
         for (int l = 0; l < nlev; ++l)
             hipLaunchKernelGGL((frsdg::k_ilu_sweep<true, ND, T>),
                                dim3(static_cast<unsigned>(cp.lvl_ptr[l + 1] - cp.lvl_ptr[l]), rch), dim3(256), 0, st, cp.S,
@@ -536,6 +559,15 @@ namespace
             default: ilu_sweeps<0, T>(cp, st, P, Dinv, x, z); break;
         }
         return hip_rc(hipGetLastError());
+    }
+    // FRSDG_ERR_HIP if a dependence-ordered sweep gave up a wait since the
+    // last apply's reset (a broken wait protocol, never expected); synchronises
+    int ilu_df_status(const frsdg::ColorPlan& cp, hipStream_t st)
+    {
+        int e = 0;
+        FR_TRY(hipMemcpyAsync(&e, cp.df + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+        FR_TRY(hipStreamSynchronize(st));
+        return e ? FRSDG_ERR_HIP : FRSDG_OK;
     }
 
     struct Ctx
@@ -892,13 +924,14 @@ namespace
                 FR_TRY(hipStreamSynchronize(st));
                 if (done || its >= o->ksp_max_it) break;
             }
-            if (left || flex) return FRSDG_OK;
+            if (left || flex) return pc_status();
             // solution = M^-1 x
             int rc = prec(x, s->z);
             if (rc) return rc;
             FR_TRY(hipMemcpyAsync(x, s->z, sizeof(double) * N, hipMemcpyDeviceToDevice, st));
-            return FRSDG_OK;
+            return pc_status();
         }
+        int pc_status() const { return o->precond >= 3 && !transpose ? ilu_df_status(*s->cp, st) : FRSDG_OK; }
     };
 
     int ensure_ws(frsdg_ctx* c, int m)
@@ -1303,7 +1336,8 @@ extern "C" int frsdg_ilu0_apply(frsdg_ctx* c, const double* P, const double* Din
     if (!aligned(P) || !aligned(Dinv) || !aligned(x) || !aligned(z)) return FRSDG_ERR_ALIGN;
     int rc = ensure_ws(c, c->ws ? c->ws->m : 1);
     if (rc || (rc = color_plan_ensure(c, c->ws))) return rc;
-    return ilu_apply(c->ws, static_cast<hipStream_t>(stream), P, Dinv, x, z);
+    if ((rc = ilu_apply(c->ws, static_cast<hipStream_t>(stream), P, Dinv, x, z))) return rc;
+    return ilu_df_status(*c->ws->cp, static_cast<hipStream_t>(stream));
 }
 
 // (sigma I + dR/du(u)) x = b by the device GMRES, with the residual history.

@@ -323,7 +323,8 @@ int frsdg_color_jacobian(frsdg_ctx* ctx, double t, const double* u, const double
 int frsdg_ilu0_factor(frsdg_ctx* ctx, double* P, double* Dinv, void* stream);
 
 /* z = (L U)^-1 x with the factors of frsdg_ilu0_factor (PCApply of ILU);
- * z may alias x. */
+ * z may alias x.  Synchronises the stream (it checks that the
+ * dependence-ordered sweeps of 2D blocks completed every wait). */
 int frsdg_ilu0_apply(frsdg_ctx* ctx, const double* P, const double* Dinv, const double* x, double* z,
                      void* stream);
 

@@ -99,3 +99,7 @@ def test_color_matrix_and_ilu_at_full_size():
         # the whole factors' (1.4e-8); a wrong factor or sweep gives O(1)
         assert back <= 1e-8
         assert fwd <= 1e-6
+        # repeated applies give the same bits (each sum keeps its order whatever
+        # the order in which the rows' workgroups run and wait)
+        for _ in range(3):
+            assert torch.equal(solve(P, Dinv, x.reshape(-1)).reshape(E, nd), z)
